@@ -1,0 +1,152 @@
+/* C ABI of the MI355X successive-orders-of-scattering engine (libsosfuse2_hip.so).
+ *
+ * Plain pointers and sizes only. Every pointer named *_dev is device memory owned by the caller (the Python host
+ * allocates it through PyTorch); the library allocates nothing except where a function says so. Every call is
+ * asynchronous on the HIP stream it is given (NULL = the default stream) and returns 0 or a negative SOS_E_* code;
+ * sos_last_error() gives the message of the last failure on the calling thread.
+ *
+ * Reference interfaces these entry points replace (paths relative to the reference repository):
+ *   sos_transport      Kernel.launch(bundle, s, field, first, phase, boundary) -> (moments, flux, query)
+ *                      src/sosfuse2/solve/source_solver.py:2428-2456 (device transport of sos_fused,
+ *                      src/sosfuse2/backends/kernel.cu:4138-4266), and the NumPy mirror
+ *                      src/sosfuse2/backends/numpy_reference.py:461-643
+ *   sos_project        the projection of kernel.cu fl_project :4323 / sh_project :4900 / sos_slim_proj :5702,
+ *                      NumPy numpy_reference.py:683-711
+ *   sos_compose        Solver.apply composition, src/sosfuse2/solve/source_iteration.py:176-212
+ *   sos_make_plane     combined SH plane, src/sosfuse2/backends/sh_layout.py:135-152
+ *   sos_source_metrics _SOURCE_METRICS kernel, src/sosfuse2/solve/source_iteration.py:52-97
+ *   sos_anderson_*     Anderson residual, Gram products and mixing, source_iteration.py:432-443,618-635,25-33
+ */
+#ifndef SOSFUSE2_HIP_H
+#define SOSFUSE2_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SOS_OK 0
+#define SOS_E_ARGUMENT (-1)
+#define SOS_E_LAUNCH (-2)
+#define SOS_E_UNSUPPORTED (-3)
+#define SOS_E_DEVICE (-4)
+
+/* Per-lane optical constants: float [SOS_LANE_CONSTANTS][lanes], row order below. */
+#define SOS_LANE_EXT0 0 /* air (Rayleigh) extinction per km at unit profile */
+#define SOS_LANE_EXT1 1 /* tropospheric aerosol */
+#define SOS_LANE_EXT2 2 /* ozone */
+#define SOS_LANE_EXT3 3 /* NO2 */
+#define SOS_LANE_EXT4 4 /* O4 */
+#define SOS_LANE_ASCA 5 /* aerosol scattering */
+#define SOS_LANE_G 6    /* aerosol HG asymmetry */
+#define SOS_LANE_MIX 7  /* Rayleigh depolarisation factor (1 - delta) / (1 + delta / 2) */
+#define SOS_LANE_ALBEDO 8
+#define SOS_LANE_SEXT 9  /* sulfate extinction */
+#define SOS_LANE_SSCA 10 /* sulfate scattering */
+#define SOS_LANE_SG 11   /* sulfate HG asymmetry */
+#define SOS_LANE_CONSTANTS 12
+
+/* Transport flags */
+#define SOS_FIRST 1     /* first light (single scattering of the direct and ground-reflected sun) */
+#define SOS_BOUNDARY 2  /* add the attenuated ground boundary term at ray ends that hit the ground */
+#define SOS_MIXED 4     /* SH plane stored as FP16 (mixed precision) */
+
+typedef struct sos_bundle {
+    int64_t groups;               /* destination cells or query groups */
+    int64_t rays;
+    const int64_t *group_off_dev; /* [groups + 1] rays of each group */
+    const double *starts_dev;     /* [groups][3] km */
+    const float *targets_dev;     /* [groups][9] target frame, column i = basis vector i (ex, ey, up) */
+    const int32_t *ray_group_dev; /* [rays] */
+    const double *dirs_dev;       /* [rays][3] unit directions (global frame) */
+    const float *dloc_dev;        /* [rays][3] directions in the target frame */
+    const float *weights_dev;     /* [rays] solid-angle weights */
+    const int64_t *edge_off_dev;  /* [rays + 1] */
+    const double *edges_dev;      /* [edges] path coordinates (km) */
+    const uint16_t *lower_dev;    /* [rays] bit i: segment i grows in radius */
+    const int32_t *end_j_dev;     /* [rays] chi interval of the ray end */
+    const float *end_f_dev;       /* [rays] fraction in that interval */
+    const float *end_col_dev;     /* [rays][6] slant columns to the ground (5 components + sulfate) */
+    const uint8_t *hits_dev;      /* [rays] 1 when the ray ends on the ground */
+} sos_bundle;
+
+typedef struct sos_grid {
+    int32_t nh, nc;            /* heights, solar zenith nodes; cells = nh * nc */
+    int32_t L;                 /* SH order: 8, 16, 24 or 32 */
+    int32_t lanes;             /* lanes per ray: 1, 16 or 32 (>= active wavelengths) */
+    int32_t nq;                /* Gauss-Legendre nodes per segment */
+    int32_t strato;            /* sulfate layer present */
+    float top_km;
+    float air_H, aer_H, o3_c, o3_s, no2_c, no2_s, st_c, st_s; /* profile shapes (km) */
+    const double *gl_u_dev;    /* [nq] nodes on [0, 1] */
+    const double *gl_w_dev;    /* [nq] weights */
+    const float *h_dev;        /* [nh] */
+    const float *chi_dev;      /* [nc] rad */
+    const float *hr_dev;       /* [nh] air profile at the grid heights */
+    const float *lane_dev;     /* [SOS_LANE_CONSTANTS][lanes] */
+    int32_t sun_nh, sun_nx;
+    const float *sun_h_dev;    /* [sun_nh] */
+    const float *sun_col_dev;  /* [sun_nh][sun_nx][6] */
+    const int64_t *goff_dev;   /* [cells + 1] ground pools per cell */
+    const float *gdir_dev;     /* [pools][3] cell-frame directions */
+    const float *gval_dev;     /* [pools][lanes] */
+    const float *gm_dev;       /* [cells][4][lanes] Rayleigh tensor of the ground-reflected sun */
+} sos_grid;
+
+/* Radiance tensor of every ray: out_dev [rays][lanes][pairs] (pairs 4: xx yy zz xz; 6: + xy yz), FP32.
+ * field_dev: source [cells][rows][lanes] (rows 0..3 read); plane_dev: combined SH plane [cells][K][lanes]
+ * (FP32, or FP16 with SOS_MIXED); boundary_dev: [nc][lanes] ground radiance (with SOS_BOUNDARY). */
+int sos_transport(const sos_grid *grid, const sos_bundle *bundle, int32_t flags, int32_t pairs, int32_t rows,
+                  const float *field_dev, const void *plane_dev, const float *boundary_dev, float *out_dev,
+                  void *stream);
+
+/* Moments [groups][4 + K][lanes] and ground flux [groups][lanes] of the per-ray radiances rad_dev
+ * [rays][lanes][4] (the bundle's groups are the destination cells). ground_only writes the flux only. */
+int sos_project(const sos_grid *grid, const sos_bundle *bundle, const float *rad_dev, int32_t ground_only,
+                float *moments_dev, float *flux_dev, void *stream);
+
+/* source[cells][rows][lanes] = factor[rows][lanes] * (moments + first)[cells][row'][lanes] with row' the moment of
+ * each row; values below FLT_MIN in magnitude become 0. first_dev may be NULL. */
+int sos_compose(const sos_grid *grid, int32_t rows, const float *moments_dev, const float *first_dev,
+                const float *factor_dev, float *source_dev, void *stream);
+
+/* plane[cells][K][lanes] = source[:, 10:10+K] * ha[ih] + source[:, 10+K:10+2K] * sd[ih] (FP32 or FP16). */
+int sos_make_plane(const sos_grid *grid, int32_t rows, const float *source_dev, const float *ha_dev,
+                   const float *sd_dev, int32_t half, void *plane_dev, void *stream);
+
+/* Per lane: count of non-finite g, max |g|, max |g - x|; frozen lanes read x for g. out_dev [3][lanes] float
+ * (count as float). */
+int sos_source_metrics(int64_t rows, int32_t lanes, const float *g_dev, const float *x_dev, uint32_t frozen,
+                       float *out_dev, void *stream);
+
+/* Anderson residual of every lane (source_iteration.py:432-443):
+ * f[lane][n], n = cells*rows + nobs, f = (g - x) / max(||g_cell||, 1e-8 max_cells ||g_cell||) over the source rows,
+ * then obs[lane][nobs] (the scaled observer change). work_dev: float [cells*lanes + lanes] scratch. */
+int sos_anderson_residual(int64_t cells, int32_t rows, int32_t lanes, const float *g_dev, const float *x_dev,
+                          const float *obs_dev, int32_t nobs, float *work_dev, float *f_dev, void *stream);
+
+/* Row and column `slot` of the per-lane Gram matrices gram_dev [lanes][depth][depth] (FP64):
+ * gram[lane][slot][j] = gram[lane][j][slot] = sum_i f_ring[slot][lane][i] f_ring[j][lane][i] for every j with bit j
+ * of filled set. f_ring: [depth][lanes][n]. work_dev: double [1024 * lanes * depth] scratch. Fixed-order sums. */
+int sos_anderson_gram(int64_t n, int32_t lanes, int32_t depth, int32_t slot, uint32_t filled, const float *ring_dev,
+                      double *work_dev, double *gram_dev, void *stream);
+
+/* Per lane, FP64 on the device: the regularised Gram system of the m slots order_dev[0..m-1] (oldest first),
+ * G += max(tr G 1e-10, 1e-30) I, w = G^-1 1 / sum, accepted when finite and max |w| < 8 (source_iteration.py:25-33).
+ * w_dev [lanes][depth] float; w[lane][0] = NaN marks a lane that keeps g. Lanes with bit set in frozen are skipped. */
+int sos_anderson_solve(int32_t lanes, int32_t depth, int32_t m, const int32_t *order_dev, const double *gram_dev,
+                       uint32_t frozen, float *w_dev, void *stream);
+
+/* x[cells][rows][lanes] = sum_j w[lane][j] g_ring[order_j][cells][rows][lanes] for lanes with accepted weights;
+ * other lanes are left as they are. g_ring: [depth][cells][rows][lanes]. */
+int sos_anderson_mix(int64_t cells, int32_t rows, int32_t lanes, int32_t depth, int32_t m, const float *ring_dev,
+                     const int32_t *order_dev, const float *w_dev, float *x_dev, void *stream);
+
+const char *sos_last_error(void);
+int sos_device_info(int32_t *cus, int32_t *xcds, char *arch, int32_t arch_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

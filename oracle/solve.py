@@ -1,0 +1,225 @@
+"""Oracle source iteration, observer queries and table: a NumPy restatement of the reference's solver loop.
+
+TEST INFRASTRUCTURE ONLY (checker and timed CPU baseline); the product package never imports it.
+
+Follows:
+- `evaluate` = the NumPy branch of `src/sosfuse2/solve/ray_geometry.py:443-451` (per cell transport + projection);
+- `initialize` / `apply` = `src/sosfuse2/solve/source_iteration.py:123-216` (first light once; ground pass, boundary,
+  warm pass, composition with the eigenvalues, FP32 flush of |source| < FLT_MIN);
+- `solve` = `source_iteration.py:309-663`, list-history branch :618-635 (per-wavelength Anderson, FP32 Gram, FP64
+  regularised solve `_anderson_weights` :25-33, stop on max(observer, source) relative change from evaluation 2);
+- `query` / `boxes` = `src/sosfuse2/solve/engine_core.py:208-238`; `make_table` = `src/sosfuse2/outputs/radiance_table.py:167-219`.
+
+Inputs (grid, optics, packs, constants) come from the product's host preparation, so the oracle checks the transport
+and the iteration, not the input geometry.
+"""
+
+import math
+import time
+
+import numpy as np
+
+from .arith import TRACE_MIN32, count
+from .transport import PAIRS4, PAIRS6, project, transport
+
+
+def anderson_weights(products):
+    """Regularised Gram solve, normalised weights, guard |w| < 8 (source_iteration.py:25-33)."""
+    gram = np.array(products, dtype=np.float64, copy=True)
+    gram[np.diag_indices_from(gram)] += max(float(np.trace(gram)) * 1e-10, 1e-30)
+    try:
+        weights = np.linalg.solve(gram, np.ones(len(gram)))
+    except np.linalg.LinAlgError:
+        return None
+    weights /= weights.sum()
+    if np.isfinite(weights).all() and np.max(np.abs(weights)) < 8:
+        return weights.astype(np.float32)
+    return None
+
+
+def stokes(comp, view, F=np.float64):
+    """I, Q, U [..., 3] from the six tensor components (xx, yy, zz, xz, xy, yz) in the observer frame.
+    e_h = (-sin az, cos az, 0) and e_v = (-sin el cos az, -sin el sin az, cos el) span the plane normal to the view;
+    I = tr P, Q = e_v P e_v - e_h P e_h, U = 2 e_v P e_h."""
+    el, az = view[:, 0], view[:, 1]
+    eh = np.stack([-np.sin(az), np.cos(az), np.zeros_like(az)], -1)
+    ev = np.stack([-np.sin(el) * np.cos(az), -np.sin(el) * np.sin(az), np.cos(el)], -1)
+    xx, yy, zz, xz, xy, yz = (comp[..., k].astype('f8') for k in range(6))
+
+    def quad(a, b):
+        a = a[:, None, :]
+        b = b[:, None, :]
+        return (a[..., 0] * b[..., 0] * xx + a[..., 1] * b[..., 1] * yy + a[..., 2] * b[..., 2] * zz
+                + (a[..., 0] * b[..., 2] + a[..., 2] * b[..., 0]) * xz
+                + (a[..., 0] * b[..., 1] + a[..., 1] * b[..., 0]) * xy
+                + (a[..., 1] * b[..., 2] + a[..., 2] * b[..., 1]) * yz)
+
+    I = (xx + yy) + zz
+    return np.stack([I, quad(ev, ev) - quad(eh, eh), 2 * quad(ev, eh)], -1)
+
+
+class OracleSolver:
+    """CPU solve of one chunk. dtype float32 = the FP32-FTZ restatement; float64 = the FP64 twin."""
+
+    def __init__(self, ctx, dtype=np.float32, depth=4, packs=None):
+        from sosfuse2_amd import geometry  # input preparation only
+
+        self.ctx = ctx
+        self.F = np.dtype(dtype).type
+        self.c = ctx.constants(ctx.geo.path_n, dtype)
+        self.cq = ctx.constants(ctx.geo.query_n, dtype)
+        self.depth = depth
+        self.g = geometry
+        if packs is None:
+            packs = [geometry.with_end_columns(ctx.state, geometry.cell_pack(ctx.geo, i), ctx.geo.domain)
+                     for i in range(ctx.geo.ns)]
+        self.packs = packs
+        self.first_moments = None
+        self.first_flux = None
+        self.stats = dict(samples=0, angular_pairs=0)
+        self.timings = dict(apply_seconds=[])
+
+    def _cat(self, count_):
+        key = f'_cat{count_}'
+        if not hasattr(self, key):
+            sel = self.packs[:count_]
+            cat = self.g.concat_packs(sel)
+            cat['end_columns'] = np.concatenate([p['end_columns'] for p in sel])
+            off = np.r_[0, np.cumsum([len(p['d']) for p in sel])]
+            setattr(self, key, (cat, off))
+        return getattr(self, key)
+
+    def evaluate(self, field, first=False, phase=0, ground_only=False, boundary=None):
+        count_ = self.ctx.geo.nc if ground_only else self.ctx.geo.ns
+        cat, off = self._cat(count_)
+        value = transport(cat, self.c, field, first, boundary, PAIRS4, self.stats)
+        K = count(self.c['L'])
+        moments = np.zeros((count_, 4 + K, self.c['W']), 'f8')
+        flux = np.zeros((count_, self.c['W']), 'f8')
+        for i in range(count_):
+            sub = {k: cat[k][off[i] : off[i + 1]] for k in ('d', 'dloc', 'weight')}
+            moments[i], flux[i] = project(sub, value[off[i] : off[i + 1]], self.c, ground_only=phase == 1)
+        return moments, flux
+
+    def initialize(self):
+        if self.first_moments is None:
+            zero = np.zeros((self.ctx.geo.ns, self.ctx.rows, self.c['W']), self.F)
+            self.first_moments, self.first_flux = self.evaluate(zero, first=True, phase=0)
+
+    def compose(self, moment):
+        """Source rows from incident moments (source_iteration.py:176-212)."""
+        eig = self.ctx.eigen().astype(self.F).astype('f8')
+        K = count(self.c['L'])
+        out = np.empty((moment.shape[0], self.ctx.rows, self.c['W']), self.F)
+        out[:, :4] = (moment[:, :4] * eig[None, :4]).astype(self.F)
+        out[:, 4:10] = (moment[:, 4:10] * eig[None, 4:10]).astype(self.F)
+        out[:, 10 : 10 + K] = (moment[:, 4:] * eig[None, 10 : 10 + K]).astype(self.F)
+        if self.c['strato']:
+            out[:, 10 + K :] = (moment[:, 4:] * eig[None, 10 + K :]).astype(self.F)
+        if self.F is np.float32:
+            out[np.abs(out) < TRACE_MIN32] = 0
+        return out
+
+    def apply(self, field):
+        self.initialize()
+        t = time.perf_counter()
+        _, flux = self.evaluate(field, phase=1, ground_only=True)
+        boundary = flux + self.first_flux[: self.ctx.geo.nc]
+        moment, _ = self.evaluate(field, phase=0, boundary=boundary)
+        moment += self.first_moments
+        out = self.compose(moment)
+        self.timings['apply_seconds'].append(time.perf_counter() - t)
+        return out
+
+    # ----------------------------------------------------------------------------------------------- queries
+    def query_raw(self, h_s, el, az, field, first, pairs=PAIRS4):
+        packs = self.ctx.query_packs(h_s, el, az)
+        cat = self.g.concat_packs(packs)
+        return transport(cat, self.cq, field, first, None, pairs, self.stats), cat
+
+    def query(self, h_s, el, az, field, first=False):
+        """Radiance I [n, W] times the solar irradiance (engine_core.py:208-226)."""
+        v, _ = self.query_raw(h_s, el, az, field, first)
+        return ((v[..., 0] + v[..., 1]) + v[..., 2]).astype('f8') * self.ctx.solar[None, :]
+
+    def boxes(self, field):
+        """Observer monitor [n_solar, 7, W] (engine_core.py:228-238)."""
+        el, az, ww = self.ctx.box_directions()
+        out = []
+        for h in self.ctx.monitors:
+            v = self.query(h, el, az, field, False).reshape(len(self.ctx.directions), 16, self.c['W'])
+            out.append(np.einsum('dnw,n->dw', v, ww))
+        return np.stack(out)
+
+    def solve(self, max_eval=40, tolerance=1e-5, log=None):
+        W = self.c['W']
+        x = np.zeros((self.ctx.geo.ns, self.ctx.rows, W), self.F)
+        gs = [[] for _ in range(W)]
+        fs = [[] for _ in range(W)]
+        done = np.zeros(W, bool)
+        history = []
+        t0 = time.perf_counter()
+        for iteration in range(1, max_eval + 1):
+            old = self.boxes(x)
+            g = self.apply(x)
+            bad = int(np.count_nonzero(~np.isfinite(g)))
+            if bad:
+                raise RuntimeError(f'E_NONFINITE_SOURCE: {bad} values after evaluation {iteration}')
+            if done.any():
+                g = np.where(done[None, None, :], x, g)
+            new = self.boxes(g)
+            rr = np.max(abs(new - old) / np.maximum(abs(new), 1e-30), axis=(0, 1))
+            ss = np.max(abs(g - x), axis=(0, 1)) / np.maximum(np.max(abs(g), axis=(0, 1)), 1e-30)
+            if iteration >= 2:
+                done |= np.maximum(rr, ss) <= tolerance
+            history.append(dict(evaluation=iteration, observer_relative=rr.tolist(), source_relative=ss.tolist(),
+                                converged=done.tolist()))
+            if log:
+                log(f'ORACLE EVAL {iteration} obs={max(rr):.6g} source={max(ss):.6g} done={done.sum()}/{W}')
+            if done.all():
+                x = g
+                break
+            nxt = g.copy()
+            for w in range(W):
+                if done[w]:
+                    gs[w], fs[w] = [], []
+                    continue
+                gw = g[:, :, w]
+                norm = np.linalg.norm(gw, axis=1)
+                scale = np.maximum(norm, norm.max() * 1e-8)
+                feature = ((gw - x[:, :, w]) / scale[:, None]).astype('f4').ravel()
+                obs = np.asarray(
+                    ((new[:, :, w] - old[:, :, w]) / np.maximum(abs(new[:, :, w]), 1e-20)).ravel()
+                    * math.sqrt(len(feature) / (new.shape[0] * new.shape[1])), dtype=np.float32)
+                gs[w].append(gw.copy())
+                fs[w].append(np.concatenate([feature, obs]))
+                if len(fs[w]) > self.depth:
+                    gs[w].pop(0)
+                    fs[w].pop(0)
+                if len(fs[w]) > 1:
+                    f = np.stack(fs[w], 1)
+                    ah = anderson_weights(f.T @ f)
+                    if ah is not None:
+                        nxt[:, :, w] = sum(cw * q for cw, q in zip(ah, gs[w]))
+            x = nxt
+        if not done.all():
+            raise RuntimeError(f'E_NOT_CONVERGED: {done.sum()}/{W} within {tolerance}')
+        return x, dict(evaluations=iteration, history=history, seconds=time.perf_counter() - t0)
+
+    def table(self, source, hs, el, az, stokes_out=True):
+        """Table [len(hs), len(el), len(az), W] of I, or [..., 3] of I, Q, U (radiance_table.py:167-219)."""
+        ee, aa = np.meshgrid(el, az, indexing='ij')
+        pairs = PAIRS6 if stokes_out else PAIRS4
+        out = []
+        for h in hs:
+            a, cat = self.query_raw(h, ee, aa, source, True, pairs)
+            b, _ = self.query_raw(h, ee, aa, source, False, pairs)
+            tot = a.astype('f8') + b.astype('f8')
+            if stokes_out:
+                v = stokes(tot, cat['view'])
+            else:
+                v = ((tot[..., 0] + tot[..., 1]) + tot[..., 2])[..., None]
+            v = v * self.ctx.solar[None, :, None]
+            out.append(v.reshape(len(el), len(az), self.c['W'], -1))
+        t = np.stack(out)
+        return t if stokes_out else t[..., 0]

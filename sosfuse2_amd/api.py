@@ -1,0 +1,166 @@
+"""The reference's public entry points for this path, over the HIP backend:
+
+- `create_configured(output, cfg, wavelengths, backend, state, ...)` -> (s, scope)
+  (`src/sosfuse2/solve/build.py:14-118`);
+- `Solver(s, geometry).solve(max_eval, tolerance)` (`src/sosfuse2/solve/source_iteration.py:309-663`), in
+  `sosfuse2_amd.solver`;
+- `make_table(s, source, hs, el, az, group, batch)` -> (table, seconds) (`src/sosfuse2/outputs/radiance_table.py:167-219`),
+  here with Stokes I, Q, U.
+
+Settings (`cfg`) name the grid and quadrature the way the reference's settings do
+(`src/sosfuse2/solve/configuration.py:154-269`): nr, nchi, nmu, nphi, path_n, L, anderson_tol, observer_km, plus
+top_km, the chi grid kind, query_n and the Anderson depth of this engine.
+"""
+
+import time
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import atmosphere, engine, geometry
+from .kernel import Kernel, lane_count
+from .solver import Geometry, Solver, observer_bundle
+
+CONFIG0 = dict(nr=60, nchi=10, chi_grid='tiny', nmu=1, nphi=2, path_n=4, L=16, top_km=100.0, anderson_tol=1e-5,
+               anderson_depth=5, observer_km=0.1, query_n=6, wavelengths_nm=[550.0], solar_elevation_deg=[-3.0, -6.0],
+               view_shape=[4, 6])
+CONFIG1 = dict(nr=150, nchi=None, chi_grid='twilight', nmu=4, nphi=8, path_n=6, L=32, top_km=120.0, anderson_tol=1e-5,
+               anderson_depth=5, observer_km=0.1, query_n=12,
+               wavelengths_nm=atmosphere.log_spaced_wavelengths(30).tolist(),
+               solar_elevation_deg=np.arange(6.0, -19.0, -1.0).tolist(), view_shape=[32, 36])
+
+
+def axes(cfg):
+    """(wavelengths, solar elevations, view elevations, relative azimuths) (`radiance_table.py:34-40`)."""
+    nr, nc = cfg['view_shape']
+    el = 90 - (np.arange(nr) + 0.5) * 90 / nr
+    az = (np.arange(nc) + 0.5) * 180 / nc
+    return np.asarray(cfg['wavelengths_nm'], float), np.asarray(cfg['solar_elevation_deg'], float), el, az
+
+
+def make_grid(cfg):
+    top = float(cfg.get('top_km', 100.0))
+    h = np.asarray(cfg['h'], float) if 'h' in cfg else geometry.height_grid(int(cfg['nr']), top)
+    chi = geometry.chi_grid(cfg.get('chi_grid', 'twilight'))
+    if cfg.get('nchi') not in (None, len(chi)):
+        raise ValueError('E_SETTINGS_GRID_COUNT: nchi disagrees with the chi grid')
+    return geometry.Grid(h=h, chi=chi, nmu=int(cfg['nmu']), nphi=int(cfg['nphi']), path_n=int(cfg['path_n']),
+                         domain=geometry.Domain(top), query_n=int(cfg.get('query_n', 24)))
+
+
+class Scope:
+    """State-independent preparation shared by every chunk and state on one grid: the grid, the sun-column table,
+    the ground pools, the cell packs and their device bundles."""
+
+    def __init__(self, cfg, profile_state, kernel):
+        t = time.perf_counter()
+        self.cfg = cfg
+        self.grid = make_grid(cfg)
+        self.kernel = kernel
+        self.state = profile_state
+        self.sun = geometry.sun_table(profile_state, self.grid.domain, int(cfg.get('sun_scale', 1)))
+        self.pools = geometry.ground_pools(profile_state, self.grid, self.sun)
+        self.packs = [geometry.with_end_columns(profile_state, geometry.cell_pack(self.grid, i), self.grid.domain)
+                      for i in range(self.grid.ns)]
+        self.geometry = None
+        self.prepare_seconds = time.perf_counter() - t
+
+    def context(self, optics):
+        return engine.Context(self.grid, optics, self.kernel, self.sun, self.pools, self.cfg.get('observer_km', 0.1),
+                              self.state)
+
+    def solver(self, s, log=print):
+        if self.geometry is None:
+            self.geometry = Geometry(s, self.packs, self.kernel)
+        return Solver(s, self.geometry, depth=int(self.cfg.get('anderson_depth', 4)), log=log)
+
+
+def create_configured(output, cfg, wavelengths, backend, state, block_size=256, geometry_cache=None, *,
+                      share_sun_geometry=True, sun_scale=1, scope=None, mixed=False):
+    """Solver context `s` over `wavelengths` (nm) for one state, and its scope (`solve/build.py:14-118`).
+    `scope` reuses another state's preparation on the same grid and profile shapes."""
+    if output is not None:
+        Path(output).mkdir(parents=True, exist_ok=True)
+    if scope is None:
+        cfg = dict(cfg, sun_scale=sun_scale)
+        kernel = Kernel(backend=backend, L=int(cfg['L']), block_size=block_size, mixed=mixed)
+        scope = Scope(cfg, state, kernel)
+    wl = np.asarray(wavelengths, float)
+    if wl.ndim != 1 or len(wl) < 1:
+        raise ValueError('E_WAVELENGTHS: a one-dimensional wavelength list is required')
+    optics = atmosphere.lane_optics(state, wl)
+    lane_count(optics.W)
+    s = scope.context(optics)
+    return s, scope
+
+
+def stokes_from_tensor(comp, view):
+    """I, Q, U [rays, W, 3] from the six radiance tensor components in the observer frame. With
+    e_h = (-sin az, cos az, 0) and e_v = (-sin el cos az, -sin el sin az, cos el) spanning the plane normal to the view:
+    I = trace, Q = e_v.P.e_v - e_h.P.e_h, U = 2 e_v.P.e_h."""
+    el, az = view[:, 0], view[:, 1]
+    eh = np.stack([-np.sin(az), np.cos(az), np.zeros_like(az)], -1)
+    ev = np.stack([-np.sin(el) * np.cos(az), -np.sin(el) * np.sin(az), np.cos(el)], -1)
+    P = np.zeros(comp.shape[:2] + (3, 3))
+    c = comp.astype('f8')
+    P[..., 0, 0], P[..., 1, 1], P[..., 2, 2] = c[..., 0], c[..., 1], c[..., 2]
+    P[..., 0, 2] = P[..., 2, 0] = c[..., 3]
+    P[..., 0, 1] = P[..., 1, 0] = c[..., 4]
+    P[..., 1, 2] = P[..., 2, 1] = c[..., 5]
+    I = (c[..., 0] + c[..., 1]) + c[..., 2]
+    vPv = np.einsum('ni,nwij,nj->nw', ev, P, ev)
+    hPh = np.einsum('ni,nwij,nj->nw', eh, P, eh)
+    vPh = np.einsum('ni,nwij,nj->nw', ev, P, eh)
+    return np.stack([I, vPv - hPh, 2 * vPh], -1)
+
+
+def make_table(s, source, hs, el, az, group=64, batch=25, log=print, stokes=True):
+    """Table [len(hs), len(el), len(az), W, 3] of I, Q, U (or [..., W] of I) and per-batch seconds.
+    first + multiple orders times the solar irradiance, as `radiance_table.make_table`."""
+    if group < 1:
+        raise ValueError('E_TABLE_GROUP: group must be positive')
+    if batch < 1:
+        raise ValueError('E_TABLE_BATCH: batch must be positive')
+    k = s.kernel
+    G = lane_count(s.W)
+    if not torch.is_tensor(source):
+        src = np.asarray(source, 'f4')
+        if src.shape != (s.geo.ns, s.rows, s.W):
+            raise ValueError(f'E_SOURCE_SHAPE: {src.shape}')
+        if G > s.W:
+            src = np.concatenate([src, np.repeat(src[..., -1:], G - s.W, -1)], -1)
+        source = torch.from_numpy(np.ascontiguousarray(src)).to(k.device)
+    elif tuple(source.shape) != (s.geo.ns, s.rows, G):
+        raise ValueError(f'E_SOURCE_SHAPE: {tuple(source.shape)}')
+    ee, aa = np.meshgrid(el, az, indexing='ij')
+    pairs = 6 if stokes else 4
+    table = np.empty((len(hs), len(el), len(az), s.W) + ((3,) if stokes else ()), 'f4')
+    seconds = []
+    dc = k.constants(s, s.geo.query_n)
+    plane = k.plane(s, source, dc)
+    solar = s.solar
+    for b0 in range(0, len(hs), batch):
+        t = time.monotonic()
+        part = list(range(b0, min(b0 + batch, len(hs))))
+        host, db = observer_bundle(s, [hs[j] for j in part], ee, aa)
+        one = k.radiance(db, s, None, True, None, pairs)
+        many = k.radiance(db, s, source, False, None, pairs, plane)
+        tot = (one.double() + many.double()).cpu().numpy()[:, : s.W]
+        if stokes:
+            v = stokes_from_tensor(tot, host.view)
+        else:
+            v = (tot[..., 0] + tot[..., 1]) + tot[..., 2]
+        v = v * (solar[None, :, None] if stokes else solar[None, :])
+        off = 0
+        for j, n in zip(part, host.sizes):
+            spec = v[off : off + n].reshape(len(el), len(az), *v.shape[1:])
+            off += n
+            I = spec[..., 0] if stokes else spec
+            if not np.isfinite(spec).all() or (I < 0).any():
+                raise RuntimeError('E_RADIANCE_INVALID: nonfinite/negative')
+            table[j] = spec
+        seconds.append(time.monotonic() - t)
+        if log:
+            log(f'TABLE sun={hs[part[0]]:+g}..{hs[part[-1]]:+g} rays={db.rays} seconds={seconds[-1]:.3f}')
+    return table, seconds

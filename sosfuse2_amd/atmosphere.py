@@ -1,0 +1,282 @@
+"""Synthetic atmosphere states: height profiles, extinction coefficients and phase-function moments.
+
+The transport reads an atmosphere only through the quantities the reference's kernel constants carry
+(`src/sosfuse2/solve/source_solver.py:1288-1330`): per-component extinction coefficients ext[5, W] at unit profile
+value, the aerosol scattering coefficient asca[W], the component profiles at a height, the Henyey-Greenstein asymmetry,
+the ground albedo, and for a stratospheric layer its extinction, scattering, Legendre moments and profile.
+The component order is the reference's (`src/sosfuse2/atmosphere/components.py:23`): air, tropospheric aerosol,
+ozone, NO2, O4.
+
+Profiles here are the analytic shapes of the synthetic states (exponential air and aerosol, Gaussian ozone, NO2 and
+sulfate, O4 as the square of the normalised air density), defined on the whole calculation domain. The reference's
+profile_at (`src/sosfuse2/backends/numpy_reference.py:187-223`) evaluates its own shapes (exp(-h/8), exp(-h/H), a
+triangle ozone, a tabulated NO2); the extinction model `ext[k] * profile_k(h)` is the same
+(`numpy_reference.py:226-233`).
+
+Every lane of a solve carries its own optical state, so one launch can hold wavelengths of several states (batching).
+"""
+
+from dataclasses import dataclass, field, replace
+import math
+
+import numpy as np
+
+NCOMP = 5
+COMPONENTS = ('air', 'tropo_aerosol', 'o3', 'no2', 'o4')
+LOSCHMIDT_DU = 2.687e16  # molecules cm^-2 per Dobson unit
+KM_CM = 1e5
+
+
+@dataclass(frozen=True)
+class State:
+    """One atmosphere parameter set (a synthetic state of the configurations)."""
+
+    name: str = 'config0'
+    top_km: float = 100.0
+    rayleigh_tau550: float = 0.097
+    air_height_km: float = 8.0
+    depolarisation: float = 0.03  # delta; mix = (1 - delta) / (1 + delta / 2) (source_solver.py:1294)
+    aod550: float = 0.1
+    aerosol_height_km: float = 1.2
+    aerosol_g: float = 0.7
+    aerosol_ssa: float = 0.95
+    angstrom: float = 1.3
+    ozone_du: float = 300.0
+    ozone_center_km: float = 22.0
+    ozone_sigma_km: float = 5.0
+    no2_column: float = 0.0  # molecules cm^-2
+    no2_center_km: float = 25.0
+    no2_sigma_km: float = 5.0
+    o4: bool = False
+    sulfate_aod550: float = 0.0
+    sulfate_center_km: float = 20.0
+    sulfate_sigma_km: float = 3.0
+    sulfate_g: float = 0.6
+    sulfate_ssa: float = 1.0
+    sulfate_angstrom: float = 1.0
+    albedo: float = 0.1
+
+    @property
+    def strato(self):
+        return self.sulfate_aod550 > 0
+
+    @property
+    def mix(self):
+        d = self.depolarisation
+        return (1 - d) / (1 + d / 2)
+
+
+def config0_state():
+    """Small configuration: Rayleigh, tropospheric aerosol, ozone, ground albedo 0.1, top 100 km."""
+    return State(name='config0')
+
+
+def config1_state():
+    """Full configuration: config0's aerosol plus a sulfate layer, NO2 and O4, top 120 km."""
+    return State(name='config1', top_km=120.0, no2_column=5e15, o4=True, sulfate_aod550=0.02)
+
+
+def sample_states(n, seed=0, top_km=120.0, max_wavelengths=30, max_elevations=25):
+    """The batched distribution: AOD log-uniform 0.02-0.5, g U(0.6, 0.8), sulfate AOD U(0, 0.05), ozone U(250, 400) DU,
+    albedo U(0, 0.3); per state an uneven wavelength count U{1..30} and solar-elevation count U{2..25}."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        s = State(
+            name=f'batch{seed}_{i}',
+            top_km=top_km,
+            aod550=float(np.exp(rng.uniform(np.log(0.02), np.log(0.5)))),
+            aerosol_g=float(rng.uniform(0.6, 0.8)),
+            sulfate_aod550=float(rng.uniform(0.0, 0.05)),
+            ozone_du=float(rng.uniform(250, 400)),
+            albedo=float(rng.uniform(0.0, 0.3)),
+            no2_column=5e15,
+            o4=True,
+        )
+        nw = int(rng.integers(1, max_wavelengths + 1))
+        ne = int(rng.integers(2, max_elevations + 1))
+        out.append((s, nw, ne))
+    return out
+
+
+def gaussian(h, center, sigma):
+    t = (np.asarray(h, 'f8') - center) / sigma
+    return np.exp(-0.5 * t * t)
+
+
+def profiles(state, h):
+    """Component profiles [n, 5] at heights h (km), FP64. Air and aerosol are 1 at the ground; ozone and NO2 peak at 1;
+    O4 is the square of the air profile (`numpy_reference.py:219-220`)."""
+    h = np.maximum(np.asarray(h, 'f8'), 0.0)
+    air = np.exp(-h / state.air_height_km)
+    return np.stack(
+        [
+            air,
+            np.exp(-h / state.aerosol_height_km),
+            gaussian(h, state.ozone_center_km, state.ozone_sigma_km),
+            gaussian(h, state.no2_center_km, state.no2_sigma_km),
+            air * air,
+        ],
+        -1,
+    )
+
+
+def strato_profile(state, h):
+    """Sulfate layer profile, peak 1 (`numpy_reference.py:423-428` with norm 1)."""
+    return gaussian(np.maximum(np.asarray(h, 'f8'), 0.0), state.sulfate_center_km, state.sulfate_sigma_km)
+
+
+def _column_exp(H, top):
+    return H * (-math.expm1(-top / H))
+
+
+def _column_gauss(c, s, top):
+    a = (0 - c) / (s * math.sqrt(2))
+    b = (top - c) / (s * math.sqrt(2))
+    return s * math.sqrt(math.pi / 2) * (math.erf(b) - math.erf(a))
+
+
+def ozone_cross_section(wl):
+    """Synthetic smooth O3 cross-section (cm^2): a Chappuis band near 602 nm plus the Huggins tail."""
+    wl = np.asarray(wl, 'f8')
+    return 4.7e-21 * np.exp(-0.5 * ((wl - 602.0) / 55.0) ** 2) + 1e-19 * np.exp(-(wl - 330.0) / 12.0) + 2e-23
+
+
+def no2_cross_section(wl):
+    """Synthetic smooth NO2 cross-section (cm^2), broad maximum near 420 nm."""
+    wl = np.asarray(wl, 'f8')
+    return 6e-19 * np.exp(-0.5 * ((wl - 420.0) / 60.0) ** 2)
+
+
+def o4_cross_section(wl):
+    """Synthetic O4 collision-induced absorption (cm^5 molecule^-2): Gaussian bands at 477, 532, 577 and 630 nm."""
+    wl = np.asarray(wl, 'f8')
+    bands = ((477.0, 6.3e-46, 4.0), (532.0, 1.0e-46, 4.0), (577.0, 1.1e-45, 5.0), (630.0, 7.5e-46, 5.0))
+    return sum(a * np.exp(-0.5 * ((wl - c) / w) ** 2) for c, a, w in bands)
+
+
+def solar_irradiance(wl):
+    """Synthetic top-of-atmosphere irradiance (W m^-2 nm^-1): a 5778 K Planck curve scaled to 1.85 at 550 nm."""
+    wl = np.asarray(wl, 'f8') * 1e-9
+
+    def planck(x):
+        return 1.0 / (x**5 * np.expm1(0.014387769 / (x * 5778.0)))
+
+    return 1.85 * planck(wl) / planck(550e-9)
+
+
+@dataclass
+class Optics:
+    """Per-lane optical constants of one solve chunk (FP64 host values; the device receives FP32 copies).
+
+    ext[5, W]: extinction per km at unit profile value; asca[W]: aerosol scattering; g[W]: aerosol asymmetry;
+    albedo[W]; strato_ext/strato_sca/strato_g[W]; mix[W] (Rayleigh depolarisation factor); solar[W]."""
+
+    ext: np.ndarray
+    asca: np.ndarray
+    g: np.ndarray
+    albedo: np.ndarray
+    mix: np.ndarray
+    solar: np.ndarray
+    wavelengths: np.ndarray
+    strato_ext: np.ndarray
+    strato_sca: np.ndarray
+    strato_g: np.ndarray
+    lane_state: np.ndarray  # index of the state each lane belongs to
+    states: list = field(default_factory=list)
+
+    @property
+    def W(self):
+        return len(self.asca)
+
+    @property
+    def strato(self):
+        return bool(np.any(self.strato_ext > 0))
+
+
+def lane_optics(state, wl):
+    """Optics of one state at wavelengths wl (nm)."""
+    wl = np.atleast_1d(np.asarray(wl, 'f8'))
+    top = state.top_km
+    ray = state.rayleigh_tau550 / _column_exp(state.air_height_km, top) * (550.0 / wl) ** 4
+    aer = state.aod550 / _column_exp(state.aerosol_height_km, top) * (550.0 / wl) ** state.angstrom
+    oz_col = state.ozone_du * LOSCHMIDT_DU / (_column_gauss(state.ozone_center_km, state.ozone_sigma_km, top) * KM_CM)
+    o3 = ozone_cross_section(wl) * oz_col * KM_CM
+    no2_n = state.no2_column / (_column_gauss(state.no2_center_km, state.no2_sigma_km, top) * KM_CM)
+    no2 = no2_cross_section(wl) * no2_n * KM_CM
+    n_o2 = 0.20946 * 2.547e19  # O2 number density at the ground (cm^-3)
+    o4 = o4_cross_section(wl) * n_o2 * n_o2 * KM_CM if state.o4 else np.zeros_like(wl)
+    ext = np.stack([ray, aer, o3, no2, o4])
+    if state.strato:
+        col = _column_gauss(state.sulfate_center_km, state.sulfate_sigma_km, top)
+        sext = state.sulfate_aod550 / col * (550.0 / wl) ** state.sulfate_angstrom
+    else:
+        sext = np.zeros_like(wl)
+    W = len(wl)
+    return Optics(
+        ext=ext,
+        asca=aer * state.aerosol_ssa,
+        g=np.full(W, state.aerosol_g),
+        albedo=np.full(W, state.albedo),
+        mix=np.full(W, state.mix),
+        solar=solar_irradiance(wl),
+        wavelengths=wl,
+        strato_ext=sext,
+        strato_sca=sext * state.sulfate_ssa,
+        strato_g=np.full(W, state.sulfate_g),
+        lane_state=np.zeros(W, 'i4'),
+        states=[state],
+    )
+
+
+def concat_optics(parts):
+    """Pack the lanes of several states into one chunk. All parts must share the profile shapes (aerosol height,
+    ozone/NO2/sulfate centres and widths, top), since the transport evaluates one set of profiles per sample."""
+    base = parts[0].states[0]
+    for p in parts:
+        s = p.states[0]
+        for key in ('top_km', 'air_height_km', 'aerosol_height_km', 'ozone_center_km', 'ozone_sigma_km',
+                    'no2_center_km', 'no2_sigma_km', 'sulfate_center_km', 'sulfate_sigma_km'):
+            if getattr(s, key) != getattr(base, key):
+                raise ValueError(f'E_BATCH_PROFILE: states differ in {key}')
+    states = []
+    lane_state = []
+    for p in parts:
+        lane_state.append(p.lane_state + len(states))
+        states.extend(p.states)
+    cat = lambda k: np.concatenate([getattr(p, k) for p in parts], axis=-1)
+    return Optics(
+        ext=cat('ext'), asca=cat('asca'), g=cat('g'), albedo=cat('albedo'), mix=cat('mix'), solar=cat('solar'),
+        wavelengths=cat('wavelengths'), strato_ext=cat('strato_ext'), strato_sca=cat('strato_sca'),
+        strato_g=cat('strato_g'), lane_state=np.concatenate(lane_state), states=states,
+    )
+
+
+def pad_optics(o, W):
+    """Pad a chunk to W lanes by repeating its last lane (the reference pads the last 15-chunk the same way,
+    `src/sosfuse2/jobs/run_state.py:592-599`)."""
+    if o.W == W:
+        return o
+    if o.W > W:
+        raise ValueError('E_WAVELENGTHS: chunk wider than the lane count')
+    idx = np.r_[np.arange(o.W), np.full(W - o.W, o.W - 1)]
+    take = lambda a: a[..., idx]
+    return Optics(
+        ext=take(o.ext), asca=take(o.asca), g=take(o.g), albedo=take(o.albedo), mix=take(o.mix), solar=take(o.solar),
+        wavelengths=take(o.wavelengths), strato_ext=take(o.strato_ext), strato_sca=take(o.strato_sca),
+        strato_g=take(o.strato_g), lane_state=take(o.lane_state), states=o.states,
+    )
+
+
+def hg_moments(g, L):
+    """Legendre moments chi_l = (2l+1) g^l of a Henyey-Greenstein phase function (`docs/MODEL.md:236-242`), l = 0..L."""
+    l = np.arange(L + 1)
+    return (2 * l + 1) * np.asarray(g, 'f8')[..., None] ** l
+
+
+def log_spaced_wavelengths(n, lo=400.0, hi=700.0):
+    return np.exp(np.linspace(np.log(lo), np.log(hi), n))
+
+
+def with_albedo(state, albedo):
+    return replace(state, albedo=albedo)

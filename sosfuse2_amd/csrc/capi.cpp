@@ -1,0 +1,174 @@
+// C ABI of libsosfuse2_hip.so: argument checks, error strings, and dispatch to the kernel launchers.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "common.h"
+
+namespace sos {
+
+static thread_local char g_error[512] = "";
+
+int set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof(g_error), fmt, ap);
+    va_end(ap);
+    return SOS_E_LAUNCH;
+}
+
+static int arg_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof(g_error), fmt, ap);
+    va_end(ap);
+    return SOS_E_ARGUMENT;
+}
+
+int transport_dispatch(const TransportArgs &a, int G, int L, hipStream_t s);
+int project_dispatch(const ProjectArgs &a, int G, int L, hipStream_t s);
+int compose_launch(int64_t cells, int rows, int K, int G, const float *mom, const float *first, const float *factor,
+                   float *out, hipStream_t s);
+int plane_launch(int nh, int nc, int L, int rows, int G, int strato, const float *src, const float *ha,
+                 const float *sd, int half, void *plane, hipStream_t s);
+int metrics_launch(int64_t rows, int G, const float *g, const float *x, unsigned frozen, float *out, hipStream_t s);
+int residual_launch(int64_t cells, int rows, int G, const float *g, const float *x, const float *obs, int nobs,
+                    float *work, float *f, hipStream_t s);
+int gram_launch(int64_t n, int G, int depth, int slot, unsigned filled, const float *ring, double *work, double *gram,
+                hipStream_t s);
+int solve_launch(int G, int depth, int m, const int *order, const double *gram, unsigned frozen, float *w,
+                 hipStream_t s);
+int mix_launch(int64_t cells, int rows, int G, int depth, int m, const float *ring, const int *order, const float *w,
+               float *x, hipStream_t s);
+
+static bool lanes_ok(int G) { return G == 1 || G == 16 || G == 32; }
+static bool order_ok(int L) { return L == 8 || L == 16 || L == 32; }
+
+}  // namespace sos
+
+using namespace sos;
+
+extern "C" {
+
+const char *sos_last_error(void) { return g_error; }
+
+int sos_device_info(int32_t *cus, int32_t *xcds, char *arch, int32_t arch_len) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return set_error("no HIP device");
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return set_error("hipGetDeviceProperties failed");
+    if (cus) *cus = p.multiProcessorCount;
+    if (xcds) *xcds = 8;
+    if (arch && arch_len > 0) {
+        strncpy(arch, p.gcnArchName, arch_len - 1);
+        arch[arch_len - 1] = 0;
+    }
+    return SOS_OK;
+}
+
+int sos_transport(const sos_grid *g, const sos_bundle *b, int32_t flags, int32_t pairs, int32_t rows,
+                  const float *field_dev, const void *plane_dev, const float *boundary_dev, float *out_dev,
+                  void *stream) {
+    if (!g || !b || !out_dev) return arg_error("sos_transport: null grid, bundle or output");
+    if (!lanes_ok(g->lanes)) return arg_error("sos_transport: lanes %d not in {1, 16, 32}", g->lanes);
+    if (!order_ok(g->L)) return arg_error("sos_transport: L %d not in {8, 16, 32}", g->L);
+    if (pairs != 4 && pairs != 6) return arg_error("sos_transport: pairs %d not 4 or 6", pairs);
+    if (g->nh < 2 || g->nc < 2 || g->nq < 1) return arg_error("sos_transport: grid too small");
+    const bool first = flags & SOS_FIRST;
+    if (!first && (!field_dev || !plane_dev)) return arg_error("sos_transport: warm transport needs field and plane");
+    if (!first && rows < 10) return arg_error("sos_transport: rows %d < 10", rows);
+    if ((flags & SOS_BOUNDARY) && !boundary_dev) return arg_error("sos_transport: boundary flag without boundary");
+    if (first && (flags & SOS_BOUNDARY)) return arg_error("sos_transport: first light takes no boundary");
+    if (first && (!g->sun_col_dev || g->sun_nh < 2 || g->sun_nx < 2 || !g->gm_dev || !g->goff_dev))
+        return arg_error("sos_transport: first light needs the sun table, ground pools and gm");
+    if (b->rays <= 0) return SOS_OK;
+    TransportArgs a;
+    memset(&a, 0, sizeof(a));
+    a.nh = g->nh; a.nc = g->nc; a.nq = g->nq; a.strato = g->strato; a.flags = flags; a.pairs = pairs; a.rows = rows;
+    a.sun_nh = g->sun_nh; a.sun_nx = g->sun_nx; a.top = g->top_km;
+    a.shape = ProfileShape{1.0f / g->air_H, 1.0f / g->aer_H, g->o3_c, 1.0f / g->o3_s,
+                           g->no2_c, 1.0f / g->no2_s, g->st_c, 1.0f / g->st_s};
+    a.gl_u = g->gl_u_dev; a.gl_w = g->gl_w_dev; a.h = g->h_dev; a.chi = g->chi_dev; a.hr = g->hr_dev;
+    a.lane = g->lane_dev; a.sun_h = g->sun_h_dev; a.sun_col = g->sun_col_dev; a.goff = g->goff_dev;
+    a.gdir = g->gdir_dev; a.gval = g->gval_dev; a.gm = g->gm_dev;
+    a.rays = b->rays;
+    const int rpw = 64 / g->lanes;
+    a.tiles = (b->rays + rpw - 1) / rpw;
+    a.starts = b->starts_dev; a.targets = b->targets_dev; a.ray_group = b->ray_group_dev; a.dirs = b->dirs_dev;
+    a.dloc = b->dloc_dev; a.edge_off = b->edge_off_dev; a.edges = b->edges_dev; a.lower = b->lower_dev;
+    a.end_j = b->end_j_dev; a.end_f = b->end_f_dev; a.end_col = b->end_col_dev; a.hits = b->hits_dev;
+    a.field = field_dev; a.plane = plane_dev; a.boundary = boundary_dev; a.out = out_dev;
+    return transport_dispatch(a, g->lanes, g->L, (hipStream_t)stream);
+}
+
+int sos_project(const sos_grid *g, const sos_bundle *b, const float *rad_dev, int32_t ground_only, float *moments_dev,
+                float *flux_dev, void *stream) {
+    if (!g || !b || !rad_dev || !flux_dev) return arg_error("sos_project: null argument");
+    if (!ground_only && !moments_dev) return arg_error("sos_project: moments output missing");
+    if (!lanes_ok(g->lanes) || !order_ok(g->L)) return arg_error("sos_project: lanes %d / L %d", g->lanes, g->L);
+    ProjectArgs a;
+    a.groups = b->groups; a.group_off = b->group_off_dev; a.dirs = b->dirs_dev; a.dloc = b->dloc_dev;
+    a.weights = b->weights_dev; a.rad = rad_dev; a.lane = g->lane_dev; a.ground_only = ground_only;
+    a.moments = moments_dev; a.flux = flux_dev;
+    return project_dispatch(a, g->lanes, g->L, (hipStream_t)stream);
+}
+
+int sos_compose(const sos_grid *g, int32_t rows, const float *moments_dev, const float *first_dev,
+                const float *factor_dev, float *source_dev, void *stream) {
+    if (!g || !moments_dev || !factor_dev || !source_dev) return arg_error("sos_compose: null argument");
+    const int K = (g->L + 1) * (g->L + 2) / 2;
+    if (rows != 10 + K && rows != 10 + 2 * K) return arg_error("sos_compose: rows %d for K %d", rows, K);
+    return compose_launch((int64_t)g->nh * g->nc, rows, K, g->lanes, moments_dev, first_dev, factor_dev, source_dev,
+                          (hipStream_t)stream);
+}
+
+int sos_make_plane(const sos_grid *g, int32_t rows, const float *source_dev, const float *ha_dev, const float *sd_dev,
+                   int32_t half, void *plane_dev, void *stream) {
+    if (!g || !source_dev || !ha_dev || !plane_dev) return arg_error("sos_make_plane: null argument");
+    const int K = (g->L + 1) * (g->L + 2) / 2;
+    const int strato = rows == 10 + 2 * K;
+    if (rows != 10 + K && !strato) return arg_error("sos_make_plane: rows %d for K %d", rows, K);
+    if (strato && !sd_dev) return arg_error("sos_make_plane: sulfate plane without its density");
+    return plane_launch(g->nh, g->nc, g->L, rows, g->lanes, strato, source_dev, ha_dev, sd_dev, half, plane_dev,
+                        (hipStream_t)stream);
+}
+
+int sos_source_metrics(int64_t rows, int32_t lanes, const float *g_dev, const float *x_dev, uint32_t frozen,
+                       float *out_dev, void *stream) {
+    if (!g_dev || !x_dev || !out_dev || lanes < 1 || lanes > 32) return arg_error("sos_source_metrics: bad argument");
+    if (!lanes_ok(lanes)) return arg_error("sos_source_metrics: lanes %d", lanes);
+    return metrics_launch(rows, lanes, g_dev, x_dev, frozen, out_dev, (hipStream_t)stream);
+}
+
+int sos_anderson_residual(int64_t cells, int32_t rows, int32_t lanes, const float *g_dev, const float *x_dev,
+                          const float *obs_dev, int32_t nobs, float *work_dev, float *f_dev, void *stream) {
+    if (!g_dev || !x_dev || !work_dev || !f_dev || (nobs > 0 && !obs_dev) || lanes < 1 || lanes > 32)
+        return arg_error("sos_anderson_residual: bad argument");
+    return residual_launch(cells, rows, lanes, g_dev, x_dev, obs_dev, nobs, work_dev, f_dev, (hipStream_t)stream);
+}
+
+int sos_anderson_gram(int64_t n, int32_t lanes, int32_t depth, int32_t slot, uint32_t filled, const float *ring_dev,
+                      double *work_dev, double *gram_dev, void *stream) {
+    if (!ring_dev || !work_dev || !gram_dev || depth < 1 || depth > 8 || slot < 0 || slot >= depth || lanes < 1 ||
+        lanes > 32)
+        return arg_error("sos_anderson_gram: bad argument");
+    return gram_launch(n, lanes, depth, slot, filled, ring_dev, work_dev, gram_dev, (hipStream_t)stream);
+}
+
+int sos_anderson_solve(int32_t lanes, int32_t depth, int32_t m, const int32_t *order_dev, const double *gram_dev,
+                       uint32_t frozen, float *w_dev, void *stream) {
+    if (!order_dev || !gram_dev || !w_dev || m < 1 || m > depth || depth > 8 || lanes < 1 || lanes > 32)
+        return arg_error("sos_anderson_solve: bad argument");
+    return solve_launch(lanes, depth, m, order_dev, gram_dev, frozen, w_dev, (hipStream_t)stream);
+}
+
+int sos_anderson_mix(int64_t cells, int32_t rows, int32_t lanes, int32_t depth, int32_t m, const float *ring_dev,
+                     const int32_t *order_dev, const float *w_dev, float *x_dev, void *stream) {
+    if (!ring_dev || !order_dev || !w_dev || !x_dev || m < 1 || m > depth || lanes < 1 || lanes > 32)
+        return arg_error("sos_anderson_mix: bad argument");
+    return mix_launch(cells, rows, lanes, depth, m, ring_dev, order_dev, w_dev, x_dev, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+// Shared device helpers of the MI355X SOS kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include "../../include/sosfuse2_hip.h"
+
+#define SOS_R_EARTH 6371.0
+#define SOS_FLT_MIN 1.17549435e-38f
+#define SOS_PI 3.14159265358979323846
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+namespace sos {
+
+constexpr int kidx(int l, int m) { return l * (l + 1) / 2 + m; }
+
+// Offset of order m in the m-major SH plane: orders 0..m-1 hold L+1-m' degrees each.
+template <int L>
+constexpr int moff(int m) { return m * (L + 1) - m * (m - 1) / 2; }
+
+// Largest i with a[i] <= x, clipped to [0, n-2] (numpy searchsorted(side='right') - 1, then clip).
+__device__ inline int interval(const float *__restrict__ a, int n, float x) {
+    int lo = 0, hi = n;  // first index with a[i] > x lies in [lo, hi]
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    int i = lo - 1;
+    i = i < 0 ? 0 : i;
+    return i > n - 2 ? n - 2 : i;
+}
+
+__device__ inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+
+// Henyey-Greenstein per steradian (numpy_reference.py:390-392).
+__device__ inline float hg(float u, float g) {
+    float den = fmaxf(1e-20f, (1.0f + g * g) - (2.0f * g) * u);
+    return (1.0f - g * g) / (12.566370614359172f * (den * sqrtf(den)));
+}
+
+struct ProfileShape {
+    float inv_air_H, inv_aer_H, o3_c, inv_o3_s, no2_c, inv_no2_s, st_c, inv_st_s;
+};
+
+// Five component profiles and the sulfate profile at height h.
+__device__ inline void profiles(const ProfileShape &s, float h, float *p, float &st) {
+    float air = expf(-h * s.inv_air_H);
+    p[0] = air;
+    p[1] = expf(-h * s.inv_aer_H);
+    float t = (h - s.o3_c) * s.inv_o3_s;
+    p[2] = expf(-0.5f * (t * t));
+    t = (h - s.no2_c) * s.inv_no2_s;
+    p[3] = expf(-0.5f * (t * t));
+    p[4] = air * air;
+    t = (h - s.st_c) * s.inv_st_s;
+    st = expf(-0.5f * (t * t));
+}
+
+struct TransportArgs {
+    // grid
+    int nh, nc, nq, strato, flags, pairs, rows, sun_nh, sun_nx;
+    float top;
+    ProfileShape shape;
+    const double *gl_u, *gl_w;
+    const float *h, *chi, *hr, *lane;
+    const float *sun_h, *sun_col;
+    const int64_t *goff;
+    const float *gdir, *gval, *gm;
+    // bundle
+    int64_t rays, tiles;
+    const double *starts;
+    const float *targets;
+    const int32_t *ray_group;
+    const double *dirs;
+    const float *dloc;
+    const int64_t *edge_off;
+    const double *edges;
+    const uint16_t *lower;
+    const int32_t *end_j;
+    const float *end_f, *end_col;
+    const uint8_t *hits;
+    // fields
+    const float *field;
+    const void *plane;
+    const float *boundary;
+    float *out;
+};
+
+struct ProjectArgs {
+    int64_t groups;
+    const int64_t *group_off;
+    const double *dirs;
+    const float *dloc, *weights, *rad, *lane;
+    int ground_only;
+    float *moments, *flux;
+};
+
+}  // namespace sos

@@ -1,0 +1,318 @@
+// Source composition, the combined SH plane, convergence metrics and the Anderson update on the device (gfx950).
+//
+// compose      src/sosfuse2/solve/source_iteration.py:176-212 (eigenvalue scaling, FP32 flush of |s| < FLT_MIN)
+// make_plane   src/sosfuse2/backends/sh_layout.py:135-152 (both aerosol planes weighted by their densities),
+//              stored m-major per cell so the transport's recurrence reads it in order
+// metrics      source_iteration.py:52-97 (non-finite count, max |g|, max |g - x| per lane; integer atomics only)
+// anderson_*   source_iteration.py:432-443 (scaled residual), :618-635 (Gram products and mixing), :25-33 (FP64
+//              regularised solve and guard). All floating sums have a fixed order.
+
+#include "common.h"
+
+namespace sos {
+int set_error(const char *fmt, ...);
+
+static int check(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error("%s: %s", what, hipGetErrorString(e));
+    return SOS_OK;
+}
+
+static unsigned blocks_for(int64_t n, int threads = 256, int64_t cap = 65535 * 8) {
+    int64_t b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+// ----------------------------------------------------------------------------------------------------- compose
+__global__ void compose_kernel(int64_t cells, int rows, int K, int G, const float *__restrict__ mom,
+                               const float *__restrict__ first, const float *__restrict__ factor,
+                               float *__restrict__ out) {
+    const int64_t total = cells * rows * G;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int w = (int)(e % G);
+        const int64_t cr = e / G;
+        const int row = (int)(cr % rows);
+        const int64_t cell = cr / rows;
+        const int mr = row < 10 ? row : 4 + (row - 10) % K;
+        const int64_t src = (cell * (4 + K) + mr) * G + w;
+        float m = mom[src];
+        if (first) m = m + first[src];
+        float v = factor[row * G + w] * m;
+        if (fabsf(v) < SOS_FLT_MIN) v = 0.0f;
+        out[e] = v;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------- make_plane
+template <bool HALF>
+__global__ void plane_kernel(int nh, int nc, int L, int rows, int G, int strato, const float *__restrict__ src,
+                             const float *__restrict__ ha, const float *__restrict__ sd, void *__restrict__ plane) {
+    const int K = (L + 1) * (L + 2) / 2;
+    const int64_t total = (int64_t)nh * nc * K * G;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int w = (int)(e % G);
+        const int64_t ck = e / G;
+        const int j = (int)(ck % K);  // m-major position
+        const int64_t cell = ck / K;
+        // invert j -> (m, l): order m holds L + 1 - m degrees
+        int m = 0, off = 0;
+        while (off + (L + 1 - m) <= j) { off += L + 1 - m; ++m; }
+        const int l = m + (j - off);
+        const int k = l * (l + 1) / 2 + m;
+        const int ih = (int)(cell / nc);
+        const float *s = src + cell * rows * G + w;
+        float v = s[(10 + k) * G] * ha[ih];
+        if (strato) v = v + s[(10 + K + k) * G] * sd[ih];
+        if constexpr (HALF) reinterpret_cast<__half *>(plane)[e] = __float2half(v);
+        else reinterpret_cast<float *>(plane)[e] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------- metrics
+__global__ void zero_u32(unsigned *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void metrics_kernel(int64_t rows, int G, const float *__restrict__ g, const float *__restrict__ x,
+                               unsigned frozen, unsigned *__restrict__ acc) {
+    // acc: [3][G] = count, max |value| bits, max |value - x| bits. The grid stride is a multiple of G, so every
+    // thread sees one lane; it reduces in registers and publishes once (max and integer sums are order-free).
+    __shared__ unsigned s[3][32];
+    if (threadIdx.x < 3 * 32) s[threadIdx.x / 32][threadIdx.x % 32] = 0;
+    __syncthreads();
+    const int64_t total = rows * G;
+    const int64_t start = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int w = (int)(start % G);
+    unsigned bad = 0, mag = 0, dif = 0;
+    for (int64_t e = start; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const float cur = g[e], old = x[e];
+        const float val = (frozen >> w) & 1u ? old : cur;
+        bad += !isfinite(cur);
+        mag = max(mag, __float_as_uint(fabsf(val)));
+        dif = max(dif, __float_as_uint(fabsf(val - old)));
+    }
+    atomicAdd(&s[0][w], bad);
+    atomicMax(&s[1][w], mag);
+    atomicMax(&s[2][w], dif);
+    __syncthreads();
+    if (threadIdx.x < G) {
+        atomicAdd(&acc[threadIdx.x], s[0][threadIdx.x]);
+        atomicMax(&acc[G + threadIdx.x], s[1][threadIdx.x]);
+        atomicMax(&acc[2 * G + threadIdx.x], s[2][threadIdx.x]);
+    }
+}
+
+__global__ void metrics_finish(int G, const unsigned *acc, float *out) {
+    int w = threadIdx.x;
+    if (w < G) {
+        out[w] = (float)acc[w];
+        out[G + w] = __uint_as_float(acc[G + w]);
+        out[2 * G + w] = __uint_as_float(acc[2 * G + w]);
+    }
+}
+
+// --------------------------------------------------------------------------------------------------- anderson
+__global__ void norms_kernel(int64_t cells, int rows, int G, const float *__restrict__ g, float *__restrict__ norms,
+                             unsigned *__restrict__ maxbits) {
+    const int64_t total = cells * G;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int w = (int)(e % G);
+        const int64_t cell = e / G;
+        const float *p = g + cell * rows * G + w;
+        double s = 0.0;
+        for (int r = 0; r < rows; ++r) {
+            const double v = p[(int64_t)r * G];
+            s += v * v;
+        }
+        const float nrm = (float)sqrt(s);
+        norms[e] = nrm;
+        atomicMax(&maxbits[w], __float_as_uint(nrm));
+    }
+}
+
+__global__ void residual_kernel(int64_t cells, int rows, int G, const float *__restrict__ g,
+                                const float *__restrict__ x, const float *__restrict__ norms,
+                                const unsigned *__restrict__ maxbits, const float *__restrict__ obs, int nobs,
+                                float *__restrict__ f) {
+    const int64_t nsrc = cells * rows;
+    const int64_t n = nsrc + nobs;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (i < nsrc) {
+            const int64_t cell = i / rows;
+            for (int w = 0; w < G; ++w) {
+                const float nrm = norms[cell * G + w];
+                const float scale = fmaxf(nrm, __uint_as_float(maxbits[w]) * 1e-8f);
+                f[(int64_t)w * n + i] = (g[i * G + w] - x[i * G + w]) / scale;
+            }
+        } else {
+            const int64_t o = i - nsrc;
+            for (int w = 0; w < G; ++w) f[(int64_t)w * n + i] = obs[(int64_t)w * nobs + o];
+        }
+    }
+}
+
+// Block partial dot products: part[b][w][j] = sum over the block's range of f[slot][w][i] f[j][w][i].
+__global__ __launch_bounds__(256) void gram_partial(int64_t n, int G, int depth, int slot, unsigned filled,
+                                                    const float *__restrict__ ring, double *__restrict__ part) {
+    __shared__ double red[256];
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t i0 = blockIdx.x * per, i1 = min(n, i0 + per);
+    for (int w = 0; w < G; ++w) {
+        const float *fs = ring + ((int64_t)slot * G + w) * n;
+        for (int j = 0; j < depth; ++j) {
+            double s = 0.0;
+            if ((filled >> j) & 1u) {
+                const float *fj = ring + ((int64_t)j * G + w) * n;
+                for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) s += (double)fs[i] * (double)fj[i];
+            }
+            red[threadIdx.x] = s;
+            __syncthreads();
+            for (int st = 128; st > 0; st >>= 1) {
+                if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) part[((int64_t)blockIdx.x * G + w) * depth + j] = red[0];
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void gram_finish(int blocks, int G, int depth, int slot, unsigned filled, const double *__restrict__ part,
+                            double *__restrict__ gram) {
+    const int t = threadIdx.x;
+    if (t >= G * depth) return;
+    const int w = t / depth, j = t % depth;
+    if (!((filled >> j) & 1u)) return;
+    double s = 0.0;
+    for (int b = 0; b < blocks; ++b) s += part[((int64_t)b * G + w) * depth + j];
+    double *gm = gram + (int64_t)w * depth * depth;
+    gm[slot * depth + j] = s;
+    gm[j * depth + slot] = s;
+}
+
+__global__ void solve_kernel(int G, int depth, int m, const int *__restrict__ order, const double *__restrict__ gram,
+                             unsigned frozen, float *__restrict__ wout) {
+    const int w = threadIdx.x;
+    if (w >= G) return;
+    float *wo = wout + w * depth;
+    if ((frozen >> w) & 1u) { wo[0] = __int_as_float(0x7fc00000); return; }
+    double A[8][9];
+    const double *gm = gram + (int64_t)w * depth * depth;
+    double tr = 0.0;
+    for (int i = 0; i < m; ++i) tr += gm[order[i] * depth + order[i]];
+    const double reg = fmax(tr * 1e-10, 1e-30);
+    for (int i = 0; i < m; ++i) {
+        for (int j = 0; j < m; ++j) A[i][j] = gm[order[i] * depth + order[j]] + (i == j ? reg : 0.0);
+        A[i][m] = 1.0;
+    }
+    bool ok = true;
+    for (int c = 0; c < m; ++c) {  // Gaussian elimination, partial pivoting
+        int piv = c;
+        for (int r = c + 1; r < m; ++r) if (fabs(A[r][c]) > fabs(A[piv][c])) piv = r;
+        if (A[piv][c] == 0.0) { ok = false; break; }
+        if (piv != c) for (int k = 0; k <= m; ++k) { double t = A[c][k]; A[c][k] = A[piv][k]; A[piv][k] = t; }
+        for (int r = c + 1; r < m; ++r) {
+            const double f = A[r][c] / A[c][c];
+            for (int k = c; k <= m; ++k) A[r][k] -= f * A[c][k];
+        }
+    }
+    double x[8];
+    if (ok) {
+        for (int r = m - 1; r >= 0; --r) {
+            double s = A[r][m];
+            for (int k = r + 1; k < m; ++k) s -= A[r][k] * x[k];
+            x[r] = s / A[r][r];
+        }
+        double sum = 0.0;
+        for (int i = 0; i < m; ++i) sum += x[i];
+        double mx = 0.0;
+        for (int i = 0; i < m; ++i) {
+            x[i] /= sum;
+            if (!isfinite(x[i])) ok = false;
+            mx = fmax(mx, fabs(x[i]));
+        }
+        if (!(mx < 8.0)) ok = false;
+    }
+    if (!ok) { wo[0] = __int_as_float(0x7fc00000); return; }
+    for (int i = 0; i < m; ++i) wo[i] = (float)x[i];
+}
+
+__global__ void mix_kernel(int64_t elems, int G, int depth, int m, const float *__restrict__ ring,
+                           const int *__restrict__ order, const float *__restrict__ wts, float *__restrict__ x) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < elems * G;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int w = (int)(e % G);
+        const float *wl = wts + w * depth;
+        if (isnan(wl[0])) continue;
+        float v = ring[(int64_t)order[0] * elems * G + e] * wl[0];
+        for (int j = 1; j < m; ++j) v = v + ring[(int64_t)order[j] * elems * G + e] * wl[j];
+        x[e] = v;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ host entries
+int compose_launch(int64_t cells, int rows, int K, int G, const float *mom, const float *first, const float *factor,
+                   float *out, hipStream_t s) {
+    hipLaunchKernelGGL(compose_kernel, dim3(blocks_for(cells * rows * G)), dim3(256), 0, s, cells, rows, K, G, mom,
+                       first, factor, out);
+    return check("compose");
+}
+
+int plane_launch(int nh, int nc, int L, int rows, int G, int strato, const float *src, const float *ha,
+                 const float *sd, int half, void *plane, hipStream_t s) {
+    const int K = (L + 1) * (L + 2) / 2;
+    const unsigned b = blocks_for((int64_t)nh * nc * K * G);
+    if (half)
+        hipLaunchKernelGGL(plane_kernel<true>, dim3(b), dim3(256), 0, s, nh, nc, L, rows, G, strato, src, ha, sd, plane);
+    else
+        hipLaunchKernelGGL(plane_kernel<false>, dim3(b), dim3(256), 0, s, nh, nc, L, rows, G, strato, src, ha, sd, plane);
+    return check("make_plane");
+}
+
+int metrics_launch(int64_t rows, int G, const float *g, const float *x, unsigned frozen, float *out, hipStream_t s) {
+    // out doubles as the uint accumulator [3][G] before the final conversion
+    unsigned *acc = reinterpret_cast<unsigned *>(out);
+    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(128), 0, s, acc, 3 * G);
+    hipLaunchKernelGGL(metrics_kernel, dim3(blocks_for(rows * G, 256, 4096)), dim3(256), 0, s, rows, G, g, x, frozen,
+                       acc);
+    hipLaunchKernelGGL(metrics_finish, dim3(1), dim3(64), 0, s, G, acc, out);
+    return check("source_metrics");
+}
+
+int residual_launch(int64_t cells, int rows, int G, const float *g, const float *x, const float *obs, int nobs,
+                    float *work, float *f, hipStream_t s) {
+    float *norms = work;
+    unsigned *maxbits = reinterpret_cast<unsigned *>(work + cells * G);
+    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, maxbits, G);
+    hipLaunchKernelGGL(norms_kernel, dim3(blocks_for(cells * G)), dim3(256), 0, s, cells, rows, G, g, norms, maxbits);
+    hipLaunchKernelGGL(residual_kernel, dim3(blocks_for(cells * rows + nobs)), dim3(256), 0, s, cells, rows, G, g, x,
+                       norms, maxbits, obs, nobs, f);
+    return check("anderson_residual");
+}
+
+int gram_launch(int64_t n, int G, int depth, int slot, unsigned filled, const float *ring, double *work, double *gram,
+                hipStream_t s) {
+    int blocks = (int)min((int64_t)1024, (n + 4095) / 4096);
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(gram_partial, dim3(blocks), dim3(256), 0, s, n, G, depth, slot, filled, ring, work);
+    hipLaunchKernelGGL(gram_finish, dim3(1), dim3(256), 0, s, blocks, G, depth, slot, filled, work, gram);
+    return check("anderson_gram");
+}
+
+int solve_launch(int G, int depth, int m, const int *order, const double *gram, unsigned frozen, float *w,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(solve_kernel, dim3(1), dim3(64), 0, s, G, depth, m, order, gram, frozen, w);
+    return check("anderson_solve");
+}
+
+int mix_launch(int64_t cells, int rows, int G, int depth, int m, const float *ring, const int *order, const float *w,
+               float *x, hipStream_t s) {
+    const int64_t elems = cells * rows;
+    hipLaunchKernelGGL(mix_kernel, dim3(blocks_for(elems * G)), dim3(256), 0, s, elems, G, depth, m, ring, order, w, x);
+    return check("anderson_mix");
+}
+
+}  // namespace sos

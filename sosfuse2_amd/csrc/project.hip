@@ -1,0 +1,208 @@
+// Projection of per-ray radiances onto the incident-radiance moments of each destination cell (gfx950).
+//
+// moments[cell][k<4][w]  = sum_rays weight * rad[ray][w][k]                     (tensor moments)
+// moments[cell][4+k][w]  = sum_rays (Y_k(mu) cos(m phi)) * (trace * weight)    (SH moments, k = l(l+1)/2 + m)
+// flux[cell][w]          = albedo/pi * sum_rays trace * weight * max(0, dloc_z) (ground flux)
+// (src/sosfuse2/backends/numpy_reference.py:683-711.) The SH moments are a dense contraction
+// Z[rays x K]^T . V[rays x lanes]: one workgroup per cell stages 16 rays of Z (FP64 recurrence rounded to FP32, times
+// the FP32 Chebyshev cos(m phi)) in LDS, and each wavefront accumulates 16-harmonic tiles with
+// v_mfma_f32_16x16x4f32 (4 rays per instruction, lanes as the N dimension). Tensor moments and flux are FP64 sums in
+// ray order. Sums are in a fixed order: results do not depend on timing.
+
+#include "common.h"
+
+namespace sos {
+
+
+
+template <int G, int L>
+__global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
+    constexpr int K = (L + 1) * (L + 2) / 2;
+    constexpr int KT = (K + 15) / 16;
+    constexpr int KP = KT * 16;
+    constexpr int NT = G >= 16 ? G / 16 : 1;
+    constexpr int TPW = (KT + 3) / 4;  // harmonic tiles per wavefront
+    __shared__ float Z[16][KP];
+    __shared__ float V[16][G];
+    __shared__ float cmv[16][L + 1];
+    __shared__ double seed[16][L + 1];
+    __shared__ double mus[16];
+    const int cell = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int64_t r0 = a.group_off[cell], r1 = a.group_off[cell + 1];
+    // tensor moments and flux, FP64 in ray order
+    if (tid < 5 * G) {
+        const int comp = tid / G, w = tid % G;
+        if (comp < 4 && !a.ground_only) {
+            double s = 0.0;
+            for (int64_t r = r0; r < r1; ++r) s += (double)(a.rad[(r * G + w) * 4 + comp] * a.weights[r]);
+            a.moments[((int64_t)cell * (4 + K) + comp) * G + w] = (float)s;
+        } else if (comp == 4) {
+            double s = 0.0;
+            for (int64_t r = r0; r < r1; ++r) {
+                const float *x = a.rad + (r * G + w) * 4;
+                const float tr = (x[0] + x[1]) + x[2];
+                s += (double)(tr * (a.weights[r] * fmaxf(0.0f, a.dloc[3 * r + 2])));
+            }
+            a.flux[(int64_t)cell * G + w] = (float)(s * (double)a.lane[SOS_LANE_ALBEDO * G + w] / SOS_PI);
+        }
+    }
+    if (a.ground_only) return;
+    const int wave = tid >> 6, lane = tid & 63;
+    floatx4 acc[TPW][NT];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[t][n] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float accv[(K + 255) / 256];
+#pragma unroll
+    for (int i = 0; i < (K + 255) / 256; ++i) accv[i] = 0.f;
+    for (int64_t c0 = r0; c0 < r1; c0 += 16) {
+        const int n = (int)min((int64_t)16, r1 - c0);
+        if (tid < 16) {
+            const int i = tid;
+            double mu = 0.0, st = 0.0;
+            float cp = 1.0f;
+            if (i < n) {
+                const int64_t r = c0 + i;
+                const double dx = a.dirs[3 * r], dz = a.dirs[3 * r + 2];
+                mu = fmin(1.0, fmax(-1.0, dz));
+                st = sqrt(fmax(0.0, 1.0 - dz * dz));
+                cp = (float)(st > 1e-12 ? dx / fmax(st, 1e-30) : 1.0);
+                cp = clampf(cp, -1.0f, 1.0f);
+            }
+            mus[i] = mu;
+            const double stc = sqrt(fmax(0.0, 1.0 - mu * mu));
+            double diag = 0.28209479177387814;
+            float c0m = 1.0f, c1m = cp;
+            for (int m = 0; m <= L; ++m) {
+                if (m) diag = (diag * -sqrt((2.0 * m + 1) / (2.0 * m))) * stc;
+                seed[i][m] = diag;
+                float cm;
+                if (m == 0) cm = 1.0f;
+                else if (m == 1) cm = cp;
+                else { cm = (2.0f * cp) * c1m - c0m; c0m = c1m; c1m = cm; }
+                cmv[i][m] = cm;
+            }
+        }
+        for (int e = tid; e < 16 * G; e += 256) {
+            const int i = e / G, w = e % G;
+            float v = 0.0f;
+            if (i < n) {
+                const int64_t r = c0 + i;
+                const float *x = a.rad + (r * G + w) * 4;
+                v = ((x[0] + x[1]) + x[2]) * a.weights[r];
+            }
+            V[i][w] = v;
+        }
+        __syncthreads();
+        for (int e = tid; e < 16 * (L + 1); e += 256) {
+            const int i = e / (L + 1), m = e % (L + 1);
+            const double mu = mus[i];
+            const float mult = m ? 1.41421356237309515f : 1.0f;
+            const float cm = cmv[i][m];
+            double z0 = seed[i][m];
+            if (i >= n) {
+                for (int l = m; l <= L; ++l) Z[i][kidx(l, m)] = 0.0f;
+                continue;
+            }
+            Z[i][kidx(m, m)] = ((float)(mult * z0)) * cm;
+            if (m < L) {
+                double z1 = (sqrt(2.0 * m + 3) * mu) * z0;
+                Z[i][kidx(m + 1, m)] = ((float)(mult * z1)) * cm;
+                for (int l = m + 2; l <= L; ++l) {
+                    const double aa = sqrt((4.0 * l * l - 1) / ((double)l * l - (double)m * m));
+                    const double bb = sqrt((2.0 * l + 1) * ((l - 1.0) * (l - 1.0) - (double)m * m) /
+                                           ((2.0 * l - 3) * ((double)l * l - (double)m * m)));
+                    const double z2 = (aa * mu) * z1 - bb * z0;
+                    Z[i][kidx(l, m)] = ((float)(mult * z2)) * cm;
+                    z0 = z1;
+                    z1 = z2;
+                }
+            }
+        }
+        if constexpr (KP > K)
+            for (int e = tid; e < 16 * (KP - K); e += 256) Z[e / (KP - K)][K + e % (KP - K)] = 0.0f;
+        __syncthreads();
+        if constexpr (G >= 16) {
+            const int col = lane & 15, kk = lane >> 4;
+#pragma unroll
+            for (int t = 0; t < TPW; ++t) {
+                const int kt = wave + 4 * t;
+                if (kt < KT) {
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const int ray = 4 * s + kk;
+                        const float av = Z[ray][kt * 16 + col];
+#pragma unroll
+                        for (int nt = 0; nt < NT; ++nt)
+                            acc[t][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, V[ray][nt * 16 + col], acc[t][nt], 0, 0, 0);
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < (K + 255) / 256; ++q) {
+                const int k = tid + 256 * q;
+                if (k < K)
+                    for (int i = 0; i < n; ++i) accv[q] = accv[q] + Z[i][k] * V[i][0];
+            }
+        }
+        __syncthreads();
+    }
+    float *out = a.moments + (int64_t)cell * (4 + K) * G;
+    if constexpr (G >= 16) {
+        const int col = lane & 15, rb = 4 * (lane >> 4);
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+            const int kt = wave + 4 * t;
+            if (kt < KT) {
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int k = kt * 16 + rb + v;
+                        if (k < K) out[(4 + k) * G + nt * 16 + col] = acc[t][nt][v];
+                    }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < (K + 255) / 256; ++q) {
+            const int k = tid + 256 * q;
+            if (k < K) out[(4 + k) * G] = accv[q];
+        }
+    }
+}
+
+int set_error(const char *fmt, ...);
+
+template <int G, int L>
+static int project_launch(const ProjectArgs &a, hipStream_t s) {
+    if (a.groups <= 0) return SOS_OK;
+    hipLaunchKernelGGL((project_kernel<G, L>), dim3((unsigned)a.groups), dim3(256), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error("project launch: %s", hipGetErrorString(e));
+    return SOS_OK;
+}
+
+template <int G>
+static int project_L(const ProjectArgs &a, int L, hipStream_t s) {
+    switch (L) {
+        case 8: return project_launch<G, 8>(a, s);
+        case 16: return project_launch<G, 16>(a, s);
+        case 32: return project_launch<G, 32>(a, s);
+        default: return set_error("unsupported L %d", L);
+    }
+}
+
+int project_dispatch(const ProjectArgs &a, int G, int L, hipStream_t s) {
+    switch (G) {
+        case 1: return project_L<1>(a, L, s);
+        case 16: return project_L<16>(a, L, s);
+        case 32: return project_L<32>(a, L, s);
+        default: return set_error("unsupported lane count %d", G);
+    }
+}
+
+}  // namespace sos

@@ -1,0 +1,493 @@
+// Line-of-sight transport through the spherical shell (gfx950).
+//
+// One lane per (ray, wavelength lane): G lanes per ray, 64/G rays per wavefront. Each lane walks its ray's segments
+// and Gauss-Legendre nodes, keeps its own optical depth (Simpson prefix with Kahan at segment ends) and its own Kahan
+// radiance sum, and reconstructs the source at the four bilinear corners of every node. Per ray and corner cell the
+// 33 azimuthal amplitudes sum_l Y_lm(mu) c_lm are cached in registers and refilled only when the corner cell
+// changes, after the reference's adjacent-stencil reuse (src/sosfuse2/backends/numpy_reference.py:106-129,573-600).
+// Node positions, radii and the shadow test are FP64 (the chord near the tangent point and the solar terminator are
+// where FP32 cancels); everything per wavelength is FP32 with denormals flushed.
+//
+// Operation order follows numpy_reference.py:467-643 (transport), :294-387 (sample), :236-291 (Simpson prefix),
+// :431-458 (sun columns), :89-103 (log-space trace blend).
+
+#include "common.h"
+#include "sh_constants.h"
+
+namespace sos {
+
+
+
+template <bool HALF>
+__device__ inline float ld_plane(const void *p, int64_t i) {
+    if constexpr (HALF) return __half2float(reinterpret_cast<const __half *>(p)[i]);
+    else return reinterpret_cast<const float *>(p)[i];
+}
+
+// Azimuthal amplitudes of one corner cell for a ray with direction cosine mu = d_z (global frame):
+// out[m] = sqrt2^(m>0) * sum_{l>=m} P_lm(mu) c[cell][(l,m)], the recurrence of arithmetic.py:86-109 in FP32.
+template <int L, int G, bool HALF>
+__device__ inline void refill(const void *plane, int cell, int w, float mu, float st, float (&out)[L + 1]) {
+    constexpr int K = (L + 1) * (L + 2) / 2;
+    const int64_t base = (int64_t)cell * K * G + w;
+    float diag = 0.28209479177387814f;
+#pragma unroll
+    for (int m = 0; m <= L; ++m) {
+        if (m) diag = (diag * ShConst<L>::dd()[m]) * st;
+        float z0 = diag;
+        const int64_t o = base + (int64_t)moff<L>(m) * G;
+        float acc = z0 * ld_plane<HALF>(plane, o);
+        if (m < L) {
+            float z1 = (ShConst<L>::sq()[m] * mu) * z0;
+            acc = acc + z1 * ld_plane<HALF>(plane, o + G);
+            const float *aa = ShConst<L>::aa(), *bb = ShConst<L>::bb();
+#pragma unroll 4
+            for (int l = m + 2; l <= L; ++l) {
+                const int k = kidx(l, m);
+                float z2 = (aa[k] * mu) * z1 - bb[k] * z0;
+                acc = acc + z2 * ld_plane<HALF>(plane, o + (int64_t)(l - m) * G);
+                z0 = z1;
+                z1 = z2;
+            }
+        }
+        out[m] = m ? acc * 1.41421356237309515f : acc;
+    }
+}
+
+template <int P>
+struct PairIdx;
+template <>
+struct PairIdx<4> { static constexpr int i[4] = {0, 1, 2, 0}, k[4] = {0, 1, 2, 2}; };
+template <>
+struct PairIdx<6> { static constexpr int i[6] = {0, 1, 2, 0, 0, 1}, k[6] = {0, 1, 2, 2, 1, 2}; };
+
+// Node geometry shared by the lanes of a ray.
+struct Node {
+    float h, ds, ah, ac, cx, sx, sn, cc;
+    int ir, j;
+    double r, qz;
+};
+
+__device__ inline void node_position(const double *p, const double *d, double t, double &r, double &qz, double &qx,
+                                     double &qy) {
+    qx = p[0] + t * d[0];
+    qy = p[1] + t * d[1];
+    qz = p[2] + t * d[2];
+    r = sqrt(qx * qx + qy * qy + qz * qz);
+}
+
+__device__ inline float node_height(double r, float top) {
+    return clampf((float)(r - SOS_R_EARTH), 0.0f, top);
+}
+
+// Extinction of this lane at a height.
+__device__ inline float extinction(const ProfileShape &s, const float *lc, float sext, float h) {
+    float p[5], st;
+    profiles(s, h, p, st);
+    float e = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) e = e + p[k] * lc[k];
+    return e + st * sext;
+}
+
+template <int G, int L, int P, bool FIRST, bool BOUNDARY, bool HALF>
+__global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
+    constexpr int RPW = 64 / G;
+    const int lane = threadIdx.x & 63;
+    const int w = lane % G;
+    const int sub = lane / G;
+    // lane constants
+    float lc[SOS_LANE_CONSTANTS];
+#pragma unroll
+    for (int k = 0; k < SOS_LANE_CONSTANTS; ++k) lc[k] = a.lane[k * G + w];
+    const float mix = lc[SOS_LANE_MIX], gasy = lc[SOS_LANE_G], asca = lc[SOS_LANE_ASCA];
+    const float sext = a.strato ? lc[SOS_LANE_SEXT] : 0.0f;
+    // XCD-contiguous tile ranges: workgroup b runs on XCD b % 8.
+    const int64_t nwg = gridDim.x;
+    const int xcd = blockIdx.x & 7, idx = blockIdx.x >> 3;
+    const int64_t per_xcd_wg = (nwg + 7 - xcd) >> 3;  // workgroups on this XCD
+    const int64_t waves_xcd = per_xcd_wg * 4;
+    const int64_t chunk = (a.tiles + 7) >> 3;
+    const int64_t t0 = xcd * chunk, t1 = min(a.tiles, t0 + chunk);
+    for (int64_t tile = t0 + idx * 4 + (threadIdx.x >> 6); tile < t1; tile += waves_xcd) {
+        const int64_t ray = tile * RPW + sub;
+        if (ray >= a.rays) continue;
+        const int grp = a.ray_group[ray];
+        double p[3], d[3];
+        p[0] = a.starts[3 * grp]; p[1] = a.starts[3 * grp + 1]; p[2] = a.starts[3 * grp + 2];
+        d[0] = a.dirs[3 * ray]; d[1] = a.dirs[3 * ray + 1]; d[2] = a.dirs[3 * ray + 2];
+        float B[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) B[k] = a.targets[9 * grp + k];  // B[3*k + i]: component k of basis vector i
+        const float df0 = (float)d[0], df1 = (float)d[1], df2 = (float)d[2];
+        const float dl[3] = {a.dloc[3 * ray], a.dloc[3 * ray + 1], a.dloc[3 * ray + 2]};
+        const float st = (float)sqrt(fmax(0.0, 1.0 - d[2] * d[2]));
+        const float mu = df2;
+        float pu[P], direct[P];
+        {
+            // sun direction in the target frame: row 2 of B
+            const float u0 = B[6], u1 = B[7], u2 = B[8];
+            const float dot = (dl[0] * u0 + dl[1] * u1) + dl[2] * u2;
+            const float e[3] = {u0 - dl[0] * dot, u1 - dl[1] * dot, u2 - dl[2] * dot};
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const int i = PairIdx<P>::i[q], k = PairIdx<P>::k[q];
+                pu[q] = (i == k ? 1.0f : 0.0f) - dl[i] * dl[k];
+                direct[q] = (pu[q] - e[i] * e[k]) / 2.0f;
+            }
+        }
+        float phase = 0.0f, sphase = 0.0f;
+        if constexpr (FIRST) {
+            phase = hg(df2, gasy);
+            if (a.strato) sphase = hg(df2, lc[SOS_LANE_SG]);
+        }
+        float total[P], corr[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) total[q] = corr[q] = 0.0f;
+        float tau = 0.0f, tc = 0.0f;
+        float cache[4][L + 1];
+        int tags[4] = {-1, -1, -1, -1};
+        const int64_t e0 = a.edge_off[ray];
+        const int nseg = (int)(a.edge_off[ray + 1] - e0) - 1;
+        const unsigned bits = a.lower[ray];
+        for (int seg = 0; seg < nseg; ++seg) {
+            const double sa = a.edges[e0 + seg], sb = a.edges[e0 + seg + 1];
+            if (!(sb - sa > 1e-8)) continue;
+            const bool lower = (bits >> seg) & 1u;
+            const double len = sb - sa;
+            // Simpson prefix: panel start at the segment start
+            double tprev = sa;
+            double r_, qz_, qx_, qy_;
+            node_position(p, d, sa, r_, qz_, qx_, qy_);
+            float bprev = extinction(a.shape, lc, sext, node_height(r_, a.top));
+            float acc = 0.0f;
+            for (int step = 0; step < a.nq; ++step) {
+                const double u = a.gl_u[step], wu = a.gl_w[step];
+                const double v = lower ? u : 1.0 - u;
+                const double t = lower ? sa + len * (u * u) : sb - len * (v * v);
+                const float ds = (float)(len * v * wu) * 2.0f;
+                double r, qz, qx, qy;
+                node_position(p, d, t, r, qz, qx, qy);
+                const float h = node_height(r, a.top);
+                const double rho = sqrt(qx * qx + qy * qy);
+                const float cx = rho > 1e-12 ? (float)(qx / rho) : 1.0f;
+                const float sx = rho > 1e-12 ? (float)(qy / rho) : 0.0f;
+                const float angle = atan2f((float)rho, (float)qz);
+                const int ir = interval(a.h, a.nh, h);
+                const int j = interval(a.chi, a.nc, angle);
+                const float h0 = a.h[ir], h1 = a.h[ir + 1], c0 = a.chi[j], c1 = a.chi[j + 1];
+                const float ah = clampf((h - h0) / (h1 - h0), 0.0f, 1.0f);
+                const float ac = clampf((angle - c0) / (c1 - c0), 0.0f, 1.0f);
+                const float ang = c0 + ac * (c1 - c0);
+                float sn, cc;
+                sincosf(ang, &sn, &cc);
+                // rotation rows Q[k][j] (numpy_reference.py:327)
+                const float Q[9] = {-cc * cx, sx, sn * cx, -cc * sx, -cx, sn * sx, sn, 0.0f, cc};
+                float prof[5], stp;
+                profiles(a.shape, h, prof, stp);
+                float bnode = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 5; ++k) bnode = bnode + prof[k] * lc[k];
+                bnode = bnode + stp * sext;
+                {
+                    const double tm = 0.5 * (tprev + t);
+                    double rm, a1, a2, a3;
+                    node_position(p, d, tm, rm, a1, a2, a3);
+                    const float bm = extinction(a.shape, lc, sext, node_height(rm, a.top));
+                    const float w6 = (float)(t - tprev) / 6.0f;
+                    acc = acc + w6 * ((bprev + bnode) + 4.0f * bm);
+                    tprev = t;
+                    bprev = bnode;
+                }
+                const float mid = (tau - tc) + acc;
+                const float att = ds * expf(-mid);
+                const int nc = a.nc;
+                const int cells[4] = {ir * nc + j, ir * nc + j + 1, (ir + 1) * nc + j, (ir + 1) * nc + j + 1};
+                const float sw[4] = {(1.0f - ah) * (1.0f - ac), (1.0f - ah) * ac, ah * (1.0f - ac), ah * ac};
+                // tensor rotation T[q][u] (numpy_reference.py:328-351)
+                float T[P][4];
+                {
+                    float A[3][3], dt[3];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int jj = 0; jj < 3; ++jj)
+                            A[i][jj] = (B[i] * Q[jj] + B[3 + i] * Q[3 + jj]) + B[6 + i] * Q[6 + jj];
+#pragma unroll
+                    for (int jj = 0; jj < 3; ++jj) dt[jj] = (df0 * Q[jj] + df1 * Q[3 + jj]) + df2 * Q[6 + jj];
+                    float Kx[3][3];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int jj = 0; jj < 3; ++jj) Kx[i][jj] = A[i][jj] - dl[i] * dt[jj];
+#pragma unroll
+                    for (int q = 0; q < P; ++q) {
+                        const int i = PairIdx<P>::i[q], k = PairIdx<P>::k[q];
+                        T[q][0] = Kx[i][0] * Kx[k][0];
+                        T[q][1] = Kx[i][1] * Kx[k][1];
+                        T[q][2] = Kx[i][2] * Kx[k][2];
+                        T[q][3] = Kx[i][0] * Kx[k][2] + Kx[i][2] * Kx[k][0];
+                    }
+                }
+                float val[P];
+#pragma unroll
+                for (int q = 0; q < P; ++q) val[q] = 0.0f;
+                if constexpr (FIRST) {
+                    // sun columns (numpy_reference.py:431-458), FP64 near the terminator
+                    double height = r - SOS_R_EARTH;
+                    if (!(height > 0.0)) height = 0.0;
+                    const double rad = SOS_R_EARTH + height;
+                    const double mg = -sqrt(height * (2.0 * SOS_R_EARTH + height)) / rad;
+                    double delta = qz / r - mg;
+                    if (fabs(delta) < (double)SOS_FLT_MIN) delta = 0.0;
+                    const bool lit = delta >= 0.0;
+                    const float x = clampf((float)sqrt((lit ? delta : 0.0) / (1.0 - mg)), 0.0f, 1.0f);
+                    const float hh = clampf(h, 0.0f, a.top);
+                    const int si = interval(a.sun_h, a.sun_nh, hh);
+                    const float al = (hh - a.sun_h[si]) / (a.sun_h[si + 1] - a.sun_h[si]);
+                    const float fx = x * (float)(a.sun_nx - 1);
+                    int sj = (int)fx;
+                    sj = sj < 0 ? 0 : (sj > a.sun_nx - 2 ? a.sun_nx - 2 : sj);
+                    const float bx = fx - (float)sj;
+                    const float w00 = (1.0f - al) * (1.0f - bx), w10 = al * (1.0f - bx), w01 = (1.0f - al) * bx,
+                                w11 = al * bx;
+                    const float *c00 = a.sun_col + ((int64_t)si * a.sun_nx + sj) * 6;
+                    const float *c10 = c00 + (int64_t)a.sun_nx * 6;
+                    float optical = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        float col = w00 * c00[k];
+                        col = col + w10 * c10[k];
+                        col = col + w01 * c00[6 + k];
+                        col = col + w11 * c10[6 + k];
+                        optical = optical + col * (k < 5 ? lc[k] : sext);
+                    }
+                    const float sun = lit ? expf(-optical) : 0.0f;
+                    // ground pools: exact paired HG phases (numpy_reference.py:526-557)
+                    float mus = 0.0f, exd = 0.0f, eyd = 0.0f;
+                    mus = (df0 * Q[2] + df1 * Q[5]) + df2 * Q[8];
+                    exd = (df0 * Q[0] + df1 * Q[3]) + df2 * Q[6];
+                    eyd = fabsf((df0 * Q[1] + df1 * Q[4]) + df2 * Q[7]);
+                    float ground = 0.0f, sground = 0.0f;
+                    const float sg = lc[SOS_LANE_SG];
+#pragma unroll 1
+                    for (int c = 0; c < 4; ++c) {
+                        const int64_t g0 = a.goff[cells[c]], g1 = a.goff[cells[c] + 1];
+                        const float swc = sw[c];
+                        for (int64_t gi = g0; gi < g1; ++gi) {
+                            const float gx = a.gdir[3 * gi], gy = a.gdir[3 * gi + 1], gz = a.gdir[3 * gi + 2];
+                            const float m0 = mus * gz + exd * gx;
+                            const float m1 = eyd * gy;
+                            const float up = clampf(m0 + m1, -1.0f, 1.0f), dn = clampf(m0 - m1, -1.0f, 1.0f);
+                            const float gvv = a.gval[gi * G + w];
+                            ground = ground + (swc * ((hg(up, gasy) + hg(dn, gasy)) / 2.0f)) * gvv;
+                            if (a.strato) sground = sground + (swc * ((hg(up, sg) + hg(dn, sg)) / 2.0f)) * gvv;
+                        }
+                    }
+                    const float beam = (prof[1] * asca) * (sun * phase + ground);
+                    const float ray_ = (sun * prof[0]) * lc[0];
+                    const float sbeam = (stp * lc[SOS_LANE_SSCA]) * (sun * sphase + sground);
+#pragma unroll
+                    for (int q = 0; q < P; ++q) {
+                        val[q] = ray_ * (mix * 0.119366207318921496f * direct[q] +
+                                         (1.0f - mix) * 0.0397887357729738339f * pu[q]) +
+                                 beam * pu[q] / 2.0f;
+                        if (a.strato) val[q] = val[q] + (sbeam * pu[q]) / 2.0f;
+                    }
+                    // Rayleigh scattering of the ground-reflected sun (gm), density at the node
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const float *g = a.gm + (int64_t)cells[c] * 4 * G + w;
+                        const float rm[4] = {g[0], g[G], g[2 * G], g[3 * G]};
+                        const float trace = (rm[0] + rm[1]) + rm[2];
+#pragma unroll
+                        for (int q = 0; q < P; ++q) {
+                            float rot = 0.0f;
+#pragma unroll
+                            for (int uu = 0; uu < 4; ++uu) rot = rot + T[q][uu] * rm[uu];
+                            const float tensor = prof[0] * (mix * rot + ((1.0f - mix) / 3.0f * trace) * pu[q]);
+                            val[q] = val[q] + sw[c] * tensor;
+                        }
+                    }
+                } else {
+                    // adjacent-stencil reuse of cached corners (numpy_reference.py:106-129)
+                    const int old = tags[0];
+                    if (old >= 0) {
+                        const int di = cells[0] / nc - old / nc, dj = cells[0] % nc - old % nc;
+                        auto copy = [&](int dst, int src) {
+#pragma unroll
+                            for (int m = 0; m <= L; ++m) cache[dst][m] = cache[src][m];
+                            tags[dst] = cells[dst];
+                        };
+                        if (di == 0 && dj == 1) { copy(0, 1); copy(2, 3); }
+                        else if (di == 0 && dj == -1) { copy(1, 0); copy(3, 2); }
+                        else if (di == 1 && dj == 0) { copy(0, 2); copy(1, 3); }
+                        else if (di == -1 && dj == 0) { copy(2, 0); copy(3, 1); }
+                        else if (di == 1 && dj == 1) { copy(0, 3); }
+                        else if (di == 1 && dj == -1) { copy(1, 2); }
+                        else if (di == -1 && dj == 1) { copy(2, 1); }
+                        else if (di == -1 && dj == -1) { copy(3, 0); }
+                    }
+#pragma unroll 1
+                    for (int c = 0; c < 4; ++c) {
+                        const int cell = c == 0 ? cells[0] : c == 1 ? cells[1] : c == 2 ? cells[2] : cells[3];
+                        const int tag = c == 0 ? tags[0] : c == 1 ? tags[1] : c == 2 ? tags[2] : tags[3];
+                        if (tag != cell) {
+                            float fresh[L + 1];
+                            refill<L, G, HALF>(a.plane, cell, w, mu, st, fresh);
+#pragma unroll
+                            for (int s = 0; s < 4; ++s)
+                                if (s == c) {
+#pragma unroll
+                                    for (int m = 0; m <= L; ++m) cache[s][m] = fresh[m];
+                                    tags[s] = cell;
+                                }
+                        }
+                    }
+                    // cos(m phi) of this node (arithmetic.py:112-119) and the four azimuthal sums
+                    const float cp = clampf(st > 1e-12f ? (df0 * cx + df1 * sx) / fmaxf(st, 1e-30f) : 1.0f, -1.0f, 1.0f);
+                    float aero[4];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) aero[c] = cache[c][0];
+                    {
+                        float cm0 = 1.0f, cm1 = cp;
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) aero[c] = aero[c] + cm1 * cache[c][1];
+#pragma unroll
+                        for (int m = 2; m <= L; ++m) {
+                            const float cm2 = (2.0f * cp) * cm1 - cm0;
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) aero[c] = aero[c] + cm2 * cache[c][m];
+                            cm0 = cm1;
+                            cm1 = cm2;
+                        }
+                    }
+                    float logs = 0.0f, wg = 0.0f, am = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const float *f = a.field + (int64_t)cells[c] * a.rows * G + w;
+                        const float rm[4] = {f[0], f[G], f[2 * G], f[3 * G]};
+                        const float trace = (rm[0] + rm[1]) + rm[2];
+                        const float density = a.hr[ir + (c >> 1)];
+                        float tens[P];
+#pragma unroll
+                        for (int q = 0; q < P; ++q) {
+                            float rot = 0.0f;
+#pragma unroll
+                            for (int uu = 0; uu < 4; ++uu) rot = rot + T[q][uu] * rm[uu];
+                            tens[q] = density * (mix * rot + ((1.0f - mix) / 3.0f * trace) * pu[q]);
+                            tens[q] = tens[q] + (aero[c] * pu[q]) / 2.0f;
+                            val[q] = val[q] + sw[c] * tens[q];
+                        }
+                        const float tr = (tens[0] + tens[1]) + tens[2];
+                        if (tr >= SOS_FLT_MIN) {
+                            logs = logs + sw[c] * logf(tr);
+                            wg = wg + sw[c];
+                            am = am + sw[c] * tr;
+                        }
+                    }
+                    const bool ok = (wg > 0.0f) && (am >= SOS_FLT_MIN);
+                    const float mean = ok ? am / wg : 1.0f;
+                    const float scale = expf(ok ? logs - wg * logf(mean) : 0.0f);
+#pragma unroll
+                    for (int q = 0; q < P; ++q) val[q] = val[q] * scale;
+                }
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    const float y = val[q] * att - corr[q];
+                    const float tn = total[q] + y;
+                    corr[q] = (tn - total[q]) - y;
+                    total[q] = tn;
+                }
+            }
+            // end piece of the segment, then the segment's optical depth into the ray total (Kahan)
+            {
+                const double tm = 0.5 * (tprev + sb);
+                double rm, a1, a2, a3;
+                node_position(p, d, tm, rm, a1, a2, a3);
+                const float bm = extinction(a.shape, lc, sext, node_height(rm, a.top));
+                node_position(p, d, sb, rm, a1, a2, a3);
+                const float be = extinction(a.shape, lc, sext, node_height(rm, a.top));
+                const float w6 = (float)(sb - tprev) / 6.0f;
+                acc = acc + w6 * ((bprev + be) + 4.0f * bm);
+                const float y = acc - tc;
+                const float tn = tau + y;
+                tc = (tn - tau) - y;
+                tau = tn;
+            }
+        }
+        if constexpr (BOUNDARY) {
+            const float ef = a.end_f[ray];
+            const int ej = a.end_j[ray];
+            float E = (1.0f - ef) * a.boundary[ej * G + w] + ef * a.boundary[(ej + 1) * G + w];
+            float optical = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) optical = optical + a.end_col[6 * ray + k] * lc[k];
+            optical = optical + a.end_col[6 * ray + 5] * sext;
+            E = E * (expf(-optical) * (a.hits[ray] ? 1.0f : 0.0f));
+#pragma unroll
+            for (int q = 0; q < P; ++q) total[q] = total[q] + (E * pu[q]) / 2.0f;
+        }
+        float *o = a.out + (ray * G + w) * P;
+#pragma unroll
+        for (int q = 0; q < P; ++q) o[q] = total[q];
+    }
+}
+
+}  // namespace sos
+
+// ------------------------------------------------------------------------------------------------ host dispatch
+namespace sos {
+int set_error(const char *fmt, ...);
+
+template <int G, int L, int P, bool FIRST, bool BOUNDARY, bool HALF>
+static int launch_one(const TransportArgs &a, hipStream_t s) {
+    int dev = 0, cus = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int64_t wgs_needed = (a.tiles + 3) / 4;
+    int64_t grid = cus * 8;
+    if (grid > wgs_needed) grid = wgs_needed;
+    grid = ((grid + 7) / 8) * 8;
+    if (grid < 8) grid = 8;
+    hipLaunchKernelGGL((transport_kernel<G, L, P, FIRST, BOUNDARY, HALF>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error("transport launch: %s", hipGetErrorString(e));
+    return SOS_OK;
+}
+
+template <int G, int L, int P>
+static int launch_flags(const TransportArgs &a, hipStream_t s) {
+    const bool first = a.flags & SOS_FIRST, bnd = a.flags & SOS_BOUNDARY, half = a.flags & SOS_MIXED;
+    if (first) return launch_one<G, L, P, true, false, false>(a, s);
+    if (bnd) return half ? launch_one<G, L, P, false, true, true>(a, s) : launch_one<G, L, P, false, true, false>(a, s);
+    return half ? launch_one<G, L, P, false, false, true>(a, s) : launch_one<G, L, P, false, false, false>(a, s);
+}
+
+template <int G, int L>
+static int launch_pairs(const TransportArgs &a, hipStream_t s) {
+    if (a.pairs == 4) return launch_flags<G, L, 4>(a, s);
+    if (a.pairs == 6) return launch_flags<G, L, 6>(a, s);
+    return set_error("pairs must be 4 or 6");
+}
+
+template <int G>
+static int launch_L(const TransportArgs &a, int L, hipStream_t s) {
+    switch (L) {
+        case 8: return launch_pairs<G, 8>(a, s);
+        case 16: return launch_pairs<G, 16>(a, s);
+        case 32: return launch_pairs<G, 32>(a, s);
+        default: return set_error("unsupported L %d (8, 16, 32)", L);
+    }
+}
+
+int transport_dispatch(const TransportArgs &a, int G, int L, hipStream_t s) {
+    switch (G) {
+        case 1: return launch_L<1>(a, L, s);
+        case 16: return launch_L<16>(a, L, s);
+        case 32: return launch_L<32>(a, L, s);
+        default: return set_error("unsupported lane count %d (1, 16, 32)", G);
+    }
+}
+
+}  // namespace sos

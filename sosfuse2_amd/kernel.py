@@ -1,0 +1,195 @@
+"""The backend seam: `Kernel(backend='hip')` owns the device copies of bundles and constants and launches the HIP
+kernels (the role of `src/sosfuse2/solve/source_solver.py:1109-2484` Kernel: setup, launch, transport).
+
+Device memory is PyTorch's (one process, one GPU); every launch runs on torch's current stream. Lanes: a chunk of W
+wavelengths runs on G = 1, 16 or 32 lanes per ray (W <= G), padded by repeating the last lane.
+"""
+
+import math
+
+import numpy as np
+import torch
+
+from . import geometry, hip, layout
+
+LANE_ROWS = hip.LANE_CONSTANTS
+
+
+def lane_count(W):
+    if W == 1:
+        return 1
+    if W <= 16:
+        return 16
+    if W <= 32:
+        return 32
+    raise ValueError(f'E_WAVELENGTHS: {W} lanes exceed 32; split the chunk')
+
+
+def _pad_lanes(a, G):
+    """Pad the last axis to G by repeating its last entry."""
+    a = np.asarray(a)
+    if a.shape[-1] == G:
+        return a
+    idx = np.r_[np.arange(a.shape[-1]), np.full(G - a.shape[-1], a.shape[-1] - 1)]
+    return a[..., idx]
+
+
+class DeviceBundle:
+    """Device copy of a `geometry.Bundle` (the reference's upload_geometry, `solve/ray_geometry.py:256-287`)."""
+
+    def __init__(self, b, device):
+        t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(device)
+        self.host = b
+        self.groups, self.rays, self.nq = b.groups, b.rays, b.nq
+        self.group_off = t(b.group_off, 'i8')
+        self.starts = t(b.starts, 'f8')
+        self.targets = t(b.targets.reshape(-1, 9), 'f4')
+        self.ray_group = t(b.ray_group, 'i4')
+        self.dirs = t(b.dirs, 'f8')
+        self.dloc = t(b.dloc, 'f4')
+        self.weights = t(b.weights, 'f4')
+        self.edge_off = t(b.edge_off, 'i8')
+        self.edges = t(b.edges, 'f8')
+        self.lower = t(b.lower.view('i2'), 'i2')
+        self.end_j = t(b.end_j, 'i4')
+        self.end_f = t(b.end_f, 'f4')
+        self.end_col = t(b.end_col, 'f4')
+        self.hits = t(b.hits, 'u1')
+        self.samples = b.samples
+        self.struct = hip.Bundle(
+            self.groups, self.rays, *(hip.ptr(x) for x in (
+                self.group_off, self.starts, self.targets, self.ray_group, self.dirs, self.dloc, self.weights,
+                self.edge_off, self.edges, self.lower, self.end_j, self.end_f, self.end_col, self.hits)))
+
+    def nbytes(self):
+        return sum(x.numel() * x.element_size() for x in (
+            self.group_off, self.starts, self.targets, self.ray_group, self.dirs, self.dloc, self.weights,
+            self.edge_off, self.edges, self.lower, self.end_j, self.end_f, self.end_col, self.hits))
+
+
+class DeviceConstants:
+    """Device copy of one context's constants for one node count (the reference's Kernel.setup + constants_device)."""
+
+    def __init__(self, ctx, nq, L, G, device):
+        c = ctx.constants(nq)
+        o = ctx.optics
+        t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(device)
+        self.G, self.L, self.nq = G, L, nq
+        self.W = o.W
+        lanes = np.zeros((LANE_ROWS, G), 'f4')
+        lanes[0:5] = _pad_lanes(o.ext, G)
+        lanes[5] = _pad_lanes(o.asca, G)
+        lanes[6] = _pad_lanes(o.g, G)
+        lanes[7] = _pad_lanes(o.mix, G)
+        lanes[8] = _pad_lanes(o.albedo, G)
+        lanes[9] = _pad_lanes(o.strato_ext, G)
+        lanes[10] = _pad_lanes(o.strato_sca, G)
+        lanes[11] = _pad_lanes(o.strato_g, G)
+        self.lane = t(lanes, 'f4')
+        self.gl_u, self.gl_w = t(c['u'], 'f8'), t(c['wu'], 'f8')
+        self.h, self.chi, self.hr = t(c['h'], 'f4'), t(c['chi'], 'f4'), t(c['hr'], 'f4')
+        self.ha, self.sd = t(c['ha'], 'f4'), t(c['strato_density'], 'f4')
+        self.sun_h, self.sun_col = t(c['sun_h'], 'f4'), t(c['sun_col'], 'f4')
+        pools = ctx.pools
+        self.goff = t(pools.off, 'i8')
+        self.gdir = t(pools.loc if len(pools.loc) else np.zeros((1, 3)), 'f4')
+        gv = pools.values(o) if len(pools.loc) else np.zeros((1, o.W))
+        self.gval = t(_pad_lanes(gv, G), 'f4')
+        self.gm = t(_pad_lanes(c['gm'], G), 'f4')
+        self.factor = t(_pad_lanes(ctx.eigen(), G), 'f4')
+        p = c['profile']
+        g = ctx.geo
+        self.struct = hip.Grid(
+            g.nh, g.nc, L, G, nq, int(c['strato']), float(c['top']), p['air_H'], p['aer_H'], p['o3_c'], p['o3_s'],
+            p['no2_c'], p['no2_s'], p['st_c'], p['st_s'],
+            *(hip.ptr(x) for x in (self.gl_u, self.gl_w, self.h, self.chi, self.hr, self.lane)),
+            len(c['sun_h']), c['sun_col'].shape[1], hip.ptr(self.sun_h), hip.ptr(self.sun_col), hip.ptr(self.goff),
+            hip.ptr(self.gdir), hip.ptr(self.gval), hip.ptr(self.gm))
+
+
+class Kernel:
+    """HIP transport kernel of one solve chunk (backend 'hip' only: `source_solver.py:1113-1114` accepts 'numpy' and
+    'cupy'; this build adds 'hip' and runs nothing else)."""
+
+    backend = 'hip'
+
+    def __init__(self, backend='hip', L=16, block_size=256, device='cuda', mixed=False):
+        if backend != 'hip':
+            raise ValueError(f"E_BACKEND: backend {backend!r}; this engine runs 'hip' only")
+        layout.count(L)
+        if L == 24:
+            raise ValueError('E_SH_L: the HIP kernels are built for L = 8, 16 and 32')
+        if not torch.cuda.is_available():
+            raise RuntimeError('E_HIP_DEVICE: no GPU visible to torch')
+        hip.lib()
+        self.L = L
+        self.K = layout.count(L)
+        self.block_size = block_size
+        self.device = torch.device(device)
+        self.mixed = mixed
+        self.xp = torch
+        self._constants = {}
+        self.metrics = dict(samples=0, launches=0)
+
+    def sync(self):
+        torch.cuda.synchronize(self.device)
+
+    def stream(self):
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def host(self, x):
+        return x.detach().cpu().numpy() if torch.is_tensor(x) else np.asarray(x)
+
+    def constants(self, ctx, nq):
+        key = (id(ctx), nq)
+        dc = self._constants.get(key)
+        if dc is None:
+            dc = DeviceConstants(ctx, nq, self.L, lane_count(ctx.W), self.device)
+            self._constants[key] = (ctx, dc)
+            return dc
+        return dc[1]
+
+    def bundle(self, b):
+        return DeviceBundle(b, self.device)
+
+    # ------------------------------------------------------------------------------------------------ launches
+    def plane(self, ctx, field, dc):
+        """Combined SH plane of a source field (m-major per cell)."""
+        K = self.K
+        dt = torch.float16 if self.mixed else torch.float32
+        out = torch.empty((ctx.geo.ns, K, dc.G), dtype=dt, device=self.device)
+        hip.call('sos_make_plane', dc.struct, ctx.rows, hip.ptr(field), hip.ptr(dc.ha), hip.ptr(dc.sd),
+                 int(self.mixed), hip.ptr(out), self.stream())
+        return out
+
+    def radiance(self, db, ctx, field, first=False, boundary=None, pairs=4, plane=None):
+        """Per-ray radiance tensor [rays, G, pairs]."""
+        dc = self.constants(ctx, db.nq)
+        out = torch.empty((db.rays, dc.G, pairs), dtype=torch.float32, device=self.device)
+        flags = (hip.FIRST if first else 0) | (hip.BOUNDARY if boundary is not None else 0)
+        if not first:
+            if plane is None:
+                plane = self.plane(ctx, field, dc)
+            if self.mixed:
+                flags |= hip.MIXED
+        hip.call('sos_transport', dc.struct, db.struct, flags, pairs, ctx.rows,
+                 None if first else hip.ptr(field), None if first else hip.ptr(plane), hip.ptr(boundary),
+                 hip.ptr(out), self.stream())
+        self.metrics['launches'] += 1
+        self.metrics['samples'] += db.samples
+        return out
+
+    def launch(self, db, ctx, field, first=False, phase=0, boundary=None, plane=None):
+        """(moments [groups, 4+K, G], flux [groups, G], query) of one bundle (`source_solver.py:2428-2456`).
+        phase 0: moments and flux; 1: flux only; 2: the per-ray query [rays, G, 4]."""
+        rad = self.radiance(db, ctx, field, first, boundary, 4, plane)
+        if phase == 2:
+            return None, None, rad
+        dc = self.constants(ctx, db.nq)
+        flux = torch.empty((db.groups, dc.G), dtype=torch.float32, device=self.device)
+        moments = None
+        if phase == 0:
+            moments = torch.empty((db.groups, 4 + self.K, dc.G), dtype=torch.float32, device=self.device)
+        hip.call('sos_project', dc.struct, db.struct, hip.ptr(rad), int(phase == 1), hip.ptr(moments), hip.ptr(flux),
+                 self.stream())
+        return moments, flux, None

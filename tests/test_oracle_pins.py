@@ -1,0 +1,198 @@
+"""Pins of the CPU oracle: closed forms, invariants and the reference's own known-answer checks.
+
+Each test names the reference check it restates or the mathematics that fixes the answer. A plausible mistake in the
+oracle (a dropped term, a wrong sign or index, a transposed operand) fails at least one of them.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import arith
+from oracle.arith import Arith, basis, cosine, flush_to_zero, kahan, tree_sum
+from oracle.solve import anderson_weights, stokes
+from oracle import transport as ot
+
+
+def test_ftz_mode_flushes_subnormals():
+    # numpy_reference.py:69-75: a normal input with a subnormal result, and a subnormal input, both give 0
+    tiny = np.float32(np.finfo(np.float32).tiny)
+    with flush_to_zero():
+        assert tiny * np.float32(0.5) == 0
+        assert (np.full(64, tiny, 'f4') * np.float32(0.5))[0] == 0
+    assert tiny * np.float32(0.5) != 0  # restored on exit
+
+
+def test_sh_low_orders():
+    # checks/validate.py:198-201: Y00 = 1/sqrt(4 pi), Y10 at mu=0 is 0, Y10 at mu=-1 is -sqrt(3/(4 pi))
+    y = basis(np.array([0.0, 1.0, -1.0]), 32)
+    np.testing.assert_allclose(y[:, 0], 1 / math.sqrt(4 * math.pi), rtol=1e-7)
+    assert abs(y[0, 1]) < 1e-7
+    np.testing.assert_allclose(y[2, 1], -math.sqrt(3 / (4 * math.pi)), rtol=1e-7)
+
+
+@pytest.mark.parametrize('L', [8, 16, 32])
+def test_sh_orthonormal_on_the_sphere(L):
+    # The cosine-only real harmonics Y_lm(mu) cos(m phi) (sqrt2 for m > 0) are orthonormal over the sphere: a wrong
+    # recurrence constant, seed or sqrt2 factor breaks the Gram matrix.
+    x, w = np.polynomial.legendre.leggauss(2 * L + 4)
+    Y = basis(x, L).astype('f8')
+    K = Y.shape[1]
+    m = np.array([mm for l in range(L + 1) for mm in range(l + 1)])
+    nphi = 4 * L + 8
+    phi = (np.arange(nphi) + 0.5) * 2 * math.pi / nphi
+    cm = np.cos(np.outer(phi, np.arange(L + 1)))  # [phi, m]
+    az = cm[:, m].T @ cm[:, m] * (2 * math.pi / nphi)  # [K, K] azimuthal integrals
+    gram = (Y * w[:, None]).T @ Y * az
+    np.testing.assert_allclose(gram, np.eye(K), atol=2e-6)
+
+
+def test_sh_basis_against_scipy_legendre():
+    # Independent closed form: Y_lm = sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!) P_lm(mu), times sqrt2 for m > 0
+    from scipy.special import lpmv
+
+    mu = np.linspace(-0.95, 0.95, 7)
+    Y = basis(mu, 16).astype('f8')
+    for l in range(17):
+        for m in range(l + 1):
+            k = l * (l + 1) // 2 + m
+            norm = math.sqrt((2 * l + 1) / (4 * math.pi) * math.factorial(l - m) / math.factorial(l + m))
+            ref = norm * lpmv(m, l, mu) * (math.sqrt(2) if m else 1)
+            np.testing.assert_allclose(Y[:, k], ref, rtol=2e-5, atol=1e-7)
+
+
+def test_chebyshev_cosines():
+    cp = np.linspace(-1, 1, 11).astype('f4')
+    cm = cosine(cp, 33).astype('f8')
+    ref = np.cos(np.outer(np.arccos(cp.astype('f8')), np.arange(33)))
+    np.testing.assert_allclose(cm, ref, atol=2e-5)
+
+
+def test_tree_sum_of_33_lanes():
+    # checks/validate.py:203-205: the 32-lane shuffle tree over 0..32 gives 528
+    x = np.arange(33, dtype='f4')[None, :, None]
+    assert tree_sum(x)[0, 0] == 528
+
+
+def test_kahan_recovers_lost_bits():
+    # Kahan's error term carries what a plain FP32 sum drops: 1 + 1e-8 * 1e4 exactly 1.0001 in compensated form
+    total = np.zeros(1, 'f4')
+    corr = np.zeros(1, 'f4')
+    plain = np.float32(1)
+    total[0] = 1
+    for _ in range(10000):
+        total, corr = kahan(total, corr, np.array([1e-8], 'f4'))
+        plain = plain + np.float32(1e-8)
+    assert plain == 1.0
+    assert abs(float(total[0] - corr[0]) - 1.0001) < 2e-7
+
+
+def test_double_single_coordinate_cancellation():
+    # checks/validate.py:166-179: heights at Earth radius and on a near-tangent path to better than 2e-7 km
+    a = Arith(np.float32)
+    x = np.array([6371.0 + 0.000001, 6371.0 + 0.1, 6371.0 + 80.0])
+    height = a.rounded(a.sub(a.split(x), (a.R, np.float32(0))))
+    err = float(np.max(abs(height - (x - 6371.0))))
+    p = np.array([[6371.1, 0.0, 0.0]])
+    d = np.array([[-0.005, math.sqrt(1 - 0.005**2), 0.0]])
+    t = -np.sum(p * d, axis=-1)
+    q = a.add(a.split(p), a.mul(a.split(t[:, None]), a.split(d)))
+    q2 = a.mul(q, q)
+    rr = a.root(a.add(a.add((q2[0][:, 0], q2[1][:, 0]), (q2[0][:, 1], q2[1][:, 1])), (q2[0][:, 2], q2[1][:, 2])))
+    h = a.rounded(a.sub(rr, (a.R, np.float32(0))))
+    expected = np.linalg.norm(p + t[:, None] * d, axis=-1) - 6371.0
+    err = max(err, float(np.max(abs(h - expected))))
+    assert err < 2e-7
+
+
+def test_log_blend_rule():
+    # checks/validate.py:182-196: all corners valid -> geometric mean (3.0); two valid -> 2.5 sqrt(0.6); none -> linear
+    F = np.float32
+    traces = np.array([[1.0, 9.0, 1.0, 9.0], [1.0, 9.0, 0.0, 0.0], [0.0, 0.0, 0.0, 0.0]], 'f4')
+    value = np.zeros((3, 1, 4), 'f4')
+    value[:, 0, 0] = traces.mean(axis=1)
+    logs = np.zeros((3, 1), 'f4')
+    wg = np.zeros_like(logs)
+    am = np.zeros_like(logs)
+    for corner in range(4):
+        ot.trace_accumulate(traces[:, corner, None], F(0.25), logs, wg, am, F)
+    out = ot.trace_scale(value, logs, wg, am, F)
+    np.testing.assert_allclose(out[:, 0, 0], [3.0, 2.5 * math.sqrt(0.6), 0.0], atol=2e-6)
+
+
+def test_anderson_weights_equal_columns():
+    # tests/test_transport_fp32.py:179-185: equal residual columns give weights 1/4
+    f = np.full((32, 4), 3, 'f4')
+    w = anderson_weights(f.T @ f)
+    assert w.dtype == np.float32
+    np.testing.assert_allclose(w, 0.25, rtol=1e-3)
+
+
+def test_anderson_weights_minimise_the_residual():
+    # The weights solve min |F w| subject to sum w = 1 (the Lagrange conditions of the regularised Gram system)
+    rng = np.random.default_rng(3)
+    F = rng.normal(size=(50, 3))
+    w = anderson_weights(F.T @ F).astype('f8')
+    G = F.T @ F
+    ref = np.linalg.solve(G, np.ones(3))
+    ref /= ref.sum()
+    np.testing.assert_allclose(w, ref, rtol=1e-5)
+    for _ in range(20):  # no other affine combination has a smaller residual
+        v = ref + rng.normal(size=3) * 1e-3
+        v /= v.sum()
+        assert np.linalg.norm(F @ v) >= np.linalg.norm(F @ ref) - 1e-12
+
+
+def test_anderson_guard_rejects_large_weights():
+    # source_iteration.py:31: weights with max |w| >= 8 are rejected (the plain fixed-point step is taken).
+    # G = [[1, a], [a, b]] gives w proportional to (b - a, 1 - a): a = 0.99, b = 0.9802 -> w = (-49, 50).
+    assert anderson_weights(np.array([[1.0, 0.99], [0.99, 0.9802]])) is None
+    w = anderson_weights(np.array([[1.0, 0.0], [0.0, 1.0]]))
+    np.testing.assert_allclose(w, 0.5, rtol=1e-6)
+
+
+def test_hg_moments_are_legendre_projections():
+    # docs/MODEL.md:236-242: the HG phase function has Legendre moments (2l+1) g^l
+    from sosfuse2_amd.atmosphere import hg_moments
+
+    g = 0.6
+    x, w = np.polynomial.legendre.leggauss(400)
+    p = (1 - g * g) / (4 * math.pi * (1 + g * g - 2 * g * x) ** 1.5)
+    chi = hg_moments(g, 12)
+    for l in range(13):
+        Pl = np.polynomial.legendre.Legendre.basis(l)(x)
+        # p = sum chi_l P_l / (4 pi)  =>  chi_l = 2 pi (2l+1) int p P_l
+        val = 2 * math.pi * (2 * l + 1) * np.sum(w * p * Pl)
+        np.testing.assert_allclose(val, chi[l], rtol=1e-9)
+
+
+def test_hg_phase_normalised():
+    x, w = np.polynomial.legendre.leggauss(400)
+    for g in (0.0, 0.6, 0.8):
+        v = ot.hg(x[:, None].astype('f8'), np.array([g]), np.float64)
+        assert abs(2 * math.pi * np.sum(w * v[:, 0]) - 1) < 1e-9
+
+
+def test_stokes_of_rayleigh_single_scattering():
+    # Rayleigh single scattering of unpolarised sunlight (docs/MODEL.md:214-221): with the depolarisation factor
+    # D = (1 - delta)/(1 + delta/2) the degree of polarisation is D sin^2 T / (D (1 + cos^2 T) + 4 (1 - D)/3).
+    # The direct-term tensor of the transport, D 3/(8pi) (P - e e)/2 + (1 - D) 1/(8pi) P, fed through the Stokes
+    # projection, must give it for every view.
+    D = (1 - 0.03) / (1 + 0.015)
+    rng = np.random.default_rng(5)
+    el = rng.uniform(0.05, 1.5, 40)
+    az = rng.uniform(0, math.pi, 40)
+    sun = np.array([math.cos(0.3), 0.0, math.sin(0.3)])  # sun direction in the observer frame
+    d = np.stack([np.cos(el) * np.cos(az), np.cos(el) * np.sin(az), np.sin(el)], -1)
+    P = np.eye(3)[None] - d[:, :, None] * d[:, None, :]
+    e = sun[None] - d * (d @ sun)[:, None]
+    T = D * 3 / (8 * math.pi) * (P - e[:, :, None] * e[:, None, :]) / 2 + (1 - D) / (8 * math.pi) * P
+    comp = np.stack([T[:, 0, 0], T[:, 1, 1], T[:, 2, 2], T[:, 0, 2], T[:, 0, 1], T[:, 1, 2]], -1)[:, None, :]
+    s = stokes(comp, np.stack([el, az], -1))[:, 0]
+    cos = d @ sun
+    dop = np.hypot(s[:, 1], s[:, 2]) / s[:, 0]
+    ref = D * (1 - cos**2) / (D * (1 + cos**2) + 4 * (1 - D) / 3)
+    np.testing.assert_allclose(dop, ref, rtol=1e-10)
+    # the phase function: I = 3/(16 pi) D (1 + cos^2) + (1 - D)/(4 pi), normalised over the sphere
+    np.testing.assert_allclose(s[:, 0], D * 3 / (16 * math.pi) * (1 + cos**2) + (1 - D) / (4 * math.pi), rtol=1e-12)
