@@ -96,10 +96,11 @@ typedef struct sos_grid {
 
 /* Radiance tensor of every ray: out_dev [rays][lanes][pairs] (pairs 4: xx yy zz xz; 6: + xy yz), FP32.
  * field_dev: source [cells][rows][lanes] (rows 0..3 read); plane_dev: combined SH plane [cells][K][lanes]
- * (FP32, or FP16 with SOS_MIXED); boundary_dev: [nc][lanes] ground radiance (with SOS_BOUNDARY). */
+ * (FP32, or FP16 with SOS_MIXED); boundary_dev: [nc][lanes] ground radiance (with SOS_BOUNDARY).
+ * counters_dev (may be NULL): uint64 [2] += quadrature nodes evaluated and SH corner refills, summed over rays. */
 int sos_transport(const sos_grid *grid, const sos_bundle *bundle, int32_t flags, int32_t pairs, int32_t rows,
                   const float *field_dev, const void *plane_dev, const float *boundary_dev, float *out_dev,
-                  void *stream);
+                  uint64_t *counters_dev, void *stream);
 
 /* Moments [groups][4 + K][lanes] and ground flux [groups][lanes] of the per-ray radiances rad_dev
  * [rays][lanes][4] (the bundle's groups are the destination cells). ground_only writes the flux only. */

@@ -60,9 +60,9 @@ class Scope:
         self.kernel = kernel
         self.state = profile_state
         self.sun = geometry.sun_table(profile_state, self.grid.domain, int(cfg.get('sun_scale', 1)))
-        self.pools = geometry.ground_pools(profile_state, self.grid, self.sun)
-        self.packs = [geometry.with_end_columns(profile_state, geometry.cell_pack(self.grid, i), self.grid.domain)
-                      for i in range(self.grid.ns)]
+        workers = int(cfg.get('workers', geometry.default_workers()))
+        self.pools = geometry.ground_pools(profile_state, self.grid, self.sun, workers=workers)
+        self.packs = geometry.cell_packs(profile_state, self.grid, workers=workers)
         self.geometry = None
         self.prepare_seconds = time.perf_counter() - t
 

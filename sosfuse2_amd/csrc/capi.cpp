@@ -70,7 +70,7 @@ int sos_device_info(int32_t *cus, int32_t *xcds, char *arch, int32_t arch_len) {
 
 int sos_transport(const sos_grid *g, const sos_bundle *b, int32_t flags, int32_t pairs, int32_t rows,
                   const float *field_dev, const void *plane_dev, const float *boundary_dev, float *out_dev,
-                  void *stream) {
+                  uint64_t *counters_dev, void *stream) {
     if (!g || !b || !out_dev) return arg_error("sos_transport: null grid, bundle or output");
     if (!lanes_ok(g->lanes)) return arg_error("sos_transport: lanes %d not in {1, 16, 32}", g->lanes);
     if (!order_ok(g->L)) return arg_error("sos_transport: L %d not in {8, 16, 32}", g->L);
@@ -100,6 +100,7 @@ int sos_transport(const sos_grid *g, const sos_bundle *b, int32_t flags, int32_t
     a.dloc = b->dloc_dev; a.edge_off = b->edge_off_dev; a.edges = b->edges_dev; a.lower = b->lower_dev;
     a.end_j = b->end_j_dev; a.end_f = b->end_f_dev; a.end_col = b->end_col_dev; a.hits = b->hits_dev;
     a.field = field_dev; a.plane = plane_dev; a.boundary = boundary_dev; a.out = out_dev;
+    a.counters = (unsigned long long *)counters_dev;
     return transport_dispatch(a, g->lanes, g->L, (hipStream_t)stream);
 }
 

@@ -86,6 +86,7 @@ struct TransportArgs {
     const void *plane;
     const float *boundary;
     float *out;
+    unsigned long long *counters;
 };
 
 struct ProjectArgs {

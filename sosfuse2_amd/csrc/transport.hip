@@ -147,6 +147,7 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
         float tau = 0.0f, tc = 0.0f;
         float cache[4][L + 1];
         int tags[4] = {-1, -1, -1, -1};
+        unsigned nodes = 0, refills = 0;
         const int64_t e0 = a.edge_off[ray];
         const int nseg = (int)(a.edge_off[ray + 1] - e0) - 1;
         const unsigned bits = a.lower[ray];
@@ -162,6 +163,7 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
             float bprev = extinction(a.shape, lc, sext, node_height(r_, a.top));
             float acc = 0.0f;
             for (int step = 0; step < a.nq; ++step) {
+                ++nodes;
                 const double u = a.gl_u[step], wu = a.gl_w[step];
                 const double v = lower ? u : 1.0 - u;
                 const double t = lower ? sa + len * (u * u) : sb - len * (v * v);
@@ -333,6 +335,7 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
                         const int cell = c == 0 ? cells[0] : c == 1 ? cells[1] : c == 2 ? cells[2] : cells[3];
                         const int tag = c == 0 ? tags[0] : c == 1 ? tags[1] : c == 2 ? tags[2] : tags[3];
                         if (tag != cell) {
+                            ++refills;
                             float fresh[L + 1];
                             refill<L, G, HALF>(a.plane, cell, w, mu, st, fresh);
 #pragma unroll
@@ -427,6 +430,10 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
             E = E * (expf(-optical) * (a.hits[ray] ? 1.0f : 0.0f));
 #pragma unroll
             for (int q = 0; q < P; ++q) total[q] = total[q] + (E * pu[q]) / 2.0f;
+        }
+        if (a.counters && w == 0) {
+            atomicAdd(&a.counters[0], (unsigned long long)nodes);
+            atomicAdd(&a.counters[1], (unsigned long long)refills);
         }
         float *o = a.out + (ray * G + w) * P;
 #pragma unroll

@@ -475,11 +475,10 @@ class GroundPools:
         return gd, gvp
 
 
-def ground_pools(state, grid, sun, qscale=1):
-    """Build the ground pools of every cell (`engine_core.py:103-147`)."""
+def _pools_of_cells(state, grid, sun, cells, qscale):
     domain = grid.domain
-    locs, facs, cols, off = [], [], [], [0]
-    for cell in range(grid.ns):
+    locs, facs, cols, counts = [], [], [], []
+    for cell in cells:
         ih, ic = divmod(cell, grid.nc)
         r = R + grid.h[ih]
         loc, w = ground_incoming(r, grid.chi[ic], qscale)
@@ -502,13 +501,61 @@ def ground_pools(state, grid, sun, qscale=1):
         locs.append(loc[take])
         facs.append(factor[take])
         cols.append(col)
-        off.append(off[-1] + len(take))
+        counts.append(len(take))
+    return locs, facs, cols, counts
+
+
+def _cells_job(args):
+    kind, state, grid, sun, cells, qscale = args
+    if kind == 'pools':
+        return _pools_of_cells(state, grid, sun, cells, qscale)
+    return [with_end_columns(state, cell_pack(grid, c), grid.domain) for c in cells]
+
+
+def _parallel(kind, state, grid, sun, workers, qscale=1):
+    """Run a per-cell preparation over a spawn process pool (numpy only in the workers), in cell order."""
+    cells = list(range(grid.ns))
+    if workers <= 1 or grid.ns < 64:
+        return [_cells_job((kind, state, grid, sun, cells, qscale))]
+    from concurrent.futures import ProcessPoolExecutor
+    import multiprocessing as mp
+
+    chunk = max(1, (grid.ns + 4 * workers - 1) // (4 * workers))
+    jobs = [(kind, state, grid, sun, cells[i : i + chunk], qscale) for i in range(0, grid.ns, chunk)]
+    with ProcessPoolExecutor(workers, mp_context=mp.get_context('spawn')) as pool:
+        return list(pool.map(_cells_job, jobs))
+
+
+def default_workers():
+    import os
+
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def ground_pools(state, grid, sun, qscale=1, workers=1):
+    """Build the ground pools of every cell (`engine_core.py:103-147`)."""
+    parts = _parallel('pools', state, grid, sun, workers, qscale)
+    locs, facs, cols, counts = [], [], [], []
+    for a, b, c, d in parts:
+        locs += a
+        facs += b
+        cols += c
+        counts += d
     return GroundPools(
-        off=np.array(off, 'i8'),
+        off=np.r_[0, np.cumsum(counts)].astype('i8'),
         loc=np.concatenate(locs) if locs else np.zeros((0, 3)),
         factor=np.concatenate(facs) if facs else np.zeros(0),
         column=np.concatenate(cols) if cols else np.zeros((0, 6)),
     )
+
+
+def cell_packs(state, grid, workers=1):
+    """Incident-ray packs with end columns of every cell, in cell order."""
+    return [p for part in _parallel('packs', state, grid, None, workers) for p in part]
 
 
 # ------------------------------------------------------------------------------------------------------- bundles

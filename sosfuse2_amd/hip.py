@@ -32,7 +32,7 @@ class Grid(ctypes.Structure):
 
 
 _SIGNATURES = {
-    'sos_transport': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), c_i32, c_i32, c_i32, P, P, P, P, P],
+    'sos_transport': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), c_i32, c_i32, c_i32, P, P, P, P, P, P],
     'sos_project': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, c_i32, P, P, P],
     'sos_compose': [ctypes.POINTER(Grid), c_i32, P, P, P, P, P],
     'sos_make_plane': [ctypes.POINTER(Grid), c_i32, P, P, P, c_i32, P, P],

@@ -130,6 +130,8 @@ class Kernel:
         self.xp = torch
         self._constants = {}
         self.metrics = dict(samples=0, launches=0)
+        self.timed = None  # list of (name, start event, end event) while timing is on
+        self.counters = None  # device uint64 [2] while counting is on
 
     def sync(self):
         torch.cuda.synchronize(self.device)
@@ -172,9 +174,17 @@ class Kernel:
                 plane = self.plane(ctx, field, dc)
             if self.mixed:
                 flags |= hip.MIXED
+        name = 'first' if first else ('warm' if boundary is not None else ('query' if pairs == 6 or db.groups != ctx.geo.nc else 'ground'))
+        ev = None
+        if self.timed is not None:
+            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            ev[0].record()
         hip.call('sos_transport', dc.struct, db.struct, flags, pairs, ctx.rows,
                  None if first else hip.ptr(field), None if first else hip.ptr(plane), hip.ptr(boundary),
-                 hip.ptr(out), self.stream())
+                 hip.ptr(out), hip.ptr(self.counters), self.stream())
+        if ev is not None:
+            ev[1].record()
+            self.timed.append((name, ev[0], ev[1]))
         self.metrics['launches'] += 1
         self.metrics['samples'] += db.samples
         return out

@@ -1,0 +1,262 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the same inputs.
+
+Tolerances: the HIP kernels and the FP32 oracle run the same operation sequence in FP32 with flushed denormals; the
+differences are FMA contraction, FP64 (kernel) versus double-single (oracle) node coordinates, and the FP32 versus
+FP64 recurrence of the harmonics. Radiances and moments agree to 1e-5 relative to the largest entry; full solves to
+the iteration tolerance. Against the FP64 twin the FP32 table is within 1e-4 relative on every entry.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from sosfuse2_amd import api, atmosphere, geometry, hip
+from sosfuse2_amd.solver import Geometry, observer_bundle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, 'f8')
+    b = np.asarray(b, 'f8')
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def padded(a, G):
+    a = np.asarray(a)
+    if a.shape[-1] == G:
+        return a
+    return np.concatenate([a, np.repeat(a[..., -1:], G - a.shape[-1], -1)], -1)
+
+
+def setup_case(W, L=16, nr=5, state=None, nmu=1, nphi=2, path_n=4):
+    from oracle.solve import OracleSolver
+
+    cfg = dict(nr=nr, chi_grid='tiny', nmu=nmu, nphi=nphi, path_n=path_n, L=L, top_km=100.0, observer_km=0.1,
+               query_n=6, anderson_depth=4)
+    wl = atmosphere.log_spaced_wavelengths(W, 420, 680) if W > 1 else [550.0]
+    st = state or atmosphere.config0_state()
+    s, scope = api.create_configured(None, cfg, wl, 'hip', st)
+    orc = OracleSolver(s, np.float32, depth=4, packs=scope.packs)
+    return s, scope, orc
+
+
+@pytest.fixture(scope='module', params=[(1, 16), (3, 16), (20, 8)], ids=['W1-G1', 'W3-G16', 'W20-G32'])
+def case(request, gpu):
+    W, L = request.param
+    print(f'case W={W} L={L}', flush=True)
+    return setup_case(W, L)
+
+
+def test_first_light_moments(case):
+    s, scope, orc = case
+    geo = Geometry(s, scope.packs)
+    m, f = geo.evaluate(s, None, first=True, phase=0)
+    mo, fo = orc.evaluate(np.zeros((s.geo.ns, s.rows, s.W), 'f4'), first=True, phase=0)
+    assert rel(m.cpu().numpy()[..., : s.W], mo) < 1e-5
+    assert rel(f.cpu().numpy()[..., : s.W], fo) < 1e-5
+
+
+def test_warm_moments_with_boundary(case):
+    s, scope, orc = case
+    geo = Geometry(s, scope.packs)
+    rng = np.random.default_rng(1)
+    field = (np.abs(rng.normal(size=(s.geo.ns, s.rows, s.W))) * 1e-3).astype('f4')
+    field[:, 10:] *= 0.1
+    G = s.kernel.constants(s, s.geo.path_n).G
+    bnd = (np.abs(rng.normal(size=(s.geo.nc, s.W))) * 1e-2).astype('f4')
+    fd = torch.from_numpy(np.ascontiguousarray(padded(field, G))).cuda()
+    bd = torch.from_numpy(np.ascontiguousarray(padded(bnd, G))).cuda()
+    m, _ = geo.evaluate(s, fd, phase=0, boundary=bd)
+    mo, _ = orc.evaluate(field, phase=0, boundary=bnd.astype('f8'))
+    mh = m.cpu().numpy()[..., : s.W]
+    assert rel(mh[:, :4], mo[:, :4]) < 1e-5
+    assert rel(mh[:, 4:], mo[:, 4:]) < 1e-5
+    _, f = geo.evaluate(s, fd, phase=1, ground_only=True)
+    _, fo = orc.evaluate(field, phase=1, ground_only=True)
+    assert rel(f.cpu().numpy()[..., : s.W], fo) < 1e-5
+
+
+def test_query_six_components(case):
+    from oracle.transport import PAIRS6, transport
+
+    s, scope, orc = case
+    G = s.kernel.constants(s, s.geo.query_n).G
+    rng = np.random.default_rng(2)
+    field = (np.abs(rng.normal(size=(s.geo.ns, s.rows, s.W))) * 1e-3).astype('f4')
+    fd = torch.from_numpy(np.ascontiguousarray(padded(field, G))).cuda()
+    el, az, _ = s.box_directions()
+    _, db = observer_bundle(s, [-3.0, -9.0], el, az)
+    cat = geometry.concat_packs(s.query_packs(-3.0, el, az) + s.query_packs(-9.0, el, az))
+    for first in (True, False):
+        r = s.kernel.radiance(db, s, None if first else fd, first, None, 6).cpu().numpy()[:, : s.W]
+        ro = transport(cat, orc.cq, field, first, None, PAIRS6)
+        assert rel(r, ro) < 2e-5, first
+
+
+def test_solve_and_table_against_fp32_oracle(gpu):
+    s, scope, orc = setup_case(3, 16)
+    solver = scope.solver(s, log=None)
+    x, info = solver.solve(40, 1e-5)
+    xo, io = orc.solve(40, 1e-5)
+    assert info['evaluations'] == io['evaluations']
+    xh = x.cpu().numpy()[..., : s.W]
+    assert rel(xh, xo) < 1e-5
+    hs = [-3.0, -6.0]
+    el = 90 - (np.arange(4) + 0.5) * 90 / 4
+    az = (np.arange(6) + 0.5) * 180 / 6
+    tab, _ = api.make_table(s, x, hs, el, az, log=None)
+    tabo = orc.table(xo, hs, el, az)
+    assert tab.shape == (2, 4, 6, 3, 3)
+    I, Io = tab[..., 0], tabo[..., 0]
+    assert np.max(np.abs(I - Io) / np.abs(Io)) < 1e-4
+    for k in (1, 2):
+        assert np.max(np.abs(tab[..., k] - tabo[..., k]) / np.abs(Io)) < 1e-4
+
+
+def test_fp32_table_within_1e4_of_fp64(gpu):
+    """The metric's accuracy qualifier on a small grid: every I entry of the HIP FP32 table is within 1e-4 relative
+    of the FP64 twin's converged table (same discretisation, tolerance 1e-6 for the FP64 solve)."""
+    from oracle.solve import OracleSolver
+
+    s, scope, orc = setup_case(2, 16)
+    solver = scope.solver(s, log=None)
+    x, _ = solver.solve(40, 1e-5)
+    o64 = OracleSolver(s, np.float64, depth=4, packs=scope.packs)
+    x64, _ = o64.solve(60, 1e-7)
+    hs = [-3.0, -6.0]
+    el = 90 - (np.arange(4) + 0.5) * 90 / 4
+    az = (np.arange(6) + 0.5) * 180 / 6
+    tab, _ = api.make_table(s, x, hs, el, az, log=None)
+    t64 = o64.table(x64, hs, el, az)
+    err = np.max(np.abs(tab[..., 0] - t64[..., 0]) / np.abs(t64[..., 0]))
+    print(f'max relative error of I vs FP64: {err:.3e}')
+    assert err < 1e-4
+
+
+def test_projection_matches_numpy(gpu):
+    """The MFMA projection (16x16x4 f32 lane layout) against an FP64 NumPy contraction of random radiances."""
+    from oracle.arith import basis, cosine
+
+    for W in (1, 7, 24):
+        s, scope, orc = setup_case(W, 16, nr=3)
+        geo = Geometry(s, scope.packs)
+        dc = s.kernel.constants(s, s.geo.path_n)
+        rng = np.random.default_rng(W)
+        R = geo.bundle.rays
+        rad = rng.normal(size=(R, dc.G, 4)).astype('f4')
+        moments = torch.empty((s.geo.ns, 4 + s.K, dc.G), dtype=torch.float32, device='cuda')
+        flux = torch.empty((s.geo.ns, dc.G), dtype=torch.float32, device='cuda')
+        rd = torch.from_numpy(rad).cuda()
+        hip.call('sos_project', dc.struct, geo.bundle.struct, hip.ptr(rd), 0, hip.ptr(moments), hip.ptr(flux),
+                 s.kernel.stream())
+        b = geo.host_bundle
+        Y = basis(b.dirs[:, 2], 16).astype('f8')
+        st = np.sqrt(np.maximum(0, 1 - b.dirs[:, 2] ** 2))
+        cp = np.where(st > 1e-12, b.dirs[:, 0] / np.maximum(st, 1e-30), 1.0)
+        cm = cosine(cp.astype('f4'), 17).astype('f8')
+        m = np.array([mm for l in range(17) for mm in range(l + 1)])
+        Z = Y * cm[:, m]
+        tr = (rad[..., 0].astype('f8') + rad[..., 1]) + rad[..., 2]
+        V = tr * b.weights[:, None]
+        mh = moments.cpu().numpy()
+        for c in range(s.geo.ns):
+            r0, r1 = b.group_off[c], b.group_off[c + 1]
+            ref = Z[r0:r1].T @ V[r0:r1]
+            scale = np.abs(Z[r0:r1]).T @ np.abs(V[r0:r1]) + 1e-30
+            assert np.max(np.abs(mh[c, 4:] - ref) / scale.max()) < 1e-5
+            ref4 = np.einsum('rwk,r->kw', rad[r0:r1].astype('f8'), b.weights[r0:r1])
+            assert np.max(np.abs(mh[c, :4] - ref4)) < 1e-5 * max(1.0, np.abs(ref4).max())
+
+
+def test_device_anderson_pieces_match_host(gpu):
+    """Residual scaling, FP64 Gram column, FP64 solve with guard and mixing on the device against NumPy."""
+    from oracle.solve import anderson_weights
+
+    rng = np.random.default_rng(7)
+    C, rows, G, D, nobs = 37, 19, 16, 4, 5
+    n = C * rows + nobs
+    gs = [rng.normal(size=(C, rows, G)).astype('f4') for _ in range(D)]
+    xs = [rng.normal(size=(C, rows, G)).astype('f4') for _ in range(D)]
+    obs = rng.normal(size=(G, nobs)).astype('f4')
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    fring = torch.zeros((D, G, n), dtype=torch.float32, device='cuda')
+    gram = torch.zeros((G, D, D), dtype=torch.float64, device='cuda')
+    work = torch.empty(C * G + G, dtype=torch.float32, device='cuda')
+    gwork = torch.empty(1024 * G * D, dtype=torch.float64, device='cuda')
+    st = torch.cuda.current_stream().cuda_stream
+    fh = []
+    for slot in range(D):
+        hip.call('sos_anderson_residual', C, rows, G, hip.ptr(dev(gs[slot])), hip.ptr(dev(xs[slot])), hip.ptr(dev(obs)),
+                 nobs, hip.ptr(work), hip.ptr(fring[slot]), st)
+        torch.cuda.synchronize()
+        f = np.zeros((G, n), 'f4')
+        for w in range(G):
+            gw = gs[slot][:, :, w]
+            norm = np.linalg.norm(gw.astype('f8'), axis=1).astype('f4')
+            scale = np.maximum(norm, norm.max() * np.float32(1e-8))
+            f[w, : C * rows] = ((gw - xs[slot][:, :, w]) / scale[:, None]).ravel()
+            f[w, C * rows :] = obs[w]
+        np.testing.assert_allclose(fring[slot].cpu().numpy(), f, rtol=2e-6, atol=1e-7)
+        fh.append(f)
+        filled = (1 << (slot + 1)) - 1
+        hip.call('sos_anderson_gram', n, G, D, slot, filled, hip.ptr(fring), hip.ptr(gwork), hip.ptr(gram), st)
+    torch.cuda.synchronize()
+    F = np.stack(fh).astype('f8')  # [D, G, n]
+    ref = np.einsum('agn,bgn->gab', F, F)
+    np.testing.assert_allclose(gram.cpu().numpy(), ref, rtol=1e-10)
+    order = [2, 3, 0, 1]
+    od = torch.tensor(order, dtype=torch.int32, device='cuda')
+    wts = torch.empty((G, D), dtype=torch.float32, device='cuda')
+    hip.call('sos_anderson_solve', G, D, D, hip.ptr(od), hip.ptr(gram), 1 << 5, hip.ptr(wts), st)
+    x = torch.from_numpy(np.ascontiguousarray(gs[1])).cuda()
+    ring = torch.from_numpy(np.ascontiguousarray(np.stack(gs))).cuda()
+    hip.call('sos_anderson_mix', C, rows, G, D, D, hip.ptr(ring), hip.ptr(od), hip.ptr(wts), hip.ptr(x), st)
+    torch.cuda.synchronize()
+    w = wts.cpu().numpy()
+    xm = x.cpu().numpy()
+    assert math.isnan(w[5, 0])  # frozen lane
+    np.testing.assert_array_equal(xm[..., 5], gs[1][..., 5])
+    for lane in (0, 3, 15):
+        sub = ref[lane][np.ix_(order, order)]
+        wr = anderson_weights(sub)
+        if wr is None:
+            assert math.isnan(w[lane, 0])
+            continue
+        np.testing.assert_allclose(w[lane], wr, rtol=1e-5, atol=1e-6)
+        mix = sum(np.float32(wr[j]) * gs[order[j]][..., lane] for j in range(D))
+        np.testing.assert_allclose(xm[..., lane], mix, rtol=1e-5, atol=1e-5)
+
+
+def test_source_metrics(gpu):
+    rng = np.random.default_rng(9)
+    G = 16
+    g = rng.normal(size=(1000, G)).astype('f4')
+    x = rng.normal(size=(1000, G)).astype('f4')
+    g[17, 3] = np.nan
+    g[40, 5] = np.inf
+    out = torch.empty((3, G), dtype=torch.float32, device='cuda')
+    frozen = 1 << 7
+    hip.call('sos_source_metrics', 1000, G, hip.ptr(torch.from_numpy(g).cuda()), hip.ptr(torch.from_numpy(x).cuda()),
+             frozen, hip.ptr(out), torch.cuda.current_stream().cuda_stream)
+    m = out.cpu().numpy()
+    assert m[0, 3] == 1 and m[0, 5] == 1 and m[0].sum() == 2
+    for w in (0, 1, 7, 15):
+        v = x[:, w] if w == 7 else g[:, w]
+        assert m[1, w] == np.max(np.abs(v))
+        assert m[2, w] == np.max(np.abs(v - x[:, w]))
+
+
+def test_rejects_bad_arguments(gpu):
+    s, scope, orc = setup_case(1, 16, nr=3)
+    dc = s.kernel.constants(s, s.geo.path_n)
+    geo = Geometry(s, scope.packs)
+    out = torch.empty((geo.bundle.rays, 1, 4), dtype=torch.float32, device='cuda')
+    with pytest.raises(RuntimeError, match='pairs'):
+        hip.call('sos_transport', dc.struct, geo.bundle.struct, 0, 5, s.rows, None, None, None, hip.ptr(out), None,
+                 s.kernel.stream())
+    with pytest.raises(RuntimeError, match='warm transport needs field'):
+        hip.call('sos_transport', dc.struct, geo.bundle.struct, 0, 4, s.rows, None, None, None, hip.ptr(out), None,
+                 s.kernel.stream())
