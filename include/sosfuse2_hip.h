@@ -51,6 +51,7 @@ extern "C" {
 #define SOS_FIRST 1     /* first light (single scattering of the direct and ground-reflected sun) */
 #define SOS_BOUNDARY 2  /* add the attenuated ground boundary term at ray ends that hit the ground */
 #define SOS_MIXED 4     /* SH plane stored as FP16 (mixed precision) */
+#define SOS_RECORDS 8   /* read per-(node, corner) SH values from records_dev instead of reconstructing them */
 
 typedef struct sos_bundle {
     int64_t groups;               /* destination cells or query groups */
@@ -69,6 +70,7 @@ typedef struct sos_bundle {
     const float *end_f_dev;       /* [rays] fraction in that interval */
     const float *end_col_dev;     /* [rays][6] slant columns to the ground (5 components + sulfate) */
     const uint8_t *hits_dev;      /* [rays] 1 when the ray ends on the ground */
+    const int64_t *node_off_dev;  /* [rays + 1] first quadrature node of every ray (records mode, from the plan) */
 } sos_bundle;
 
 typedef struct sos_grid {
@@ -97,10 +99,26 @@ typedef struct sos_grid {
 /* Radiance tensor of every ray: out_dev [rays][lanes][pairs] (pairs 4: xx yy zz xz; 6: + xy yz), FP32.
  * field_dev: source [cells][rows][lanes] (rows 0..3 read); plane_dev: combined SH plane [cells][K][lanes]
  * (FP32, or FP16 with SOS_MIXED); boundary_dev: [nc][lanes] ground radiance (with SOS_BOUNDARY).
- * counters_dev (may be NULL): uint64 [2] += quadrature nodes evaluated and SH corner refills, summed over rays. */
+ * counters_dev (may be NULL): uint64 [2] += quadrature nodes evaluated and SH corner refills, summed over rays.
+ * records_dev (SOS_RECORDS): [nodes][4][lanes] SH values of every (node, corner) from sos_sh_records; the plane is
+ * then not read. */
 int sos_transport(const sos_grid *grid, const sos_bundle *bundle, int32_t flags, int32_t pairs, int32_t rows,
-                  const float *field_dev, const void *plane_dev, const float *boundary_dev, float *out_dev,
-                  uint64_t *counters_dev, void *stream);
+                  const float *field_dev, const void *plane_dev, const float *boundary_dev, const float *records_dev,
+                  float *out_dev, uint64_t *counters_dev, void *stream);
+
+/* Visit plan of a bundle (fixed by its geometry): counts_dev [rays] = quadrature nodes of every ray; then with
+ * node_off_dev = exclusive prefix sum of the counts ([rays + 1]), every node's cell (ir * nc + j, its corner 0),
+ * the ray's mu and the node's cos(phi). The caller sorts the visits (node * 4 + corner) by cell. */
+int sos_plan_count(const sos_grid *grid, const sos_bundle *bundle, int64_t *counts_dev, void *stream);
+int sos_plan_nodes(const sos_grid *grid, const sos_bundle *bundle, const int64_t *node_off_dev, int32_t *node_cell_dev,
+                   float *node_mu_dev, float *node_cp_dev, void *stream);
+
+/* SH values of every visit as an MFMA contraction per source cell: records[id][w] = sum_k Z[id][k] plane[cell][k][w]
+ * with Z the basis row of the visit (the replaced per-corner sum of numpy_reference.py:579-600). Work item i covers
+ * the visits vis_dev[v0[i] .. v1[i]) of cell item_cell[i]; vis_dev holds visit ids node * 4 + corner. */
+int sos_sh_records(const sos_grid *grid, const void *plane_dev, int32_t half, int64_t items, const int32_t *item_cell_dev,
+                   const int64_t *item_v0_dev, const int64_t *item_v1_dev, const int32_t *vis_dev,
+                   const float *node_mu_dev, const float *node_cp_dev, float *records_dev, void *stream);
 
 /* Moments [groups][4 + K][lanes] and ground flux [groups][lanes] of the per-ray radiances rad_dev
  * [rays][lanes][4] (the bundle's groups are the destination cells). ground_only writes the flux only. */

@@ -77,14 +77,14 @@ class Scope:
 
 
 def create_configured(output, cfg, wavelengths, backend, state, block_size=256, geometry_cache=None, *,
-                      share_sun_geometry=True, sun_scale=1, scope=None, mixed=False):
+                      share_sun_geometry=True, sun_scale=1, scope=None, mixed=False, records=True):
     """Solver context `s` over `wavelengths` (nm) for one state, and its scope (`solve/build.py:14-118`).
     `scope` reuses another state's preparation on the same grid and profile shapes."""
     if output is not None:
         Path(output).mkdir(parents=True, exist_ok=True)
     if scope is None:
         cfg = dict(cfg, sun_scale=sun_scale)
-        kernel = Kernel(backend=backend, L=int(cfg['L']), block_size=block_size, mixed=mixed)
+        kernel = Kernel(backend=backend, L=int(cfg['L']), block_size=block_size, mixed=mixed, records=records)
         scope = Scope(cfg, state, kernel)
     wl = np.asarray(wavelengths, float)
     if wl.ndim != 1 or len(wl) < 1:

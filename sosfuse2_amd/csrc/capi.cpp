@@ -29,6 +29,11 @@ static int arg_error(const char *fmt, ...) {
 
 int transport_dispatch(const TransportArgs &a, int G, int L, hipStream_t s);
 int project_dispatch(const ProjectArgs &a, int G, int L, hipStream_t s);
+int plan_count_launch(const TransportArgs &a, int64_t *counts, hipStream_t s);
+int plan_nodes_launch(const TransportArgs &a, const int64_t *node_off, int32_t *cell, float *mu, float *cp,
+                      hipStream_t s);
+
+int sh_records_dispatch(const ShArgs &a, bool half, hipStream_t s);
 int compose_launch(int64_t cells, int rows, int K, int G, const float *mom, const float *first, const float *factor,
                    float *out, hipStream_t s);
 int plane_launch(int nh, int nc, int L, int rows, int G, int strato, const float *src, const float *ha,
@@ -68,16 +73,60 @@ int sos_device_info(int32_t *cus, int32_t *xcds, char *arch, int32_t arch_len) {
     return SOS_OK;
 }
 
+static void fill_args(TransportArgs &a, const sos_grid *g, const sos_bundle *b) {
+    memset(&a, 0, sizeof(a));
+    a.nh = g->nh; a.nc = g->nc; a.nq = g->nq; a.strato = g->strato; a.top = g->top_km;
+    a.gl_u = g->gl_u_dev; a.gl_w = g->gl_w_dev; a.h = g->h_dev; a.chi = g->chi_dev;
+    a.rays = b->rays;
+    a.starts = b->starts_dev; a.ray_group = b->ray_group_dev; a.dirs = b->dirs_dev;
+    a.edge_off = b->edge_off_dev; a.edges = b->edges_dev; a.lower = b->lower_dev;
+}
+
+int sos_plan_count(const sos_grid *g, const sos_bundle *b, int64_t *counts_dev, void *stream) {
+    if (!g || !b || !counts_dev) return arg_error("sos_plan_count: null argument");
+    TransportArgs a;
+    fill_args(a, g, b);
+    return plan_count_launch(a, counts_dev, (hipStream_t)stream);
+}
+
+int sos_plan_nodes(const sos_grid *g, const sos_bundle *b, const int64_t *node_off_dev, int32_t *node_cell_dev,
+                   float *node_mu_dev, float *node_cp_dev, void *stream) {
+    if (!g || !b || !node_off_dev || !node_cell_dev || !node_mu_dev || !node_cp_dev)
+        return arg_error("sos_plan_nodes: null argument");
+    if (g->nh < 2 || g->nc < 2) return arg_error("sos_plan_nodes: grid too small");
+    TransportArgs a;
+    fill_args(a, g, b);
+    return plan_nodes_launch(a, node_off_dev, node_cell_dev, node_mu_dev, node_cp_dev, (hipStream_t)stream);
+}
+
+int sos_sh_records(const sos_grid *g, const void *plane_dev, int32_t half, int64_t items, const int32_t *item_cell_dev,
+                   const int64_t *item_v0_dev, const int64_t *item_v1_dev, const int32_t *vis_dev,
+                   const float *node_mu_dev, const float *node_cp_dev, float *records_dev, void *stream) {
+    if (!g || !plane_dev || !item_cell_dev || !item_v0_dev || !item_v1_dev || !vis_dev || !node_mu_dev ||
+        !node_cp_dev || !records_dev)
+        return arg_error("sos_sh_records: null argument");
+    if (!lanes_ok(g->lanes) || !order_ok(g->L)) return arg_error("sos_sh_records: lanes %d / L %d", g->lanes, g->L);
+    ShArgs a;
+    a.L = g->L; a.G = g->lanes; a.K = (g->L + 1) * (g->L + 2) / 2; a.plane = plane_dev; a.items = items;
+    a.item_cell = item_cell_dev; a.item_v0 = item_v0_dev; a.item_v1 = item_v1_dev; a.vis = vis_dev;
+    a.node_mu = node_mu_dev; a.node_cp = node_cp_dev; a.records = records_dev;
+    return sh_records_dispatch(a, half != 0, (hipStream_t)stream);
+}
+
 int sos_transport(const sos_grid *g, const sos_bundle *b, int32_t flags, int32_t pairs, int32_t rows,
-                  const float *field_dev, const void *plane_dev, const float *boundary_dev, float *out_dev,
-                  uint64_t *counters_dev, void *stream) {
+                  const float *field_dev, const void *plane_dev, const float *boundary_dev, const float *records_dev,
+                  float *out_dev, uint64_t *counters_dev, void *stream) {
     if (!g || !b || !out_dev) return arg_error("sos_transport: null grid, bundle or output");
     if (!lanes_ok(g->lanes)) return arg_error("sos_transport: lanes %d not in {1, 16, 32}", g->lanes);
     if (!order_ok(g->L)) return arg_error("sos_transport: L %d not in {8, 16, 32}", g->L);
     if (pairs != 4 && pairs != 6) return arg_error("sos_transport: pairs %d not 4 or 6", pairs);
     if (g->nh < 2 || g->nc < 2 || g->nq < 1) return arg_error("sos_transport: grid too small");
     const bool first = flags & SOS_FIRST;
-    if (!first && (!field_dev || !plane_dev)) return arg_error("sos_transport: warm transport needs field and plane");
+    const bool records = flags & SOS_RECORDS;
+    if (!first && (!field_dev || (!plane_dev && !records)))
+        return arg_error("sos_transport: warm transport needs field and plane");
+    if (!first && records && (!records_dev || !b->node_off_dev))
+        return arg_error("sos_transport: records mode needs the records and the bundle's node offsets");
     if (!first && rows < 10) return arg_error("sos_transport: rows %d < 10", rows);
     if ((flags & SOS_BOUNDARY) && !boundary_dev) return arg_error("sos_transport: boundary flag without boundary");
     if (first && (flags & SOS_BOUNDARY)) return arg_error("sos_transport: first light takes no boundary");
@@ -101,6 +150,8 @@ int sos_transport(const sos_grid *g, const sos_bundle *b, int32_t flags, int32_t
     a.end_j = b->end_j_dev; a.end_f = b->end_f_dev; a.end_col = b->end_col_dev; a.hits = b->hits_dev;
     a.field = field_dev; a.plane = plane_dev; a.boundary = boundary_dev; a.out = out_dev;
     a.counters = (unsigned long long *)counters_dev;
+    a.records = records_dev;
+    a.node_off = b->node_off_dev;
     return transport_dispatch(a, g->lanes, g->L, (hipStream_t)stream);
 }
 

@@ -87,6 +87,8 @@ struct TransportArgs {
     const float *boundary;
     float *out;
     unsigned long long *counters;
+    const float *records;    // [nodes][4][G] SH values per (node, corner) (records mode)
+    const int64_t *node_off;  // [rays + 1] first node of every ray (records mode)
 };
 
 struct ProjectArgs {
@@ -96,6 +98,17 @@ struct ProjectArgs {
     const float *dloc, *weights, *rad, *lane;
     int ground_only;
     float *moments, *flux;
+};
+
+struct ShArgs {
+    int L, G, K;
+    const void *plane;         // [cells][K][G] float or half
+    int64_t items;
+    const int32_t *item_cell;  // [items]
+    const int64_t *item_v0, *item_v1;
+    const int32_t *vis;        // [visits] visit ids (node * 4 + corner), sorted by cell
+    const float *node_mu, *node_cp;
+    float *records;            // [nodes * 4][G]
 };
 
 }  // namespace sos

@@ -70,10 +70,10 @@ struct Node {
 
 __device__ inline void node_position(const double *p, const double *d, double t, double &r, double &qz, double &qx,
                                      double &qy) {
-    qx = p[0] + t * d[0];
-    qy = p[1] + t * d[1];
-    qz = p[2] + t * d[2];
-    r = sqrt(qx * qx + qy * qy + qz * qz);
+    qx = fma(t, d[0], p[0]);
+    qy = fma(t, d[1], p[1]);
+    qz = fma(t, d[2], p[2]);
+    r = sqrt(fma(qx, qx, fma(qy, qy, qz * qz)));
 }
 
 __device__ inline float node_height(double r, float top) {
@@ -90,8 +90,92 @@ __device__ inline float extinction(const ProfileShape &s, const float *lc, float
     return e + st * sext;
 }
 
-template <int G, int L, int P, bool FIRST, bool BOUNDARY, bool HALF>
+// Geometry of one quadrature node: height, azimuth frame, grid interval, bilinear weights (numpy_reference.py:294-326).
+struct NodeGeom {
+    float h, cx, sx, ah, ac, sn, cc;
+    int ir, j;
+};
+
+__device__ inline NodeGeom node_geometry(double qx, double qy, double qz, double r, const float *__restrict__ hg,
+                                         int nh, const float *__restrict__ chi, int nc, float top) {
+    NodeGeom g;
+    g.h = node_height(r, top);
+    const double rho = sqrt(fma(qx, qx, qy * qy));
+    g.cx = rho > 1e-12 ? (float)(qx / rho) : 1.0f;
+    g.sx = rho > 1e-12 ? (float)(qy / rho) : 0.0f;
+    const float angle = atan2f((float)rho, (float)qz);
+    g.ir = interval(hg, nh, g.h);
+    g.j = interval(chi, nc, angle);
+    const float h0 = hg[g.ir], h1 = hg[g.ir + 1], c0 = chi[g.j], c1 = chi[g.j + 1];
+    g.ah = clampf((g.h - h0) / (h1 - h0), 0.0f, 1.0f);
+    g.ac = clampf((angle - c0) / (c1 - c0), 0.0f, 1.0f);
+    const float ang = fmaf(g.ac, c1 - c0, c0);
+    sincosf(ang, &g.sn, &g.cc);
+    return g;
+}
+
+// cos(phi) of the ray direction about the sun axis relative to the node's meridian (numpy_reference.py:354).
+__device__ inline float node_cp(float st, float df0, float df1, float cx, float sx) {
+    return clampf(st > 1e-12f ? fmaf(df0, cx, df1 * sx) / fmaxf(st, 1e-30f) : 1.0f, -1.0f, 1.0f);
+}
+
+// Path coordinate and weight of node u of a segment [sa, sb], graded quadratically toward the lower end
+// (numpy_reference.py:299-309).
+__device__ inline void node_t(double sa, double sb, bool lower, double u, double wu, double &t, float &ds) {
+    const double len = sb - sa;
+    const double v = lower ? u : 1.0 - u;
+    t = lower ? fma(len, u * u, sa) : fma(-len, v * v, sb);
+    ds = (float)(len * v * wu) * 2.0f;
+}
+
+// Visit plan of a bundle: nodes per ray, then per node its cell (ir nc + j), the ray's mu and the node's cos(phi).
+__global__ void plan_count_kernel(TransportArgs a, int64_t *counts) {
+    const int64_t ray = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (ray >= a.rays) return;
+    const int64_t e0 = a.edge_off[ray];
+    const int nseg = (int)(a.edge_off[ray + 1] - e0) - 1;
+    int64_t n = 0;
+    for (int seg = 0; seg < nseg; ++seg)
+        if (a.edges[e0 + seg + 1] - a.edges[e0 + seg] > 1e-8) n += a.nq;
+    counts[ray] = n;
+}
+
+__global__ void plan_nodes_kernel(TransportArgs a, const int64_t *node_off, int32_t *node_cell, float *node_mu,
+                                  float *node_cp_out) {
+    const int64_t ray = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (ray >= a.rays) return;
+    const int grp = a.ray_group[ray];
+    const double p[3] = {a.starts[3 * grp], a.starts[3 * grp + 1], a.starts[3 * grp + 2]};
+    const double d[3] = {a.dirs[3 * ray], a.dirs[3 * ray + 1], a.dirs[3 * ray + 2]};
+    const float df0 = (float)d[0], df1 = (float)d[1], mu = (float)d[2];
+    const float st = (float)sqrt(fmax(0.0, 1.0 - d[2] * d[2]));
+    const int64_t e0 = a.edge_off[ray];
+    const int nseg = (int)(a.edge_off[ray + 1] - e0) - 1;
+    const unsigned bits = a.lower[ray];
+    int64_t n = node_off[ray];
+    for (int seg = 0; seg < nseg; ++seg) {
+        const double sa = a.edges[e0 + seg], sb = a.edges[e0 + seg + 1];
+        if (!(sb - sa > 1e-8)) continue;
+        const bool lower = (bits >> seg) & 1u;
+        for (int step = 0; step < a.nq; ++step, ++n) {
+            double t;
+            float ds;
+            node_t(sa, sb, lower, a.gl_u[step], a.gl_w[step], t, ds);
+            double r, qz, qx, qy;
+            node_position(p, d, t, r, qz, qx, qy);
+            const NodeGeom ng = node_geometry(qx, qy, qz, r, a.h, a.nh, a.chi, a.nc, a.top);
+            node_cell[n] = ng.ir * a.nc + ng.j;
+            node_mu[n] = mu;
+            node_cp_out[n] = node_cp(st, df0, df1, ng.cx, ng.sx);
+        }
+    }
+}
+
+enum Aero { DIRECT32 = 0, DIRECT16 = 1, RECORDS = 2 };
+
+template <int G, int L, int P, bool FIRST, bool BOUNDARY, int AERO>
 __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
+    constexpr bool HALF = AERO == DIRECT16;
     constexpr int RPW = 64 / G;
     const int lane = threadIdx.x & 63;
     const int w = lane % G;
@@ -145,9 +229,10 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
 #pragma unroll
         for (int q = 0; q < P; ++q) total[q] = corr[q] = 0.0f;
         float tau = 0.0f, tc = 0.0f;
-        float cache[4][L + 1];
+        float cache[AERO == RECORDS ? 1 : 4][AERO == RECORDS ? 1 : L + 1];
         int tags[4] = {-1, -1, -1, -1};
         unsigned nodes = 0, refills = 0;
+        const int64_t node_base = AERO == RECORDS ? a.node_off[ray] : 0;
         const int64_t e0 = a.edge_off[ray];
         const int nseg = (int)(a.edge_off[ray + 1] - e0) - 1;
         const unsigned bits = a.lower[ray];
@@ -164,25 +249,14 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
             float acc = 0.0f;
             for (int step = 0; step < a.nq; ++step) {
                 ++nodes;
-                const double u = a.gl_u[step], wu = a.gl_w[step];
-                const double v = lower ? u : 1.0 - u;
-                const double t = lower ? sa + len * (u * u) : sb - len * (v * v);
-                const float ds = (float)(len * v * wu) * 2.0f;
+                double t;
+                float ds;
+                node_t(sa, sb, lower, a.gl_u[step], a.gl_w[step], t, ds);
                 double r, qz, qx, qy;
                 node_position(p, d, t, r, qz, qx, qy);
-                const float h = node_height(r, a.top);
-                const double rho = sqrt(qx * qx + qy * qy);
-                const float cx = rho > 1e-12 ? (float)(qx / rho) : 1.0f;
-                const float sx = rho > 1e-12 ? (float)(qy / rho) : 0.0f;
-                const float angle = atan2f((float)rho, (float)qz);
-                const int ir = interval(a.h, a.nh, h);
-                const int j = interval(a.chi, a.nc, angle);
-                const float h0 = a.h[ir], h1 = a.h[ir + 1], c0 = a.chi[j], c1 = a.chi[j + 1];
-                const float ah = clampf((h - h0) / (h1 - h0), 0.0f, 1.0f);
-                const float ac = clampf((angle - c0) / (c1 - c0), 0.0f, 1.0f);
-                const float ang = c0 + ac * (c1 - c0);
-                float sn, cc;
-                sincosf(ang, &sn, &cc);
+                const NodeGeom ng = node_geometry(qx, qy, qz, r, a.h, a.nh, a.chi, a.nc, a.top);
+                const float h = ng.h, cx = ng.cx, sx = ng.sx, ah = ng.ah, ac = ng.ac, sn = ng.sn, cc = ng.cc;
+                const int ir = ng.ir, j = ng.j;
                 // rotation rows Q[k][j] (numpy_reference.py:327)
                 const float Q[9] = {-cc * cx, sx, sn * cx, -cc * sx, -cx, sn * sx, sn, 0.0f, cc};
                 float prof[5], stp;
@@ -312,58 +386,66 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
                         }
                     }
                 } else {
-                    // adjacent-stencil reuse of cached corners (numpy_reference.py:106-129)
-                    const int old = tags[0];
-                    if (old >= 0) {
-                        const int di = cells[0] / nc - old / nc, dj = cells[0] % nc - old % nc;
-                        auto copy = [&](int dst, int src) {
-#pragma unroll
-                            for (int m = 0; m <= L; ++m) cache[dst][m] = cache[src][m];
-                            tags[dst] = cells[dst];
-                        };
-                        if (di == 0 && dj == 1) { copy(0, 1); copy(2, 3); }
-                        else if (di == 0 && dj == -1) { copy(1, 0); copy(3, 2); }
-                        else if (di == 1 && dj == 0) { copy(0, 2); copy(1, 3); }
-                        else if (di == -1 && dj == 0) { copy(2, 0); copy(3, 1); }
-                        else if (di == 1 && dj == 1) { copy(0, 3); }
-                        else if (di == 1 && dj == -1) { copy(1, 2); }
-                        else if (di == -1 && dj == 1) { copy(2, 1); }
-                        else if (di == -1 && dj == -1) { copy(3, 0); }
-                    }
-#pragma unroll 1
-                    for (int c = 0; c < 4; ++c) {
-                        const int cell = c == 0 ? cells[0] : c == 1 ? cells[1] : c == 2 ? cells[2] : cells[3];
-                        const int tag = c == 0 ? tags[0] : c == 1 ? tags[1] : c == 2 ? tags[2] : tags[3];
-                        if (tag != cell) {
-                            ++refills;
-                            float fresh[L + 1];
-                            refill<L, G, HALF>(a.plane, cell, w, mu, st, fresh);
-#pragma unroll
-                            for (int s = 0; s < 4; ++s)
-                                if (s == c) {
-#pragma unroll
-                                    for (int m = 0; m <= L; ++m) cache[s][m] = fresh[m];
-                                    tags[s] = cell;
-                                }
-                        }
-                    }
-                    // cos(m phi) of this node (arithmetic.py:112-119) and the four azimuthal sums
-                    const float cp = clampf(st > 1e-12f ? (df0 * cx + df1 * sx) / fmaxf(st, 1e-30f) : 1.0f, -1.0f, 1.0f);
                     float aero[4];
+                    if constexpr (AERO == RECORDS) {
+                        // per (node, corner) SH values produced by the records kernel: [node][4][G]
+                        const float *rec = a.records + (node_base + nodes - 1) * 4 * G + w;
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) aero[c] = cache[c][0];
-                    {
-                        float cm0 = 1.0f, cm1 = cp;
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) aero[c] = aero[c] + cm1 * cache[c][1];
-#pragma unroll
-                        for (int m = 2; m <= L; ++m) {
-                            const float cm2 = (2.0f * cp) * cm1 - cm0;
-#pragma unroll
-                            for (int c = 0; c < 4; ++c) aero[c] = aero[c] + cm2 * cache[c][m];
-                            cm0 = cm1;
-                            cm1 = cm2;
+                        for (int c = 0; c < 4; ++c) aero[c] = rec[c * G];
+                    } else {
+                        // adjacent-stencil reuse of cached corners (numpy_reference.py:106-129)
+                        const int old = tags[0];
+                        if (old >= 0) {
+                            const int di = cells[0] / nc - old / nc, dj = cells[0] % nc - old % nc;
+                            auto copy = [&](int dst, int src) {
+    #pragma unroll
+                                for (int m = 0; m <= L; ++m) cache[dst][m] = cache[src][m];
+                                tags[dst] = cells[dst];
+                            };
+                            if (di == 0 && dj == 1) { copy(0, 1); copy(2, 3); }
+                            else if (di == 0 && dj == -1) { copy(1, 0); copy(3, 2); }
+                            else if (di == 1 && dj == 0) { copy(0, 2); copy(1, 3); }
+                            else if (di == -1 && dj == 0) { copy(2, 0); copy(3, 1); }
+                            else if (di == 1 && dj == 1) { copy(0, 3); }
+                            else if (di == 1 && dj == -1) { copy(1, 2); }
+                            else if (di == -1 && dj == 1) { copy(2, 1); }
+                            else if (di == -1 && dj == -1) { copy(3, 0); }
                         }
+    #pragma unroll 1
+                        for (int c = 0; c < 4; ++c) {
+                            const int cell = c == 0 ? cells[0] : c == 1 ? cells[1] : c == 2 ? cells[2] : cells[3];
+                            const int tag = c == 0 ? tags[0] : c == 1 ? tags[1] : c == 2 ? tags[2] : tags[3];
+                            if (tag != cell) {
+                                ++refills;
+                                float fresh[L + 1];
+                                refill<L, G, HALF>(a.plane, cell, w, mu, st, fresh);
+    #pragma unroll
+                                for (int s = 0; s < 4; ++s)
+                                    if (s == c) {
+    #pragma unroll
+                                        for (int m = 0; m <= L; ++m) cache[s][m] = fresh[m];
+                                        tags[s] = cell;
+                                    }
+                            }
+                        }
+                        // cos(m phi) of this node (arithmetic.py:112-119) and the four azimuthal sums
+                        const float cp = node_cp(st, df0, df1, cx, sx);
+    #pragma unroll
+                        for (int c = 0; c < 4; ++c) aero[c] = cache[c][0];
+                        {
+                            float cm0 = 1.0f, cm1 = cp;
+    #pragma unroll
+                            for (int c = 0; c < 4; ++c) aero[c] = aero[c] + cm1 * cache[c][1];
+    #pragma unroll
+                            for (int m = 2; m <= L; ++m) {
+                                const float cm2 = (2.0f * cp) * cm1 - cm0;
+    #pragma unroll
+                                for (int c = 0; c < 4; ++c) aero[c] = aero[c] + cm2 * cache[c][m];
+                                cm0 = cm1;
+                                cm1 = cm2;
+                            }
+                        }
+
                     }
                     float logs = 0.0f, wg = 0.0f, am = 0.0f;
 #pragma unroll
@@ -447,54 +529,73 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
 namespace sos {
 int set_error(const char *fmt, ...);
 
-template <int G, int L, int P, bool FIRST, bool BOUNDARY, bool HALF>
+template <int G, int L, int P, bool FIRST, bool BOUNDARY, int AERO>
 static int launch_one(const TransportArgs &a, hipStream_t s) {
     int dev = 0, cus = 256;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const int64_t wgs_needed = (a.tiles + 3) / 4;
-    int64_t grid = cus * 8;
+    int64_t grid = (int64_t)cus * 8;
     if (grid > wgs_needed) grid = wgs_needed;
     grid = ((grid + 7) / 8) * 8;
     if (grid < 8) grid = 8;
-    hipLaunchKernelGGL((transport_kernel<G, L, P, FIRST, BOUNDARY, HALF>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((transport_kernel<G, L, P, FIRST, BOUNDARY, AERO>), dim3((unsigned)grid), dim3(256), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error("transport launch: %s", hipGetErrorString(e));
     return SOS_OK;
 }
 
-template <int G, int L, int P>
-static int launch_flags(const TransportArgs &a, hipStream_t s) {
+// First light and the records consumer do not depend on L; the direct (register cache) path does.
+template <int G, int P>
+static int launch_flags(const TransportArgs &a, int L, hipStream_t s) {
     const bool first = a.flags & SOS_FIRST, bnd = a.flags & SOS_BOUNDARY, half = a.flags & SOS_MIXED;
-    if (first) return launch_one<G, L, P, true, false, false>(a, s);
-    if (bnd) return half ? launch_one<G, L, P, false, true, true>(a, s) : launch_one<G, L, P, false, true, false>(a, s);
-    return half ? launch_one<G, L, P, false, false, true>(a, s) : launch_one<G, L, P, false, false, false>(a, s);
-}
-
-template <int G, int L>
-static int launch_pairs(const TransportArgs &a, hipStream_t s) {
-    if (a.pairs == 4) return launch_flags<G, L, 4>(a, s);
-    if (a.pairs == 6) return launch_flags<G, L, 6>(a, s);
-    return set_error("pairs must be 4 or 6");
+    const bool records = a.flags & SOS_RECORDS;
+    if (first) return launch_one<G, 8, P, true, false, RECORDS>(a, s);
+    if (records)
+        return bnd ? launch_one<G, 8, P, false, true, RECORDS>(a, s) : launch_one<G, 8, P, false, false, RECORDS>(a, s);
+#define SOS_DIRECT(LL)                                                                                   \
+    return bnd ? (half ? launch_one<G, LL, P, false, true, DIRECT16>(a, s)                               \
+                       : launch_one<G, LL, P, false, true, DIRECT32>(a, s))                              \
+               : (half ? launch_one<G, LL, P, false, false, DIRECT16>(a, s)                              \
+                       : launch_one<G, LL, P, false, false, DIRECT32>(a, s));
+    switch (L) {
+        case 8: SOS_DIRECT(8)
+        case 16: SOS_DIRECT(16)
+        case 32: SOS_DIRECT(32)
+        default: return set_error("unsupported L %d (8, 16, 32)", L);
+    }
+#undef SOS_DIRECT
 }
 
 template <int G>
-static int launch_L(const TransportArgs &a, int L, hipStream_t s) {
-    switch (L) {
-        case 8: return launch_pairs<G, 8>(a, s);
-        case 16: return launch_pairs<G, 16>(a, s);
-        case 32: return launch_pairs<G, 32>(a, s);
-        default: return set_error("unsupported L %d (8, 16, 32)", L);
-    }
+static int launch_pairs(const TransportArgs &a, int L, hipStream_t s) {
+    if (a.pairs == 4) return launch_flags<G, 4>(a, L, s);
+    if (a.pairs == 6) return launch_flags<G, 6>(a, L, s);
+    return set_error("pairs must be 4 or 6");
 }
 
 int transport_dispatch(const TransportArgs &a, int G, int L, hipStream_t s) {
     switch (G) {
-        case 1: return launch_L<1>(a, L, s);
-        case 16: return launch_L<16>(a, L, s);
-        case 32: return launch_L<32>(a, L, s);
+        case 1: return launch_pairs<1>(a, L, s);
+        case 16: return launch_pairs<16>(a, L, s);
+        case 32: return launch_pairs<32>(a, L, s);
         default: return set_error("unsupported lane count %d (1, 16, 32)", G);
     }
+}
+
+int plan_count_launch(const TransportArgs &a, int64_t *counts, hipStream_t s) {
+    const unsigned b = (unsigned)((a.rays + 255) / 256);
+    if (b) hipLaunchKernelGGL(plan_count_kernel, dim3(b), dim3(256), 0, s, a, counts);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SOS_OK : set_error("plan_count: %s", hipGetErrorString(e));
+}
+
+int plan_nodes_launch(const TransportArgs &a, const int64_t *node_off, int32_t *cell, float *mu, float *cp,
+                      hipStream_t s) {
+    const unsigned b = (unsigned)((a.rays + 255) / 256);
+    if (b) hipLaunchKernelGGL(plan_nodes_kernel, dim3(b), dim3(256), 0, s, a, node_off, cell, mu, cp);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SOS_OK : set_error("plan_nodes: %s", hipGetErrorString(e));
 }
 
 }  // namespace sos

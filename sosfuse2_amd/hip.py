@@ -9,7 +9,7 @@ c_i32, c_i64, c_u32, c_f32 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32, ct
 P = ctypes.c_void_p
 
 LANE_CONSTANTS = 12
-FIRST, BOUNDARY, MIXED = 1, 2, 4
+FIRST, BOUNDARY, MIXED, RECORDS = 1, 2, 4, 8
 
 
 class Bundle(ctypes.Structure):
@@ -17,6 +17,7 @@ class Bundle(ctypes.Structure):
         ('groups', c_i64), ('rays', c_i64), ('group_off_dev', P), ('starts_dev', P), ('targets_dev', P),
         ('ray_group_dev', P), ('dirs_dev', P), ('dloc_dev', P), ('weights_dev', P), ('edge_off_dev', P),
         ('edges_dev', P), ('lower_dev', P), ('end_j_dev', P), ('end_f_dev', P), ('end_col_dev', P), ('hits_dev', P),
+        ('node_off_dev', P),
     ]
 
 
@@ -32,7 +33,10 @@ class Grid(ctypes.Structure):
 
 
 _SIGNATURES = {
-    'sos_transport': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), c_i32, c_i32, c_i32, P, P, P, P, P, P],
+    'sos_transport': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), c_i32, c_i32, c_i32, P, P, P, P, P, P, P],
+    'sos_plan_count': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, P],
+    'sos_plan_nodes': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, P, P, P, P],
+    'sos_sh_records': [ctypes.POINTER(Grid), P, c_i32, c_i64, P, P, P, P, P, P, P, P],
     'sos_project': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, c_i32, P, P, P],
     'sos_compose': [ctypes.POINTER(Grid), c_i32, P, P, P, P, P],
     'sos_make_plane': [ctypes.POINTER(Grid), c_i32, P, P, P, c_i32, P, P],

@@ -56,15 +56,54 @@ class DeviceBundle:
         self.end_col = t(b.end_col, 'f4')
         self.hits = t(b.hits, 'u1')
         self.samples = b.samples
+        self.plan = None
         self.struct = hip.Bundle(
             self.groups, self.rays, *(hip.ptr(x) for x in (
                 self.group_off, self.starts, self.targets, self.ray_group, self.dirs, self.dloc, self.weights,
-                self.edge_off, self.edges, self.lower, self.end_j, self.end_f, self.end_col, self.hits)))
+                self.edge_off, self.edges, self.lower, self.end_j, self.end_f, self.end_col, self.hits)), None)
 
     def nbytes(self):
         return sum(x.numel() * x.element_size() for x in (
             self.group_off, self.starts, self.targets, self.ray_group, self.dirs, self.dloc, self.weights,
             self.edge_off, self.edges, self.lower, self.end_j, self.end_f, self.end_col, self.hits))
+
+
+class Plan:
+    """Visit plan of a bundle (fixed by its geometry): every quadrature node's cell, mu and cos(phi), the visits
+    (node, corner) sorted by their source cell, and work items of at most `chunk` visits of one cell."""
+
+    def __init__(self, kernel, dc, db, chunk=2048):
+        dev = kernel.device
+        st = kernel.stream()
+        counts = torch.empty(max(db.rays, 1), dtype=torch.int64, device=dev)
+        hip.call('sos_plan_count', dc.struct, db.struct, hip.ptr(counts), st)
+        self.node_off = torch.zeros(db.rays + 1, dtype=torch.int64, device=dev)
+        self.node_off[1:] = torch.cumsum(counts[: db.rays], 0)
+        db.struct.node_off_dev = hip.ptr(self.node_off)
+        self.nodes = N = int(self.node_off[-1].item())
+        self.node_cell = torch.empty(max(N, 1), dtype=torch.int32, device=dev)
+        self.node_mu = torch.empty(max(N, 1), dtype=torch.float32, device=dev)
+        self.node_cp = torch.empty(max(N, 1), dtype=torch.float32, device=dev)
+        hip.call('sos_plan_nodes', dc.struct, db.struct, hip.ptr(self.node_off), hip.ptr(self.node_cell),
+                 hip.ptr(self.node_mu), hip.ptr(self.node_cp), st)
+        nc = dc.struct.nc
+        cells = dc.struct.nh * nc
+        offs = torch.tensor([0, 1, nc, nc + 1], dtype=torch.int32, device=dev)
+        vcell = (self.node_cell[:N, None] + offs[None, :]).reshape(-1)
+        self.vis = torch.argsort(vcell, stable=True).to(torch.int32)
+        per_cell = torch.bincount(vcell, minlength=cells)
+        del vcell
+        cell_off = torch.zeros(cells + 1, dtype=torch.int64, device=dev)
+        cell_off[1:] = torch.cumsum(per_cell, 0)
+        nchunk = (per_cell + chunk - 1) // chunk
+        self.item_cell = torch.repeat_interleave(torch.arange(cells, device=dev, dtype=torch.int32), nchunk)
+        first_item = torch.cumsum(nchunk, 0) - nchunk
+        idx = torch.arange(len(self.item_cell), device=dev) - torch.repeat_interleave(first_item, nchunk)
+        self.item_v0 = cell_off[:-1][self.item_cell.long()] + idx * chunk
+        self.item_v1 = torch.minimum(self.item_v0 + chunk, cell_off[1:][self.item_cell.long()])
+        self.items = len(self.item_cell)
+        self.visits = 4 * N
+        self.records = torch.empty((max(4 * N, 1), dc.G), dtype=torch.float32, device=dev)
 
 
 class DeviceConstants:
@@ -113,7 +152,7 @@ class Kernel:
 
     backend = 'hip'
 
-    def __init__(self, backend='hip', L=16, block_size=256, device='cuda', mixed=False):
+    def __init__(self, backend='hip', L=16, block_size=256, device='cuda', mixed=False, records=True):
         if backend != 'hip':
             raise ValueError(f"E_BACKEND: backend {backend!r}; this engine runs 'hip' only")
         layout.count(L)
@@ -127,6 +166,7 @@ class Kernel:
         self.block_size = block_size
         self.device = torch.device(device)
         self.mixed = mixed
+        self.records = records  # SH by per-cell MFMA records (True) or per-ray register caches (False)
         self.xp = torch
         self._constants = {}
         self.metrics = dict(samples=0, launches=0)
@@ -164,27 +204,48 @@ class Kernel:
                  int(self.mixed), hip.ptr(out), self.stream())
         return out
 
+    def plan(self, db, dc):
+        if db.plan is None:
+            db.plan = Plan(self, dc, db)
+        return db.plan
+
+    def _event(self):
+        if self.timed is None:
+            return None
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        return e
+
     def radiance(self, db, ctx, field, first=False, boundary=None, pairs=4, plane=None):
         """Per-ray radiance tensor [rays, G, pairs]."""
         dc = self.constants(ctx, db.nq)
         out = torch.empty((db.rays, dc.G, pairs), dtype=torch.float32, device=self.device)
         flags = (hip.FIRST if first else 0) | (hip.BOUNDARY if boundary is not None else 0)
+        records = None
+        name = 'first' if first else ('warm' if boundary is not None else
+                                      ('query' if pairs == 6 or db.groups != ctx.geo.nc else 'ground'))
         if not first:
             if plane is None:
                 plane = self.plane(ctx, field, dc)
             if self.mixed:
                 flags |= hip.MIXED
-        name = 'first' if first else ('warm' if boundary is not None else ('query' if pairs == 6 or db.groups != ctx.geo.nc else 'ground'))
-        ev = None
-        if self.timed is not None:
-            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-            ev[0].record()
+            if self.records:
+                pl = self.plan(db, dc)
+                e0 = self._event()
+                hip.call('sos_sh_records', dc.struct, hip.ptr(plane), int(self.mixed), pl.items, hip.ptr(pl.item_cell),
+                         hip.ptr(pl.item_v0), hip.ptr(pl.item_v1), hip.ptr(pl.vis), hip.ptr(pl.node_mu),
+                         hip.ptr(pl.node_cp), hip.ptr(pl.records), self.stream())
+                if e0 is not None:
+                    e1 = self._event()
+                    self.timed.append(('sh_' + name, e0, e1))
+                records = pl.records
+                flags |= hip.RECORDS
+        e0 = self._event()
         hip.call('sos_transport', dc.struct, db.struct, flags, pairs, ctx.rows,
                  None if first else hip.ptr(field), None if first else hip.ptr(plane), hip.ptr(boundary),
-                 hip.ptr(out), hip.ptr(self.counters), self.stream())
-        if ev is not None:
-            ev[1].record()
-            self.timed.append((name, ev[0], ev[1]))
+                 hip.ptr(records), hip.ptr(out), hip.ptr(self.counters), self.stream())
+        if e0 is not None:
+            self.timed.append((name, e0, self._event()))
         self.metrics['launches'] += 1
         self.metrics['samples'] += db.samples
         return out

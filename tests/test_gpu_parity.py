@@ -31,23 +31,24 @@ def padded(a, G):
     return np.concatenate([a, np.repeat(a[..., -1:], G - a.shape[-1], -1)], -1)
 
 
-def setup_case(W, L=16, nr=5, state=None, nmu=1, nphi=2, path_n=4):
+def setup_case(W, L=16, nr=5, state=None, nmu=1, nphi=2, path_n=4, records=True):
     from oracle.solve import OracleSolver
 
     cfg = dict(nr=nr, chi_grid='tiny', nmu=nmu, nphi=nphi, path_n=path_n, L=L, top_km=100.0, observer_km=0.1,
                query_n=6, anderson_depth=4)
     wl = atmosphere.log_spaced_wavelengths(W, 420, 680) if W > 1 else [550.0]
     st = state or atmosphere.config0_state()
-    s, scope = api.create_configured(None, cfg, wl, 'hip', st)
+    s, scope = api.create_configured(None, cfg, wl, 'hip', st, records=records)
     orc = OracleSolver(s, np.float32, depth=4, packs=scope.packs)
     return s, scope, orc
 
 
-@pytest.fixture(scope='module', params=[(1, 16), (3, 16), (20, 8)], ids=['W1-G1', 'W3-G16', 'W20-G32'])
+@pytest.fixture(scope='module', params=[(1, 16, True), (3, 16, True), (20, 8, True), (3, 16, False), (20, 32, True)],
+                ids=['W1-G1', 'W3-G16', 'W20-G32', 'W3-G16-direct', 'W20-G32-L32'])
 def case(request, gpu):
-    W, L = request.param
-    print(f'case W={W} L={L}', flush=True)
-    return setup_case(W, L)
+    W, L, records = request.param
+    print(f'case W={W} L={L} records={records}', flush=True)
+    return setup_case(W, L, records=records)
 
 
 def test_first_light_moments(case):
@@ -189,7 +190,8 @@ def test_device_anderson_pieces_match_host(gpu):
     st = torch.cuda.current_stream().cuda_stream
     fh = []
     for slot in range(D):
-        hip.call('sos_anderson_residual', C, rows, G, hip.ptr(dev(gs[slot])), hip.ptr(dev(xs[slot])), hip.ptr(dev(obs)),
+        g_d, x_d, o_d = dev(gs[slot]), dev(xs[slot]), dev(obs)  # held until the launch has run
+        hip.call('sos_anderson_residual', C, rows, G, hip.ptr(g_d), hip.ptr(x_d), hip.ptr(o_d),
                  nobs, hip.ptr(work), hip.ptr(fring[slot]), st)
         torch.cuda.synchronize()
         f = np.zeros((G, n), 'f4')
@@ -239,8 +241,9 @@ def test_source_metrics(gpu):
     g[40, 5] = np.inf
     out = torch.empty((3, G), dtype=torch.float32, device='cuda')
     frozen = 1 << 7
-    hip.call('sos_source_metrics', 1000, G, hip.ptr(torch.from_numpy(g).cuda()), hip.ptr(torch.from_numpy(x).cuda()),
-             frozen, hip.ptr(out), torch.cuda.current_stream().cuda_stream)
+    g_d, x_d = torch.from_numpy(g).cuda(), torch.from_numpy(x).cuda()
+    hip.call('sos_source_metrics', 1000, G, hip.ptr(g_d), hip.ptr(x_d), frozen, hip.ptr(out),
+             torch.cuda.current_stream().cuda_stream)
     m = out.cpu().numpy()
     assert m[0, 3] == 1 and m[0, 5] == 1 and m[0].sum() == 2
     for w in (0, 1, 7, 15):
