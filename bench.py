@@ -90,16 +90,20 @@ def main():
     t0 = time.perf_counter()
     s, scope = api.create_configured(None, cfg, wl, 'hip', state, mixed=args.mixed)
     solver = scope.solver(s, log=None)
+    k = s.kernel
+    k.plan(solver.geometry.bundle, k.constants(s, solver.geometry.bundle.nq))
+    k.plan(solver.geometry.ground_bundle, k.constants(s, solver.geometry.ground_bundle.nq))
     torch.cuda.synchronize()
     prep = time.perf_counter() - t0
     log(f'prepared in {prep:.1f} s: {s.geo.ns} cells, {solver.geometry.bundle.rays} rays, '
         f'{solver.geometry.bundle.samples} nodes per evaluation, W={s.W}')
-    k = s.kernel
     tol = float(cfg['anderson_tol'])
 
     def step():
-        x, info = solver.solve(60, tol)
-        table, _ = api.make_table(s, x, hs, el, az, log=None)
+        # a table of a new state: first light, the source iteration and the query (geometry and plans are shared)
+        sv = scope.solver(s, log=log)
+        x, info = sv.solve(60, tol)
+        table, _ = api.make_table(s, x, hs, el, az, log=log)
         return info, table
 
     for i in range(args.warmup):

@@ -16,50 +16,69 @@
 namespace sos {
 int set_error(const char *fmt, ...);
 
-// 1/sqrt(4 pi) prod_{j<=m} dd_j: the diagonal seed without the st^m factor (arithmetic.py:95-97)
-__device__ inline float seed_coef(int m) {
-    double c = 0.28209479177387814;
-    for (int j = 1; j <= m; ++j) c *= -sqrt((2.0 * j + 1) / (2.0 * j));
-    return (float)c;
-}
-
-
-
 template <int L, bool HALF>
 __device__ inline float plane_at(const void *p, int64_t i) {
     if constexpr (HALF) return __half2float(reinterpret_cast<const __half *>(p)[i]);
     else return reinterpret_cast<const float *>(p)[i];
 }
 
-// Z rows of up to 32 visits into LDS: chain (visit, m) runs the Legendre recurrence over l.
-template <int L, int NT>
-__device__ inline void build_z(float (*Z)[((L + 1) * (L + 2) / 2 + 15) / 16 * 16], const float *smu, const float *scp,
-                               const float *sseed, int nv) {
+// Recurrence constants in FP64 (arithmetic.py:69-83), staged once per workgroup.
+template <int L>
+__device__ inline void stage_constants(double *aa, double *bb, int nt) {
     constexpr int K = (L + 1) * (L + 2) / 2;
-    for (int c = threadIdx.x; c < 32 * (L + 1); c += NT) {
-        const int v = c & 31, m = c >> 5;
+    for (int k = threadIdx.x; k < K; k += nt) {
+        int l = 0;
+        while ((l + 1) * (l + 2) / 2 <= k) ++l;
+        const int m = k - l * (l + 1) / 2;
+        const double fl = l, fm = m;
+        aa[k] = l > m ? sqrt((4.0 * fl * fl - 1.0) / (fl * fl - fm * fm)) : 0.0;
+        bb[k] = l > m + 1 ? sqrt((2.0 * fl + 1.0) * ((fl - 1.0) * (fl - 1.0) - fm * fm) / ((2.0 * fl - 3.0) * (fl * fl - fm * fm)))
+                          : 0.0;
+    }
+}
+
+// Per visit: the diagonal seeds 1/sqrt(4 pi) prod dd_j st (FP64, arithmetic.py:95-97) and cos(m phi) by the FP32
+// Chebyshev recurrence (arithmetic.py:112-119).
+template <int L, int V>
+__device__ inline void visit_seeds(int v, double mu, float cp, double (*seed)[V], float (*cmv)[V]) {
+    const double st = sqrt(fmax(0.0, 1.0 - mu * mu));
+    double diag = 0.28209479177387814;
+    float c0 = 1.0f, c1 = cp;
+    for (int m = 0; m <= L; ++m) {
+        if (m) diag = (diag * -sqrt((2.0 * m + 1) / (2.0 * m))) * st;
+        seed[m][v] = diag;
+        float cm;
+        if (m == 0) cm = 1.0f;
+        else if (m == 1) cm = cp;
+        else { cm = (2.0f * cp) * c1 - c0; c0 = c1; c1 = cm; }
+        cmv[m][v] = cm;
+    }
+}
+
+// Z rows of V visits into LDS: chain (visit, m) runs the Legendre recurrence over l in FP64 and rounds each basis
+// value to FP32 before the FP32 product with cos(m phi) (the reference's FP64 basis, rounded).
+template <int L, int V, int KZ>
+__device__ inline void build_z(float (*Z)[KZ], const double *mus, const double (*seed)[V], const float (*cmv)[V],
+                               const double *aa, const double *bb, int nv, int nt) {
+    for (int c = threadIdx.x; c < V * (L + 1); c += nt) {
+        const int v = c % V, m = c / V;
         const int o = moff<L>(m);
         if (v >= nv) {
             for (int l = m; l <= L; ++l) Z[v][o + l - m] = 0.0f;
             continue;
         }
-        const float mu = smu[v], cp = scp[v];
-        const float st = sqrtf(fmaxf(0.0f, 1.0f - mu * mu));
-        const float cm = m == 0 ? 1.0f : cosf((float)m * acosf(cp));
-        const float mult = m ? 1.41421356237309515f * cm : cm;
-        float z0 = sseed[m] * (m ? powf(st, (float)m) : 1.0f);
-        if (m > 0 && st < 1e-30f) z0 = 0.0f;
-        Z[v][o] = mult * z0;
+        const double mu = mus[v];
+        const float cm = cmv[m][v];
+        const double mult = m ? 1.41421356237309515 : 1.0;
+        double z0 = seed[m][v];
+        Z[v][o] = (float)(mult * z0) * cm;
         if (m < L) {
-            float z1 = (sqrtf(2.0f * m + 3.0f) * mu) * z0;
-            Z[v][o + 1] = mult * z1;
+            double z1 = (sqrt(2.0 * m + 3.0) * mu) * z0;
+            Z[v][o + 1] = (float)(mult * z1) * cm;
             for (int l = m + 2; l <= L; ++l) {
-                const float fl = (float)l, fm = (float)m;
-                const float aa = sqrtf((4.0f * fl * fl - 1.0f) / (fl * fl - fm * fm));
-                const float bb = sqrtf((2.0f * fl + 1.0f) * ((fl - 1.0f) * (fl - 1.0f) - fm * fm) /
-                                       ((2.0f * fl - 3.0f) * (fl * fl - fm * fm)));
-                const float z2 = (aa * mu) * z1 - bb * z0;
-                Z[v][o + l - m] = mult * z2;
+                const int k = kidx(l, m);
+                const double z2 = (aa[k] * mu) * z1 - bb[k] * z0;
+                Z[v][o + l - m] = (float)(mult * z2) * cm;
                 z0 = z1;
                 z1 = z2;
             }
@@ -68,20 +87,25 @@ __device__ inline void build_z(float (*Z)[((L + 1) * (L + 2) / 2 + 15) / 16 * 16
 }
 
 template <int G, int L, bool HALF>
-__global__ __launch_bounds__(G * 8) void sh_records_kernel(ShArgs a) {
+__global__ __launch_bounds__(G * 4) void sh_records_kernel(ShArgs a) {
     constexpr int K = (L + 1) * (L + 2) / 2;
     constexpr int KZ = (K + 15) / 16 * 16;   // padded row
     constexpr int KS = KZ / 4;               // MFMA k-steps
-    constexpr int NT = G * 8;                // threads: 2 M-tiles x (G/16) N-tiles wavefronts
-    __shared__ float Z[32][KZ];
-    __shared__ float smu[32], scp[32], sseed[L + 1];
-    __shared__ int sid[32];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int mt = wave / (G / 16), nt = wave % (G / 16);
+    constexpr int NT = G * 4;                // one wavefront per 16-lane N-tile
+    constexpr int V = 16;                    // visits per round (one M-tile)
+    __shared__ float Z[V][KZ];
+    __shared__ double aa[K], bb[K];
+    __shared__ double seed[L + 1][V];
+    __shared__ float cmv[L + 1][V];
+    __shared__ double smu[V];
+    __shared__ int sid[V];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int nt = tid >> 6;
     const int64_t item = blockIdx.x;
     const int cell = a.item_cell[item];
     const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
-    for (int m = tid; m <= L; m += NT) sseed[m] = seed_coef(m);
+    stage_constants<L>(aa, bb, NT);
+    for (int e = tid; e < V * (KZ - K); e += NT) Z[e / (KZ - K)][K + e % (KZ - K)] = 0.0f;
     // B operand in registers: b[s] = c[cell][k'(s, lane)][nt*16 + lane%16]
     float b[KS];
     {
@@ -93,29 +117,29 @@ __global__ __launch_bounds__(G * 8) void sh_records_kernel(ShArgs a) {
             b[s] = k < K ? plane_at<L, HALF>(a.plane, base + (int64_t)k * G) : 0.0f;
         }
     }
-    for (int64_t r0 = v0; r0 < v1; r0 += 32) {
-        const int nv = (int)min((int64_t)32, v1 - r0);
-        if (tid < 32) {
+    for (int64_t r0 = v0; r0 < v1; r0 += V) {
+        const int nv = (int)min((int64_t)V, v1 - r0);
+        if (tid < V) {
             int id = -1;
-            float mu = 0.0f, cp = 1.0f;
+            double mu = 0.0;
+            float cp = 1.0f;
             if (tid < nv) {
                 id = a.vis[r0 + tid];
                 const int64_t node = id >> 2;
-                mu = a.node_mu[node];
+                mu = (double)a.node_mu[node];
                 cp = a.node_cp[node];
             }
             sid[tid] = id;
             smu[tid] = mu;
-            scp[tid] = cp;
+            visit_seeds<L, V>(tid, mu, cp, seed, cmv);
         }
         __syncthreads();
-        build_z<L, NT>(Z, smu, scp, sseed, nv);
-        for (int e = tid; e < 32 * (KZ - K); e += NT) Z[e / (KZ - K)][K + e % (KZ - K)] = 0.0f;
+        build_z<L, V, KZ>(Z, smu, seed, cmv, aa, bb, nv, NT);
         __syncthreads();
         floatx4 acc[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
-        const float *zrow = &Z[mt * 16 + (lane & 15)][4 * (lane >> 4)];
+        const float *zrow = &Z[lane & 15][4 * (lane >> 4)];
 #pragma unroll
         for (int q = 0; q < KS / 4; ++q) {
             const floatx4 av = *reinterpret_cast<const floatx4 *>(zrow + 16 * q);
@@ -126,7 +150,7 @@ __global__ __launch_bounds__(G * 8) void sh_records_kernel(ShArgs a) {
         const floatx4 d = (acc[0] + acc[1]) + (acc[2] + acc[3]);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int row = mt * 16 + 4 * (lane >> 4) + v;
+            const int row = 4 * (lane >> 4) + v;
             if (row < nv) a.records[(int64_t)sid[row] * G + nt * 16 + (lane & 15)] = d[v];
         }
         __syncthreads();
@@ -138,43 +162,43 @@ template <int L, bool HALF>
 __global__ __launch_bounds__(256) void sh_records_scalar(ShArgs a) {
     constexpr int K = (L + 1) * (L + 2) / 2;
     __shared__ float c[K];
-    __shared__ float sseed[L + 1];
+    __shared__ double aa[K], bb[K];
     const int64_t item = blockIdx.x;
     const int cell = a.item_cell[item];
     for (int k = threadIdx.x; k < K; k += 256) c[k] = plane_at<L, HALF>(a.plane, (int64_t)cell * K + k);
-    for (int m = threadIdx.x; m <= L; m += 256) sseed[m] = seed_coef(m);
+    stage_constants<L>(aa, bb, 256);
     __syncthreads();
     for (int64_t r = a.item_v0[item] + threadIdx.x; r < a.item_v1[item]; r += 256) {
         const int id = a.vis[r];
         const int64_t node = id >> 2;
-        const float mu = a.node_mu[node], cp = a.node_cp[node];
-        const float st = sqrtf(fmaxf(0.0f, 1.0f - mu * mu));
+        const double mu = a.node_mu[node];
+        const float cp = a.node_cp[node];
+        const double st = sqrt(fmax(0.0, 1.0 - mu * mu));
         float sum = 0.0f;
         float cm0 = 1.0f, cm1 = cp;
+        double diag = 0.28209479177387814;
         for (int m = 0; m <= L; ++m) {
             float cm;
             if (m == 0) cm = 1.0f;
             else if (m == 1) cm = cp;
             else { cm = (2.0f * cp) * cm1 - cm0; cm0 = cm1; cm1 = cm; }
+            if (m) diag = (diag * -sqrt((2.0 * m + 1) / (2.0 * m))) * st;
             const int o = moff<L>(m);
-            float z0 = sseed[m] * (m ? powf(st, (float)m) : 1.0f);
-            if (m > 0 && st < 1e-30f) z0 = 0.0f;
-            float acc = z0 * c[o];
+            const double mult = m ? 1.41421356237309515 : 1.0;
+            double z0 = diag;
+            float acc = ((float)(mult * z0) * cm) * c[o];
             if (m < L) {
-                float z1 = (sqrtf(2.0f * m + 3.0f) * mu) * z0;
-                acc = acc + z1 * c[o + 1];
+                double z1 = (sqrt(2.0 * m + 3.0) * mu) * z0;
+                acc = acc + ((float)(mult * z1) * cm) * c[o + 1];
                 for (int l = m + 2; l <= L; ++l) {
-                    const float fl = (float)l, fm = (float)m;
-                    const float aa = sqrtf((4.0f * fl * fl - 1.0f) / (fl * fl - fm * fm));
-                    const float bb = sqrtf((2.0f * fl + 1.0f) * ((fl - 1.0f) * (fl - 1.0f) - fm * fm) /
-                                           ((2.0f * fl - 3.0f) * (fl * fl - fm * fm)));
-                    const float z2 = (aa * mu) * z1 - bb * z0;
-                    acc = acc + z2 * c[o + l - m];
+                    const int k = kidx(l, m);
+                    const double z2 = (aa[k] * mu) * z1 - bb[k] * z0;
+                    acc = acc + ((float)(mult * z2) * cm) * c[o + l - m];
                     z0 = z1;
                     z1 = z2;
                 }
             }
-            sum = sum + (m ? 1.41421356237309515f * cm : 1.0f) * acc;
+            sum = sum + acc;
         }
         a.records[id] = sum;
     }
@@ -187,8 +211,8 @@ static int sh_launch(const ShArgs &a, bool half, hipStream_t s) {
         if (half) hipLaunchKernelGGL((sh_records_scalar<L, true>), dim3((unsigned)a.items), dim3(256), 0, s, a);
         else hipLaunchKernelGGL((sh_records_scalar<L, false>), dim3((unsigned)a.items), dim3(256), 0, s, a);
     } else {
-        if (half) hipLaunchKernelGGL((sh_records_kernel<G, L, true>), dim3((unsigned)a.items), dim3(G * 8), 0, s, a);
-        else hipLaunchKernelGGL((sh_records_kernel<G, L, false>), dim3((unsigned)a.items), dim3(G * 8), 0, s, a);
+        if (half) hipLaunchKernelGGL((sh_records_kernel<G, L, true>), dim3((unsigned)a.items), dim3(G * 4), 0, s, a);
+        else hipLaunchKernelGGL((sh_records_kernel<G, L, false>), dim3((unsigned)a.items), dim3(G * 4), 0, s, a);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error("sh_records launch: %s", hipGetErrorString(e));

@@ -31,6 +31,23 @@ def padded(a, G):
     return np.concatenate([a, np.repeat(a[..., -1:], G - a.shape[-1], -1)], -1)
 
 
+def physical_field(s, seed):
+    """A random source shaped like a physical one: positive tensor diagonal, SH coefficients decaying like a
+    Henyey-Greenstein source (g ~ 0.6), so that corner traces stay positive and the log-space blend is well posed."""
+    rng = np.random.default_rng(seed)
+    C, rows, W = s.geo.ns, s.rows, s.W
+    f = np.zeros((C, rows, W), 'f4')
+    f[:, 0:3] = rng.uniform(0.5, 1.5, size=(C, 3, W)) * 1e-3
+    f[:, 3] = rng.normal(size=(C, W)) * 1e-4
+    f[:, 4:10] = rng.normal(size=(C, 6, W)) * 1e-4
+    lv = np.array([l for l in range(s.L + 1) for m in range(l + 1)])
+    for p0 in range((rows - 10) // s.K):
+        sh = rng.normal(size=(C, s.K, W)) * (0.6 ** lv)[None, :, None] * 2e-4
+        sh[:, 0] = rng.uniform(1.0, 2.0, size=(C, W)) * 1e-3
+        f[:, 10 + p0 * s.K : 10 + (p0 + 1) * s.K] = sh
+    return f
+
+
 def setup_case(W, L=16, nr=5, state=None, nmu=1, nphi=2, path_n=4, records=True):
     from oracle.solve import OracleSolver
 
@@ -64,8 +81,7 @@ def test_warm_moments_with_boundary(case):
     s, scope, orc = case
     geo = Geometry(s, scope.packs)
     rng = np.random.default_rng(1)
-    field = (np.abs(rng.normal(size=(s.geo.ns, s.rows, s.W))) * 1e-3).astype('f4')
-    field[:, 10:] *= 0.1
+    field = physical_field(s, 1)
     G = s.kernel.constants(s, s.geo.path_n).G
     bnd = (np.abs(rng.normal(size=(s.geo.nc, s.W))) * 1e-2).astype('f4')
     fd = torch.from_numpy(np.ascontiguousarray(padded(field, G))).cuda()
@@ -85,8 +101,7 @@ def test_query_six_components(case):
 
     s, scope, orc = case
     G = s.kernel.constants(s, s.geo.query_n).G
-    rng = np.random.default_rng(2)
-    field = (np.abs(rng.normal(size=(s.geo.ns, s.rows, s.W))) * 1e-3).astype('f4')
+    field = physical_field(s, 2)
     fd = torch.from_numpy(np.ascontiguousarray(padded(field, G))).cuda()
     el, az, _ = s.box_directions()
     _, db = observer_bundle(s, [-3.0, -9.0], el, az)
@@ -258,8 +273,8 @@ def test_rejects_bad_arguments(gpu):
     geo = Geometry(s, scope.packs)
     out = torch.empty((geo.bundle.rays, 1, 4), dtype=torch.float32, device='cuda')
     with pytest.raises(RuntimeError, match='pairs'):
-        hip.call('sos_transport', dc.struct, geo.bundle.struct, 0, 5, s.rows, None, None, None, hip.ptr(out), None,
+        hip.call('sos_transport', dc.struct, geo.bundle.struct, 0, 5, s.rows, None, None, None, None, hip.ptr(out), None,
                  s.kernel.stream())
     with pytest.raises(RuntimeError, match='warm transport needs field'):
-        hip.call('sos_transport', dc.struct, geo.bundle.struct, 0, 4, s.rows, None, None, None, hip.ptr(out), None,
+        hip.call('sos_transport', dc.struct, geo.bundle.struct, 0, 4, s.rows, None, None, None, None, hip.ptr(out), None,
                  s.kernel.stream())
