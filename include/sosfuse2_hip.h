@@ -52,6 +52,7 @@ extern "C" {
 #define SOS_BOUNDARY 2  /* add the attenuated ground boundary term at ray ends that hit the ground */
 #define SOS_MIXED 4     /* SH plane stored as FP16 (mixed precision) */
 #define SOS_RECORDS 8   /* read per-(node, corner) SH values from records_dev instead of reconstructing them */
+#define SOS_UNIFORM_PHASE 16 /* first light: every lane of a ray has the same aerosol and sulfate asymmetry */
 
 typedef struct sos_bundle {
     int64_t groups;               /* destination cells or query groups */

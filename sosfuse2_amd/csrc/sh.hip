@@ -40,12 +40,12 @@ __device__ inline void stage_constants(double *aa, double *bb, int nt) {
 // Per visit: the diagonal seeds 1/sqrt(4 pi) prod dd_j st (FP64, arithmetic.py:95-97) and cos(m phi) by the FP32
 // Chebyshev recurrence (arithmetic.py:112-119).
 template <int L, int V>
-__device__ inline void visit_seeds(int v, double mu, float cp, double (*seed)[V], float (*cmv)[V]) {
+__device__ inline void visit_seeds(int v, double mu, float cp, const double *dd, double (*seed)[V], float (*cmv)[V]) {
     const double st = sqrt(fmax(0.0, 1.0 - mu * mu));
     double diag = 0.28209479177387814;
     float c0 = 1.0f, c1 = cp;
     for (int m = 0; m <= L; ++m) {
-        if (m) diag = (diag * -sqrt((2.0 * m + 1) / (2.0 * m))) * st;
+        if (m) diag = (diag * dd[m]) * st;
         seed[m][v] = diag;
         float cm;
         if (m == 0) cm = 1.0f;
@@ -98,6 +98,7 @@ __global__ __launch_bounds__(G * 4) void sh_records_kernel(ShArgs a) {
     __shared__ double seed[L + 1][V];
     __shared__ float cmv[L + 1][V];
     __shared__ double smu[V];
+    __shared__ double dd[L + 1];
     __shared__ int sid[V];
     const int tid = threadIdx.x, lane = tid & 63;
     const int nt = tid >> 6;
@@ -105,6 +106,7 @@ __global__ __launch_bounds__(G * 4) void sh_records_kernel(ShArgs a) {
     const int cell = a.item_cell[item];
     const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
     stage_constants<L>(aa, bb, NT);
+    for (int m = tid; m <= L; m += NT) dd[m] = m ? -sqrt((2.0 * m + 1) / (2.0 * m)) : 0.0;
     for (int e = tid; e < V * (KZ - K); e += NT) Z[e / (KZ - K)][K + e % (KZ - K)] = 0.0f;
     // B operand in registers: b[s] = c[cell][k'(s, lane)][nt*16 + lane%16]
     float b[KS];
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(G * 4) void sh_records_kernel(ShArgs a) {
             }
             sid[tid] = id;
             smu[tid] = mu;
-            visit_seeds<L, V>(tid, mu, cp, seed, cmv);
+            visit_seeds<L, V>(tid, mu, cp, dd, seed, cmv);
         }
         __syncthreads();
         build_z<L, V, KZ>(Z, smu, seed, cmv, aa, bb, nv, NT);

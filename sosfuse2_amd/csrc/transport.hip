@@ -173,7 +173,7 @@ __global__ void plan_nodes_kernel(TransportArgs a, const int64_t *node_off, int3
 
 enum Aero { DIRECT32 = 0, DIRECT16 = 1, RECORDS = 2 };
 
-template <int G, int L, int P, bool FIRST, bool BOUNDARY, int AERO>
+template <int G, int L, int P, bool FIRST, bool BOUNDARY, int AERO, bool UNI = false>
 __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
     constexpr bool HALF = AERO == DIRECT16;
     constexpr int RPW = 64 / G;
@@ -350,14 +350,38 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
                     for (int c = 0; c < 4; ++c) {
                         const int64_t g0 = a.goff[cells[c]], g1 = a.goff[cells[c] + 1];
                         const float swc = sw[c];
-                        for (int64_t gi = g0; gi < g1; ++gi) {
-                            const float gx = a.gdir[3 * gi], gy = a.gdir[3 * gi + 1], gz = a.gdir[3 * gi + 2];
-                            const float m0 = mus * gz + exd * gx;
-                            const float m1 = eyd * gy;
-                            const float up = clampf(m0 + m1, -1.0f, 1.0f), dn = clampf(m0 - m1, -1.0f, 1.0f);
-                            const float gvv = a.gval[gi * G + w];
-                            ground = ground + (swc * ((hg(up, gasy) + hg(dn, gasy)) / 2.0f)) * gvv;
-                            if (a.strato) sground = sground + (swc * ((hg(up, sg) + hg(dn, sg)) / 2.0f)) * gvv;
+                        if constexpr (UNI && G > 1) {
+                            // the ray's lanes share g: lane j evaluates the phase of pool base + j once and the
+                            // values reach every lane of the ray by a shuffle; sums keep the pool order
+                            for (int64_t base = g0; base < g1; base += G) {
+                                const int64_t gi = base + w;
+                                float pt = 0.0f, ps = 0.0f;
+                                if (gi < g1) {
+                                    const float gx = a.gdir[3 * gi], gy = a.gdir[3 * gi + 1], gz = a.gdir[3 * gi + 2];
+                                    const float m0 = mus * gz + exd * gx;
+                                    const float m1 = eyd * gy;
+                                    const float up = clampf(m0 + m1, -1.0f, 1.0f), dn = clampf(m0 - m1, -1.0f, 1.0f);
+                                    pt = (hg(up, gasy) + hg(dn, gasy)) / 2.0f;
+                                    if (a.strato) ps = (hg(up, sg) + hg(dn, sg)) / 2.0f;
+                                }
+                                const int n = (int)min((int64_t)G, g1 - base);
+                                const float *gv = a.gval + base * G + w;
+                                for (int jj = 0; jj < n; ++jj) {
+                                    const float gvv = gv[jj * G];
+                                    ground = ground + (swc * __shfl(pt, jj, G)) * gvv;
+                                    if (a.strato) sground = sground + (swc * __shfl(ps, jj, G)) * gvv;
+                                }
+                            }
+                        } else {
+                            for (int64_t gi = g0; gi < g1; ++gi) {
+                                const float gx = a.gdir[3 * gi], gy = a.gdir[3 * gi + 1], gz = a.gdir[3 * gi + 2];
+                                const float m0 = mus * gz + exd * gx;
+                                const float m1 = eyd * gy;
+                                const float up = clampf(m0 + m1, -1.0f, 1.0f), dn = clampf(m0 - m1, -1.0f, 1.0f);
+                                const float gvv = a.gval[gi * G + w];
+                                ground = ground + (swc * ((hg(up, gasy) + hg(dn, gasy)) / 2.0f)) * gvv;
+                                if (a.strato) sground = sground + (swc * ((hg(up, sg) + hg(dn, sg)) / 2.0f)) * gvv;
+                            }
                         }
                     }
                     const float beam = (prof[1] * asca) * (sun * phase + ground);
@@ -529,7 +553,7 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
 namespace sos {
 int set_error(const char *fmt, ...);
 
-template <int G, int L, int P, bool FIRST, bool BOUNDARY, int AERO>
+template <int G, int L, int P, bool FIRST, bool BOUNDARY, int AERO, bool UNI = false>
 static int launch_one(const TransportArgs &a, hipStream_t s) {
     int dev = 0, cus = 256;
     hipGetDevice(&dev);
@@ -539,7 +563,7 @@ static int launch_one(const TransportArgs &a, hipStream_t s) {
     if (grid > wgs_needed) grid = wgs_needed;
     grid = ((grid + 7) / 8) * 8;
     if (grid < 8) grid = 8;
-    hipLaunchKernelGGL((transport_kernel<G, L, P, FIRST, BOUNDARY, AERO>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((transport_kernel<G, L, P, FIRST, BOUNDARY, AERO, UNI>), dim3((unsigned)grid), dim3(256), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error("transport launch: %s", hipGetErrorString(e));
     return SOS_OK;
@@ -550,7 +574,9 @@ template <int G, int P>
 static int launch_flags(const TransportArgs &a, int L, hipStream_t s) {
     const bool first = a.flags & SOS_FIRST, bnd = a.flags & SOS_BOUNDARY, half = a.flags & SOS_MIXED;
     const bool records = a.flags & SOS_RECORDS;
-    if (first) return launch_one<G, 8, P, true, false, RECORDS>(a, s);
+    if (first)
+        return (a.flags & SOS_UNIFORM_PHASE) ? launch_one<G, 8, P, true, false, RECORDS, true>(a, s)
+                                             : launch_one<G, 8, P, true, false, RECORDS, false>(a, s);
     if (records)
         return bnd ? launch_one<G, 8, P, false, true, RECORDS>(a, s) : launch_one<G, 8, P, false, false, RECORDS>(a, s);
 #define SOS_DIRECT(LL)                                                                                   \

@@ -9,7 +9,7 @@ c_i32, c_i64, c_u32, c_f32 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32, ct
 P = ctypes.c_void_p
 
 LANE_CONSTANTS = 12
-FIRST, BOUNDARY, MIXED, RECORDS = 1, 2, 4, 8
+FIRST, BOUNDARY, MIXED, RECORDS, UNIFORM_PHASE = 1, 2, 4, 8, 16
 
 
 class Bundle(ctypes.Structure):

@@ -136,6 +136,8 @@ class DeviceConstants:
         self.gval = t(_pad_lanes(gv, G), 'f4')
         self.gm = t(_pad_lanes(c['gm'], G), 'f4')
         self.factor = t(_pad_lanes(ctx.eigen(), G), 'f4')
+        # one asymmetry per launch (a single-state chunk): first light may share phase evaluations across lanes
+        self.uniform_phase = bool(np.all(lanes[6] == lanes[6][0]) and np.all(lanes[11] == lanes[11][0]))
         p = c['profile']
         g = ctx.geo
         self.struct = hip.Grid(
@@ -167,6 +169,7 @@ class Kernel:
         self.device = torch.device(device)
         self.mixed = mixed
         self.records = records  # SH by per-cell MFMA records (True) or per-ray register caches (False)
+        self.uniform_phase = True  # use the shared-phase first light when a chunk's lanes share the asymmetry
         self.xp = torch
         self._constants = {}
         self.metrics = dict(samples=0, launches=0)
@@ -221,6 +224,8 @@ class Kernel:
         dc = self.constants(ctx, db.nq)
         out = torch.empty((db.rays, dc.G, pairs), dtype=torch.float32, device=self.device)
         flags = (hip.FIRST if first else 0) | (hip.BOUNDARY if boundary is not None else 0)
+        if first and dc.uniform_phase and self.uniform_phase:
+            flags |= hip.UNIFORM_PHASE
         records = None
         name = 'first' if first else ('warm' if boundary is not None else
                                       ('query' if pairs == 6 or db.groups != ctx.geo.nc else 'ground'))
