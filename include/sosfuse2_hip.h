@@ -106,13 +106,23 @@ typedef struct sos_grid {
 int sos_transport(const sos_grid *grid, const sos_bundle *bundle, int32_t flags, int32_t pairs, int32_t rows,
                   const float *field_dev, const void *plane_dev, const float *boundary_dev, const float *records_dev,
                   float *out_dev, uint64_t *counters_dev, void *stream);
+/* first light with the ground term read from ground records [nodes][4][lanes] (sos_ground_records) */
+#define SOS_GROUND_RECORDS 32
 
 /* Visit plan of a bundle (fixed by its geometry): counts_dev [rays] = quadrature nodes of every ray; then with
  * node_off_dev = exclusive prefix sum of the counts ([rays + 1]), every node's cell (ir * nc + j, its corner 0),
  * the ray's mu and the node's cos(phi). The caller sorts the visits (node * 4 + corner) by cell. */
 int sos_plan_count(const sos_grid *grid, const sos_bundle *bundle, int64_t *counts_dev, void *stream);
 int sos_plan_nodes(const sos_grid *grid, const sos_bundle *bundle, const int64_t *node_off_dev, int32_t *node_cell_dev,
-                   float *node_mu_dev, float *node_cp_dev, void *stream);
+                   float *node_mu_dev, float *node_cp_dev, float *node_dir_dev, void *stream);
+
+/* First-light ground records: rec[id][w] = asca[w] p_aer(h) sum_p PHt gval[p][w] + ssca[w] p_sul(h) sum_p PHs gval[p][w]
+ * over the ground pools of the visited cell (the pool sums of numpy_reference.py:526-558), for lanes that share the
+ * asymmetries. node_dir_dev: [nodes][4] (ray direction in the node frame: mu, ex, |ey|; node height) from
+ * sos_plan_nodes. The records feed sos_transport's first light through ground_records_dev. */
+int sos_ground_records(const sos_grid *grid, int64_t items, const int32_t *item_cell_dev, const int64_t *item_v0_dev,
+                       const int64_t *item_v1_dev, const int32_t *vis_dev, const float *node_dir_dev,
+                       float *records_dev, void *stream);
 
 /* SH values of every visit as an MFMA contraction per source cell: records[id][w] = sum_k Z[id][k] plane[cell][k][w]
  * with Z the basis row of the visit (the replaced per-corner sum of numpy_reference.py:579-600). Work item i covers

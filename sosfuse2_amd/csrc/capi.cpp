@@ -31,7 +31,8 @@ int transport_dispatch(const TransportArgs &a, int G, int L, hipStream_t s);
 int project_dispatch(const ProjectArgs &a, int G, int L, hipStream_t s);
 int plan_count_launch(const TransportArgs &a, int64_t *counts, hipStream_t s);
 int plan_nodes_launch(const TransportArgs &a, const int64_t *node_off, int32_t *cell, float *mu, float *cp,
-                      hipStream_t s);
+                      float *dir, hipStream_t s);
+int ground_records_dispatch(const GroundArgs &a, hipStream_t s);
 
 int sh_records_dispatch(const ShArgs &a, bool half, hipStream_t s);
 int compose_launch(int64_t cells, int rows, int K, int G, const float *mom, const float *first, const float *factor,
@@ -90,13 +91,31 @@ int sos_plan_count(const sos_grid *g, const sos_bundle *b, int64_t *counts_dev, 
 }
 
 int sos_plan_nodes(const sos_grid *g, const sos_bundle *b, const int64_t *node_off_dev, int32_t *node_cell_dev,
-                   float *node_mu_dev, float *node_cp_dev, void *stream) {
+                   float *node_mu_dev, float *node_cp_dev, float *node_dir_dev, void *stream) {
     if (!g || !b || !node_off_dev || !node_cell_dev || !node_mu_dev || !node_cp_dev)
         return arg_error("sos_plan_nodes: null argument");
     if (g->nh < 2 || g->nc < 2) return arg_error("sos_plan_nodes: grid too small");
     TransportArgs a;
     fill_args(a, g, b);
-    return plan_nodes_launch(a, node_off_dev, node_cell_dev, node_mu_dev, node_cp_dev, (hipStream_t)stream);
+    return plan_nodes_launch(a, node_off_dev, node_cell_dev, node_mu_dev, node_cp_dev, node_dir_dev,
+                             (hipStream_t)stream);
+}
+
+int sos_ground_records(const sos_grid *g, int64_t items, const int32_t *item_cell_dev, const int64_t *item_v0_dev,
+                       const int64_t *item_v1_dev, const int32_t *vis_dev, const float *node_dir_dev,
+                       float *records_dev, void *stream) {
+    if (!g || !item_cell_dev || !item_v0_dev || !item_v1_dev || !vis_dev || !node_dir_dev || !records_dev ||
+        !g->goff_dev || !g->gdir_dev || !g->gval_dev || !g->lane_dev)
+        return arg_error("sos_ground_records: null argument");
+    if (g->lanes != 16 && g->lanes != 32) return arg_error("sos_ground_records: lanes %d (16 or 32)", g->lanes);
+    GroundArgs a;
+    a.G = g->lanes; a.strato = g->strato;
+    a.shape = ProfileShape{1.0f / g->air_H, 1.0f / g->aer_H, g->o3_c, 1.0f / g->o3_s,
+                           g->no2_c, 1.0f / g->no2_s, g->st_c, 1.0f / g->st_s};
+    a.items = items; a.item_cell = item_cell_dev; a.item_v0 = item_v0_dev; a.item_v1 = item_v1_dev; a.vis = vis_dev;
+    a.node_dir = reinterpret_cast<const float4 *>(node_dir_dev); a.goff = g->goff_dev; a.gdir = g->gdir_dev;
+    a.gval = g->gval_dev; a.lane = g->lane_dev; a.records = records_dev;
+    return ground_records_dispatch(a, (hipStream_t)stream);
 }
 
 int sos_sh_records(const sos_grid *g, const void *plane_dev, int32_t half, int64_t items, const int32_t *item_cell_dev,
@@ -150,8 +169,10 @@ int sos_transport(const sos_grid *g, const sos_bundle *b, int32_t flags, int32_t
     a.end_j = b->end_j_dev; a.end_f = b->end_f_dev; a.end_col = b->end_col_dev; a.hits = b->hits_dev;
     a.field = field_dev; a.plane = plane_dev; a.boundary = boundary_dev; a.out = out_dev;
     a.counters = (unsigned long long *)counters_dev;
-    a.records = records_dev;
+    a.records = (flags & SOS_FIRST) ? nullptr : records_dev;
+    a.ground_records = ((flags & SOS_FIRST) && (flags & SOS_GROUND_RECORDS)) ? records_dev : nullptr;
     a.node_off = b->node_off_dev;
+    if (a.ground_records && !a.node_off) return arg_error("sos_transport: ground records need the node offsets");
     return transport_dispatch(a, g->lanes, g->L, (hipStream_t)stream);
 }
 

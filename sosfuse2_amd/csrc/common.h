@@ -89,6 +89,7 @@ struct TransportArgs {
     unsigned long long *counters;
     const float *records;    // [nodes][4][G] SH values per (node, corner) (records mode)
     const int64_t *node_off;  // [rays + 1] first node of every ray (records mode)
+    const float *ground_records;  // [nodes][4][G] scattered ground light per (node, corner) (first light)
 };
 
 struct ProjectArgs {
@@ -108,6 +109,20 @@ struct ShArgs {
     const int64_t *item_v0, *item_v1;
     const int32_t *vis;        // [visits] visit ids (node * 4 + corner), sorted by cell
     const float *node_mu, *node_cp;
+    float *records;            // [nodes * 4][G]
+};
+
+struct GroundArgs {
+    int G, strato;
+    ProfileShape shape;
+    int64_t items;
+    const int32_t *item_cell;
+    const int64_t *item_v0, *item_v1;
+    const int32_t *vis;
+    const float4 *node_dir;    // [nodes] (mu_s, exd, |eyd|, h)
+    const int64_t *goff;       // [cells + 1]
+    const float *gdir, *gval;  // [pools][3], [pools][G]
+    const float *lane;         // [SOS_LANE_CONSTANTS][G]
     float *records;            // [nodes * 4][G]
 };
 

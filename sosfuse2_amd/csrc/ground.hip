@@ -1,0 +1,125 @@
+// First-light ground term as an MFMA contraction per destination-grid cell (gfx950).
+//
+// For every (node, corner) visit of a first-light launch, the sunlit-ground light scattered by the aerosol layers:
+//   rec[visit][w] = asca[w] p_aer(h) sum_p PHt[visit][p] gv[cell][p][w] + ssca[w] p_sul(h) sum_p PHs[visit][p] gv[p][w]
+// with PH the paired Henyey-Greenstein phases (hg(m0 + m1) + hg(m0 - m1)) / 2 between the ray direction in the node
+// frame and the cell's pool direction (src/sosfuse2/backends/numpy_reference.py:526-558). The phases do not depend
+// on the lane when every lane shares the asymmetries (one state per launch), so the pool sum is a dense
+// [visits x pools] x [pools x lanes] product: per cell the pool radiances are staged once in LDS, phase tiles of 16
+// visits are generated in LDS, and v_mfma_f32_16x16x4f32 accumulates them. The transport then adds
+// sum_corners sw * rec instead of walking the pools of four cells at every node.
+
+#include "common.h"
+
+namespace sos {
+int set_error(const char *fmt, ...);
+
+template <int G>
+__global__ __launch_bounds__(G * 4) void ground_records_kernel(GroundArgs a) {
+    constexpr int V = 16, PC = 128, PS = PC + 4;
+    constexpr int NT = G * 4;
+    __shared__ float B[PC][G];
+    __shared__ float PT[V][PS], PU[V][PS];
+    __shared__ float gd[PC][3];
+    __shared__ float4 sdir[V];
+    __shared__ int sid[V];
+    const int tid = threadIdx.x, lane = tid & 63, nt = tid >> 6;
+    const int col = nt * 16 + (lane & 15);
+    const int64_t item = blockIdx.x;
+    const int cell = a.item_cell[item];
+    const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
+    const int64_t p0 = a.goff[cell];
+    const int np = (int)(a.goff[cell + 1] - p0);
+    const int chunks = (np + PC - 1) / PC;
+    const float g = a.lane[SOS_LANE_G * G], sg = a.lane[SOS_LANE_SG * G];
+    const float asca = a.lane[SOS_LANE_ASCA * G + col], ssca = a.lane[SOS_LANE_SSCA * G + col];
+    auto stage = [&](int ch) {
+        for (int e = tid; e < PC * G; e += NT) {
+            const int p = e / G, w = e % G;
+            const int pp = ch * PC + p;
+            B[p][w] = pp < np ? a.gval[(p0 + pp) * G + w] : 0.0f;
+        }
+        for (int e = tid; e < PC * 3; e += NT) {
+            const int p = e / 3, c = e % 3;
+            const int pp = ch * PC + p;
+            gd[p][c] = pp < np ? a.gdir[(p0 + pp) * 3 + c] : 0.0f;
+        }
+    };
+    if (chunks == 1) stage(0);
+    for (int64_t r0 = v0; r0 < v1; r0 += V) {
+        const int nv = (int)min((int64_t)V, v1 - r0);
+        if (tid < V) {
+            int id = -1;
+            float4 dv = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (tid < nv) {
+                id = a.vis[r0 + tid];
+                dv = a.node_dir[id >> 2];
+            }
+            sid[tid] = id;
+            sdir[tid] = dv;
+        }
+        __syncthreads();
+        floatx4 acct = floatx4{0.f, 0.f, 0.f, 0.f}, accs = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int ch = 0; ch < chunks; ++ch) {
+            if (chunks > 1) {
+                stage(ch);
+                __syncthreads();
+            }
+            const int n = min(PC, np - ch * PC);
+            for (int e = tid; e < V * PC; e += NT) {
+                const int v = e / PC, p = e % PC;
+                float pt = 0.0f, ps = 0.0f;
+                if (v < nv && p < n) {
+                    const float4 dv = sdir[v];
+                    const float m0 = dv.x * gd[p][2] + dv.y * gd[p][0];
+                    const float m1 = dv.z * gd[p][1];
+                    const float up = clampf(m0 + m1, -1.0f, 1.0f), dn = clampf(m0 - m1, -1.0f, 1.0f);
+                    pt = (hg(up, g) + hg(dn, g)) / 2.0f;
+                    if (a.strato) ps = (hg(up, sg) + hg(dn, sg)) / 2.0f;
+                }
+                PT[v][p] = pt;
+                PU[v][p] = ps;
+            }
+            __syncthreads();
+            const int kk = lane >> 4, row = lane & 15;
+#pragma unroll 8
+            for (int s = 0; s < PC / 4; ++s) {
+                const float b = B[4 * s + kk][col];
+                acct = __builtin_amdgcn_mfma_f32_16x16x4f32(PT[row][4 * s + kk], b, acct, 0, 0, 0);
+                if (a.strato) accs = __builtin_amdgcn_mfma_f32_16x16x4f32(PU[row][4 * s + kk], b, accs, 0, 0, 0);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = 4 * (lane >> 4) + v;
+            if (row < nv) {
+                float prof[5], st;
+                profiles(a.shape, sdir[row].w, prof, st);
+                float rec = (prof[1] * asca) * acct[v];
+                if (a.strato) rec = rec + (st * ssca) * accs[v];
+                a.records[(int64_t)sid[row] * G + col] = rec;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int G>
+static int ground_launch(const GroundArgs &a, hipStream_t s) {
+    if (a.items <= 0) return SOS_OK;
+    hipLaunchKernelGGL((ground_records_kernel<G>), dim3((unsigned)a.items), dim3(G * 4), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error("ground_records launch: %s", hipGetErrorString(e));
+    return SOS_OK;
+}
+
+int ground_records_dispatch(const GroundArgs &a, hipStream_t s) {
+    switch (a.G) {
+        case 16: return ground_launch<16>(a, s);
+        case 32: return ground_launch<32>(a, s);
+        default: return set_error("ground_records: lanes %d (16 or 32)", a.G);
+    }
+}
+
+}  // namespace sos

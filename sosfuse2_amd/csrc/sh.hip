@@ -55,45 +55,70 @@ __device__ inline void visit_seeds(int v, double mu, float cp, const double *dd,
     }
 }
 
-// Z rows of V visits into LDS: chain (visit, m) runs the Legendre recurrence over l in FP64 and rounds each basis
-// value to FP32 before the FP32 product with cos(m phi) (the reference's FP64 basis, rounded).
+// Z rows of V visits into LDS. Task (visit, p) runs the Legendre recurrence over l in FP64 for the two orders
+// m = p and m = L - p side by side (their lengths add up to L + 2, so tasks are balanced and each thread has two
+// independent dependency chains); each basis value is rounded to FP32 before the FP32 product with cos(m phi)
+// (the reference's FP64 basis, rounded).
 template <int L, int V, int KZ>
 __device__ inline void build_z(float (*Z)[KZ], const double *mus, const double (*seed)[V], const float (*cmv)[V],
                                const double *aa, const double *bb, int nv, int nt) {
-    for (int c = threadIdx.x; c < V * (L + 1); c += nt) {
-        const int v = c % V, m = c / V;
-        const int o = moff<L>(m);
+    constexpr int PAIRS = L / 2 + 1;
+    for (int task = threadIdx.x; task < V * PAIRS; task += nt) {
+        const int v = task % V, p = task / V;
+        const int ma = p, mb = L - p;
+        const bool two = mb != ma;
+        const int oa = moff<L>(ma), ob = moff<L>(mb);
         if (v >= nv) {
-            for (int l = m; l <= L; ++l) Z[v][o + l - m] = 0.0f;
+            for (int l = ma; l <= L; ++l) Z[v][oa + l - ma] = 0.0f;
+            if (two)
+                for (int l = mb; l <= L; ++l) Z[v][ob + l - mb] = 0.0f;
             continue;
         }
         const double mu = mus[v];
-        const float cm = cmv[m][v];
-        const double mult = m ? 1.41421356237309515 : 1.0;
-        double z0 = seed[m][v];
-        Z[v][o] = (float)(mult * z0) * cm;
-        if (m < L) {
-            double z1 = (sqrt(2.0 * m + 3.0) * mu) * z0;
-            Z[v][o + 1] = (float)(mult * z1) * cm;
-            for (int l = m + 2; l <= L; ++l) {
-                const int k = kidx(l, m);
-                const double z2 = (aa[k] * mu) * z1 - bb[k] * z0;
-                Z[v][o + l - m] = (float)(mult * z2) * cm;
-                z0 = z1;
-                z1 = z2;
-            }
+        const float ca = cmv[ma][v], cb = cmv[mb][v];
+        const double xa = ma ? 1.41421356237309515 : 1.0, xb = 1.41421356237309515;
+        // chain a: m = p (long); chain b: m = L - p (short, length p + 1)
+        double a0 = seed[ma][v], b0 = seed[mb][v];
+        Z[v][oa] = (float)(xa * a0) * ca;
+        if (two) Z[v][ob] = (float)(xb * b0) * cb;
+        double a1 = (sqrt(2.0 * ma + 3.0) * mu) * a0;  // ma < L always
+        Z[v][oa + 1] = (float)(xa * a1) * ca;
+        double b1 = 0.0;
+        if (two && mb < L) {
+            b1 = (sqrt(2.0 * mb + 3.0) * mu) * b0;
+            Z[v][ob + 1] = (float)(xb * b1) * cb;
+        }
+        // both chains advance together while chain b lasts
+        int la = ma + 2, lb = mb + 2;
+#pragma unroll 2
+        for (; two && lb <= L; ++la, ++lb) {
+            const int ka = kidx(la, ma), kb = kidx(lb, mb);
+            const double a2 = (aa[ka] * mu) * a1 - bb[ka] * a0;
+            const double b2 = (aa[kb] * mu) * b1 - bb[kb] * b0;
+            Z[v][oa + la - ma] = (float)(xa * a2) * ca;
+            Z[v][ob + lb - mb] = (float)(xb * b2) * cb;
+            a0 = a1; a1 = a2;
+            b0 = b1; b1 = b2;
+        }
+#pragma unroll 4
+        for (; la <= L; ++la) {
+            const int ka = kidx(la, ma);
+            const double a2 = (aa[ka] * mu) * a1 - bb[ka] * a0;
+            Z[v][oa + la - ma] = (float)(xa * a2) * ca;
+            a0 = a1; a1 = a2;
         }
     }
 }
 
 template <int G, int L, bool HALF>
-__global__ __launch_bounds__(G * 4) void sh_records_kernel(ShArgs a) {
+__global__ __launch_bounds__(G * 4) __attribute__((amdgpu_waves_per_eu(2, 2))) void sh_records_kernel(ShArgs a) {
     constexpr int K = (L + 1) * (L + 2) / 2;
     constexpr int KZ = (K + 15) / 16 * 16;   // padded row
     constexpr int KS = KZ / 4;               // MFMA k-steps
     constexpr int NT = G * 4;                // one wavefront per 16-lane N-tile
     constexpr int V = 16;                    // visits per round (one M-tile)
-    __shared__ float Z[V][KZ];
+    constexpr int ZS = KZ + 4;               // row stride = 4 (mod 32 banks): 16 rows spread over the LDS banks
+    __shared__ float Z[V][ZS];
     __shared__ double aa[K], bb[K];
     __shared__ double seed[L + 1][V];
     __shared__ float cmv[L + 1][V];
@@ -136,7 +161,7 @@ __global__ __launch_bounds__(G * 4) void sh_records_kernel(ShArgs a) {
             visit_seeds<L, V>(tid, mu, cp, dd, seed, cmv);
         }
         __syncthreads();
-        build_z<L, V, KZ>(Z, smu, seed, cmv, aa, bb, nv, NT);
+        build_z<L, V, ZS>(Z, smu, seed, cmv, aa, bb, nv, NT);
         __syncthreads();
         floatx4 acc[4];
 #pragma unroll

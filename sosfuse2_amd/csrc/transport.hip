@@ -141,7 +141,7 @@ __global__ void plan_count_kernel(TransportArgs a, int64_t *counts) {
 }
 
 __global__ void plan_nodes_kernel(TransportArgs a, const int64_t *node_off, int32_t *node_cell, float *node_mu,
-                                  float *node_cp_out) {
+                                  float *node_cp_out, float4 *node_dir) {
     const int64_t ray = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (ray >= a.rays) return;
     const int grp = a.ray_group[ray];
@@ -167,6 +167,18 @@ __global__ void plan_nodes_kernel(TransportArgs a, const int64_t *node_off, int3
             node_cell[n] = ng.ir * a.nc + ng.j;
             node_mu[n] = mu;
             node_cp_out[n] = node_cp(st, df0, df1, ng.cx, ng.sx);
+            if (node_dir) {
+                // the ray direction in the node's frame (numpy_reference.py:527-533) and the node height
+                const float df2 = mu;
+                const float Q[9] = {-ng.cc * ng.cx, ng.sx, ng.sn * ng.cx, -ng.cc * ng.sx, -ng.cx, ng.sn * ng.sx,
+                                    ng.sn, 0.0f, ng.cc};
+                float4 o;
+                o.x = (df0 * Q[2] + df1 * Q[5]) + df2 * Q[8];
+                o.y = (df0 * Q[0] + df1 * Q[3]) + df2 * Q[6];
+                o.z = fabsf((df0 * Q[1] + df1 * Q[4]) + df2 * Q[7]);
+                o.w = ng.h;
+                node_dir[n] = o;
+            }
         }
     }
 }
@@ -232,7 +244,7 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
         float cache[AERO == RECORDS ? 1 : 4][AERO == RECORDS ? 1 : L + 1];
         int tags[4] = {-1, -1, -1, -1};
         unsigned nodes = 0, refills = 0;
-        const int64_t node_base = AERO == RECORDS ? a.node_off[ray] : 0;
+        const int64_t node_base = (AERO == RECORDS || (FIRST && a.ground_records)) ? a.node_off[ray] : 0;
         const int64_t e0 = a.edge_off[ray];
         const int nseg = (int)(a.edge_off[ray + 1] - e0) - 1;
         const unsigned bits = a.lower[ray];
@@ -346,43 +358,51 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
                     eyd = fabsf((df0 * Q[1] + df1 * Q[4]) + df2 * Q[7]);
                     float ground = 0.0f, sground = 0.0f;
                     const float sg = lc[SOS_LANE_SG];
-#pragma unroll 1
-                    for (int c = 0; c < 4; ++c) {
-                        const int64_t g0 = a.goff[cells[c]], g1 = a.goff[cells[c] + 1];
-                        const float swc = sw[c];
-                        if constexpr (UNI && G > 1) {
-                            // the ray's lanes share g: lane j evaluates the phase of pool base + j once and the
-                            // values reach every lane of the ray by a shuffle; sums keep the pool order
-                            for (int64_t base = g0; base < g1; base += G) {
-                                const int64_t gi = base + w;
-                                float pt = 0.0f, ps = 0.0f;
-                                if (gi < g1) {
+                    float grec = 0.0f;  // scattered ground light of the four corners from the ground records
+                    if (a.ground_records) {
+                        const float *rec = a.ground_records + (node_base + nodes - 1) * 4 * G + w;
+                        grec = ((sw[0] * rec[0] + sw[1] * rec[G]) + sw[2] * rec[2 * G]) + sw[3] * rec[3 * G];
+                    } else {
+    #pragma unroll 1
+                        for (int c = 0; c < 4; ++c) {
+                            const int64_t g0 = a.goff[cells[c]], g1 = a.goff[cells[c] + 1];
+                            const float swc = sw[c];
+                            if constexpr (UNI && G > 1) {
+                                // the ray's lanes share g: lane j evaluates the phase of pool base + j once and the
+                                // values reach every lane of the ray by a shuffle; sums keep the pool order
+                                for (int64_t base = g0; base < g1; base += G) {
+                                    const int64_t gi = base + w;
+                                    float pt = 0.0f, ps = 0.0f;
+                                    if (gi < g1) {
+                                        const float gx = a.gdir[3 * gi], gy = a.gdir[3 * gi + 1], gz = a.gdir[3 * gi + 2];
+                                        const float m0 = mus * gz + exd * gx;
+                                        const float m1 = eyd * gy;
+                                        const float up = clampf(m0 + m1, -1.0f, 1.0f), dn = clampf(m0 - m1, -1.0f, 1.0f);
+                                        pt = (hg(up, gasy) + hg(dn, gasy)) / 2.0f;
+                                        if (a.strato) ps = (hg(up, sg) + hg(dn, sg)) / 2.0f;
+                                    }
+                                    const int n = (int)min((int64_t)G, g1 - base);
+                                    const float *gv = a.gval + base * G + w;
+    #pragma unroll 4
+                                    for (int jj = 0; jj < n; ++jj) {
+                                        const float gvv = gv[jj * G];
+                                        ground = ground + (swc * __shfl(pt, jj, G)) * gvv;
+                                        if (a.strato) sground = sground + (swc * __shfl(ps, jj, G)) * gvv;
+                                    }
+                                }
+                            } else {
+                                for (int64_t gi = g0; gi < g1; ++gi) {
                                     const float gx = a.gdir[3 * gi], gy = a.gdir[3 * gi + 1], gz = a.gdir[3 * gi + 2];
                                     const float m0 = mus * gz + exd * gx;
                                     const float m1 = eyd * gy;
                                     const float up = clampf(m0 + m1, -1.0f, 1.0f), dn = clampf(m0 - m1, -1.0f, 1.0f);
-                                    pt = (hg(up, gasy) + hg(dn, gasy)) / 2.0f;
-                                    if (a.strato) ps = (hg(up, sg) + hg(dn, sg)) / 2.0f;
+                                    const float gvv = a.gval[gi * G + w];
+                                    ground = ground + (swc * ((hg(up, gasy) + hg(dn, gasy)) / 2.0f)) * gvv;
+                                    if (a.strato) sground = sground + (swc * ((hg(up, sg) + hg(dn, sg)) / 2.0f)) * gvv;
                                 }
-                                const int n = (int)min((int64_t)G, g1 - base);
-                                const float *gv = a.gval + base * G + w;
-                                for (int jj = 0; jj < n; ++jj) {
-                                    const float gvv = gv[jj * G];
-                                    ground = ground + (swc * __shfl(pt, jj, G)) * gvv;
-                                    if (a.strato) sground = sground + (swc * __shfl(ps, jj, G)) * gvv;
-                                }
-                            }
-                        } else {
-                            for (int64_t gi = g0; gi < g1; ++gi) {
-                                const float gx = a.gdir[3 * gi], gy = a.gdir[3 * gi + 1], gz = a.gdir[3 * gi + 2];
-                                const float m0 = mus * gz + exd * gx;
-                                const float m1 = eyd * gy;
-                                const float up = clampf(m0 + m1, -1.0f, 1.0f), dn = clampf(m0 - m1, -1.0f, 1.0f);
-                                const float gvv = a.gval[gi * G + w];
-                                ground = ground + (swc * ((hg(up, gasy) + hg(dn, gasy)) / 2.0f)) * gvv;
-                                if (a.strato) sground = sground + (swc * ((hg(up, sg) + hg(dn, sg)) / 2.0f)) * gvv;
                             }
                         }
+
                     }
                     const float beam = (prof[1] * asca) * (sun * phase + ground);
                     const float ray_ = (sun * prof[0]) * lc[0];
@@ -393,6 +413,7 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
                                          (1.0f - mix) * 0.0397887357729738339f * pu[q]) +
                                  beam * pu[q] / 2.0f;
                         if (a.strato) val[q] = val[q] + (sbeam * pu[q]) / 2.0f;
+                        if (a.ground_records) val[q] = val[q] + (grec * pu[q]) / 2.0f;
                     }
                     // Rayleigh scattering of the ground-reflected sun (gm), density at the node
 #pragma unroll
@@ -617,9 +638,11 @@ int plan_count_launch(const TransportArgs &a, int64_t *counts, hipStream_t s) {
 }
 
 int plan_nodes_launch(const TransportArgs &a, const int64_t *node_off, int32_t *cell, float *mu, float *cp,
-                      hipStream_t s) {
+                      float *dir, hipStream_t s) {
     const unsigned b = (unsigned)((a.rays + 255) / 256);
-    if (b) hipLaunchKernelGGL(plan_nodes_kernel, dim3(b), dim3(256), 0, s, a, node_off, cell, mu, cp);
+    if (b)
+        hipLaunchKernelGGL(plan_nodes_kernel, dim3(b), dim3(256), 0, s, a, node_off, cell, mu, cp,
+                           reinterpret_cast<float4 *>(dir));
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? SOS_OK : set_error("plan_nodes: %s", hipGetErrorString(e));
 }

@@ -84,8 +84,9 @@ class Plan:
         self.node_cell = torch.empty(max(N, 1), dtype=torch.int32, device=dev)
         self.node_mu = torch.empty(max(N, 1), dtype=torch.float32, device=dev)
         self.node_cp = torch.empty(max(N, 1), dtype=torch.float32, device=dev)
+        self.node_dir = torch.empty((max(N, 1), 4), dtype=torch.float32, device=dev)
         hip.call('sos_plan_nodes', dc.struct, db.struct, hip.ptr(self.node_off), hip.ptr(self.node_cell),
-                 hip.ptr(self.node_mu), hip.ptr(self.node_cp), st)
+                 hip.ptr(self.node_mu), hip.ptr(self.node_cp), hip.ptr(self.node_dir), st)
         nc = dc.struct.nc
         cells = dc.struct.nh * nc
         offs = torch.tensor([0, 1, nc, nc + 1], dtype=torch.int32, device=dev)
@@ -224,9 +225,19 @@ class Kernel:
         dc = self.constants(ctx, db.nq)
         out = torch.empty((db.rays, dc.G, pairs), dtype=torch.float32, device=self.device)
         flags = (hip.FIRST if first else 0) | (hip.BOUNDARY if boundary is not None else 0)
+        records = None
         if first and dc.uniform_phase and self.uniform_phase:
             flags |= hip.UNIFORM_PHASE
-        records = None
+            if self.records and dc.G > 1:
+                pl = self.plan(db, dc)
+                e0 = self._event()
+                hip.call('sos_ground_records', dc.struct, pl.items, hip.ptr(pl.item_cell), hip.ptr(pl.item_v0),
+                         hip.ptr(pl.item_v1), hip.ptr(pl.vis), hip.ptr(pl.node_dir), hip.ptr(pl.records),
+                         self.stream())
+                if e0 is not None:
+                    self.timed.append(('ground_records', e0, self._event()))
+                records = pl.records
+                flags |= hip.GROUND_RECORDS
         name = 'first' if first else ('warm' if boundary is not None else
                                       ('query' if pairs == 6 or db.groups != ctx.geo.nc else 'ground'))
         if not first:
