@@ -244,7 +244,9 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
         float cache[AERO == RECORDS ? 1 : 4][AERO == RECORDS ? 1 : L + 1];
         int tags[4] = {-1, -1, -1, -1};
         unsigned nodes = 0, refills = 0;
-        const int64_t node_base = (AERO == RECORDS || (FIRST && a.ground_records)) ? a.node_off[ray] : 0;
+        // node offsets exist only when records are read: warm records, or first light with ground records
+        const bool uses_nodes = FIRST ? (a.ground_records != nullptr) : (AERO == RECORDS);
+        const int64_t node_base = uses_nodes ? a.node_off[ray] : 0;
         const int64_t e0 = a.edge_off[ray];
         const int nseg = (int)(a.edge_off[ray + 1] - e0) - 1;
         const unsigned bits = a.lower[ray];
