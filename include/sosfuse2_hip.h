@@ -126,10 +126,11 @@ int sos_ground_records(const sos_grid *grid, int64_t items, const int32_t *item_
 
 /* SH values of every visit as an MFMA contraction per source cell: records[id][w] = sum_k Z[id][k] plane[cell][k][w]
  * with Z the basis row of the visit (the replaced per-corner sum of numpy_reference.py:579-600). Work item i covers
- * the visits vis_dev[v0[i] .. v1[i]) of cell item_cell[i]; vis_dev holds visit ids node * 4 + corner. */
+ * the visits vis_dev[v0[i] .. v1[i]) of cell item_cell[i]; vis_dev holds visit ids node * 4 + corner, and
+ * vis_mu_dev / vis_cp_dev the ray's mu and the node's cos(phi) in the same (sorted) order. */
 int sos_sh_records(const sos_grid *grid, const void *plane_dev, int32_t half, int64_t items, const int32_t *item_cell_dev,
                    const int64_t *item_v0_dev, const int64_t *item_v1_dev, const int32_t *vis_dev,
-                   const float *node_mu_dev, const float *node_cp_dev, float *records_dev, void *stream);
+                   const float *vis_mu_dev, const float *vis_cp_dev, float *records_dev, void *stream);
 
 /* Moments [groups][4 + K][lanes] and ground flux [groups][lanes] of the per-ray radiances rad_dev
  * [rays][lanes][4] (the bundle's groups are the destination cells). ground_only writes the flux only. */

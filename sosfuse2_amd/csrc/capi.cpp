@@ -120,15 +120,15 @@ int sos_ground_records(const sos_grid *g, int64_t items, const int32_t *item_cel
 
 int sos_sh_records(const sos_grid *g, const void *plane_dev, int32_t half, int64_t items, const int32_t *item_cell_dev,
                    const int64_t *item_v0_dev, const int64_t *item_v1_dev, const int32_t *vis_dev,
-                   const float *node_mu_dev, const float *node_cp_dev, float *records_dev, void *stream) {
-    if (!g || !plane_dev || !item_cell_dev || !item_v0_dev || !item_v1_dev || !vis_dev || !node_mu_dev ||
-        !node_cp_dev || !records_dev)
+                   const float *vis_mu_dev, const float *vis_cp_dev, float *records_dev, void *stream) {
+    if (!g || !plane_dev || !item_cell_dev || !item_v0_dev || !item_v1_dev || !vis_dev || !vis_mu_dev ||
+        !vis_cp_dev || !records_dev)
         return arg_error("sos_sh_records: null argument");
     if (!lanes_ok(g->lanes) || !order_ok(g->L)) return arg_error("sos_sh_records: lanes %d / L %d", g->lanes, g->L);
     ShArgs a;
     a.L = g->L; a.G = g->lanes; a.K = (g->L + 1) * (g->L + 2) / 2; a.plane = plane_dev; a.items = items;
     a.item_cell = item_cell_dev; a.item_v0 = item_v0_dev; a.item_v1 = item_v1_dev; a.vis = vis_dev;
-    a.node_mu = node_mu_dev; a.node_cp = node_cp_dev; a.records = records_dev;
+    a.vis_mu = vis_mu_dev; a.vis_cp = vis_cp_dev; a.records = records_dev;
     return sh_records_dispatch(a, half != 0, (hipStream_t)stream);
 }
 

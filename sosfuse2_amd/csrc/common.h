@@ -108,7 +108,7 @@ struct ShArgs {
     const int32_t *item_cell;  // [items]
     const int64_t *item_v0, *item_v1;
     const int32_t *vis;        // [visits] visit ids (node * 4 + corner), sorted by cell
-    const float *node_mu, *node_cp;
+    const float *vis_mu, *vis_cp;  // [visits] the ray's mu and the node's cos(phi), in sorted visit order
     float *records;            // [nodes * 4][G]
 };
 

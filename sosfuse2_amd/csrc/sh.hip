@@ -144,21 +144,31 @@ __global__ __launch_bounds__(G * 4) __attribute__((amdgpu_waves_per_eu(2, 2))) v
             b[s] = k < K ? plane_at<L, HALF>(a.plane, base + (int64_t)k * G) : 0.0f;
         }
     }
+    // visit data of the next round is loaded one round ahead (contiguous: the plan stores mu and cos(phi) in sorted
+    // visit order), so its memory latency overlaps the current round's work
+    int nid = -1;
+    float nmu = 0.0f, ncp = 1.0f;
+    if (tid < V && v0 + tid < v1) {
+        nid = a.vis[v0 + tid];
+        nmu = a.vis_mu[v0 + tid];
+        ncp = a.vis_cp[v0 + tid];
+    }
     for (int64_t r0 = v0; r0 < v1; r0 += V) {
         const int nv = (int)min((int64_t)V, v1 - r0);
         if (tid < V) {
-            int id = -1;
-            double mu = 0.0;
-            float cp = 1.0f;
-            if (tid < nv) {
-                id = a.vis[r0 + tid];
-                const int64_t node = id >> 2;
-                mu = (double)a.node_mu[node];
-                cp = a.node_cp[node];
+            const int id = nid;
+            const double mu = (double)nmu;
+            const float cp = ncp;
+            const int64_t nx = r0 + V + tid;
+            nid = -1;
+            if (nx < v1) {
+                nid = a.vis[nx];
+                nmu = a.vis_mu[nx];
+                ncp = a.vis_cp[nx];
             }
-            sid[tid] = id;
-            smu[tid] = mu;
-            visit_seeds<L, V>(tid, mu, cp, dd, seed, cmv);
+            sid[tid] = tid < nv ? id : -1;
+            smu[tid] = tid < nv ? mu : 0.0;
+            visit_seeds<L, V>(tid, tid < nv ? mu : 0.0, tid < nv ? cp : 1.0f, dd, seed, cmv);
         }
         __syncthreads();
         build_z<L, V, ZS>(Z, smu, seed, cmv, aa, bb, nv, NT);
@@ -197,9 +207,8 @@ __global__ __launch_bounds__(256) void sh_records_scalar(ShArgs a) {
     __syncthreads();
     for (int64_t r = a.item_v0[item] + threadIdx.x; r < a.item_v1[item]; r += 256) {
         const int id = a.vis[r];
-        const int64_t node = id >> 2;
-        const double mu = a.node_mu[node];
-        const float cp = a.node_cp[node];
+        const double mu = a.vis_mu[r];
+        const float cp = a.vis_cp[r];
         const double st = sqrt(fmax(0.0, 1.0 - mu * mu));
         float sum = 0.0f;
         float cm0 = 1.0f, cm1 = cp;

@@ -92,6 +92,10 @@ class Plan:
         offs = torch.tensor([0, 1, nc, nc + 1], dtype=torch.int32, device=dev)
         vcell = (self.node_cell[:N, None] + offs[None, :]).reshape(-1)
         self.vis = torch.argsort(vcell, stable=True).to(torch.int32)
+        node_of = (self.vis >> 2).long()
+        self.vis_mu = self.node_mu[node_of]
+        self.vis_cp = self.node_cp[node_of]
+        del node_of
         per_cell = torch.bincount(vcell, minlength=cells)
         del vcell
         cell_off = torch.zeros(cells + 1, dtype=torch.int64, device=dev)
@@ -249,8 +253,8 @@ class Kernel:
                 pl = self.plan(db, dc)
                 e0 = self._event()
                 hip.call('sos_sh_records', dc.struct, hip.ptr(plane), int(self.mixed), pl.items, hip.ptr(pl.item_cell),
-                         hip.ptr(pl.item_v0), hip.ptr(pl.item_v1), hip.ptr(pl.vis), hip.ptr(pl.node_mu),
-                         hip.ptr(pl.node_cp), hip.ptr(pl.records), self.stream())
+                         hip.ptr(pl.item_v0), hip.ptr(pl.item_v1), hip.ptr(pl.vis), hip.ptr(pl.vis_mu),
+                         hip.ptr(pl.vis_cp), hip.ptr(pl.records), self.stream())
                 if e0 is not None:
                     e1 = self._event()
                     self.timed.append(('sh_' + name, e0, e1))
