@@ -3,8 +3,9 @@
 25 solar elevations x 32 x 36 views, Stokes I/Q/U). One step = one table: the source iteration of all wavelengths to
 relative tolerance 1e-5 plus the table query. Inputs (geometry, optics) are resident in HBM before the timed region.
 
-Prints one JSON line (the bench contract) with a roofline entry for the dominant kernel (warm transport) and a CPU
-baseline: the NumPy FP32 oracle timed on a bounded sample of the same work on this host.
+Prints one JSON line (the bench contract) with a roofline entry for the dominant kernel (the SH contraction of the
+warm pass, sh_records) and a CPU baseline: the NumPy FP32 oracle timed on a bounded sample of the same work on this
+host. --mixed runs the mixed-precision form (FP16 SH contraction on v_mfma_f32_16x16x16f16).
 """
 
 import argparse
@@ -18,13 +19,41 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-PEAK_FP32 = 157.3e12  # MI355X FP32 (vector and matrix) FLOP/s
+PEAK_FP32 = 157.3e12  # MI355X dense FP32 MFMA FLOP/s
+PEAK_FP16 = 2.5e15  # MI355X dense FP16 MFMA FLOP/s
+PEAK_HBM = 8.0e12  # MI355X HBM3E bytes/s
 F_SAMPLE = 550  # FP32 flops per (quadrature node, wavelength) of the warm transport, counted from the kernel source
-F_VISIT_PER_K = 2  # flops per (SH corner refill, wavelength, coefficient): one FMA
+COPY_BYTES = 1 << 30  # the calibration copy reads and writes this many bytes
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel, config, mixed):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 counter summary of this workload
+    (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a copy of known size); None when absent."""
+    name = f'{kernel}_{config}_{"fp16" if mixed else "fp32"}_traffic.txt'
+    path = ROOT / 'profiles' / name
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith('bytes_per_launch:'):
+            return float(line.split(':', 1)[1])
+    return None
+
+
+def calibration_copy():
+    """A plain device copy of COPY_BYTES (read and write) that counter runs use to scale FETCH_SIZE / WRITE_SIZE."""
+    import torch
+
+    n = COPY_BYTES // 4
+    x = torch.ones(n, dtype=torch.float32, device='cuda')
+    y = torch.empty_like(x)
+    torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    log(f'calibration copy: {COPY_BYTES} bytes read and written')
+    del x, y
 
 
 def cpu_baseline(s, scope, evaluations, seconds_budget=20.0):
@@ -95,6 +124,7 @@ def main():
     k.plan(solver.geometry.ground_bundle, k.constants(s, solver.geometry.ground_bundle.nq))
     torch.cuda.synchronize()
     prep = time.perf_counter() - t0
+    calibration_copy()
     log(f'prepared in {prep:.1f} s: {s.geo.ns} cells, {solver.geometry.bundle.rays} rays, '
         f'{solver.geometry.bundle.samples} nodes per evaluation, W={s.W}')
     tol = float(cfg['anderson_tol'])
@@ -128,23 +158,20 @@ def main():
         kinds.setdefault(name, []).append(a.elapsed_time(b) * 1e-3)
     log('launch seconds per step: ' + ', '.join(f'{n} {sum(v) / args.steps:.3f} ({len(v) // args.steps})'
                                                   for n, v in kinds.items()))
-    # algorithmic work of one warm launch (fixed by the geometry): nodes and SH refills from the device counters
-    k.counters = torch.zeros(2, dtype=torch.int64, device=k.device)
-    x = solver.zeros()
-    dc = k.constants(s, solver.geometry.bundle.nq)
-    plane = k.plane(s, x, dc)
-    bnd = torch.zeros((s.geo.nc, dc.G), dtype=torch.float32, device=k.device)
-    k.radiance(solver.geometry.bundle, s, x, False, bnd, 4, plane)
-    torch.cuda.synchronize()
-    nodes, refills = (int(v) for v in k.counters.cpu())
-    k.counters = None
-    flops = (nodes * F_SAMPLE + refills * F_VISIT_PER_K * k.K) * s.W
-    t_warm = sum(warm) / max(1, len(warm))
-    achieved = flops / t_warm / 1e12
-    roofline = dict(bound='mfma', achieved=round(achieved, 3), peak=PEAK_FP32 / 1e12, unit='TFLOP/s',
-                    frac=round(achieved / (PEAK_FP32 / 1e12), 4), traffic=None)
-    log(f'warm transport: {t_warm * 1e3:.2f} ms per launch, nodes {nodes}, refills {refills}, '
-        f'{flops / 1e12:.3f} TFLOP -> {achieved:.2f} TFLOP/s')
+    launch_s = {n: round(sum(v) / args.steps, 4) for n, v in kinds.items()}
+    # the dominant kernel is the SH contraction of the warm pass: 2 K W flops per (node, corner) visit
+    sh = [x for x in kinds.get('sh_warm', [])]
+    t_sh = sum(sh) / max(1, len(sh))
+    plan = k.plan(solver.geometry.bundle, k.constants(s, solver.geometry.bundle.nq))
+    visits = int(plan.vis.numel())
+    flops = 2.0 * visits * k.K * s.W
+    peak = PEAK_FP16 if args.mixed else PEAK_FP32
+    achieved = flops / t_sh
+    traffic = measured_traffic('sh_records', args.config, args.mixed)
+    roofline = dict(bound='mfma', achieved=round(achieved / 1e12, 3), peak=peak / 1e12, unit='TFLOP/s',
+                    frac=round(achieved / peak, 4), traffic=traffic)
+    log(f'SH records (warm): {t_sh * 1e3:.2f} ms per launch, {visits} visits x K {k.K} x W {s.W} -> '
+        f'{flops / 1e12:.3f} TFLOP, {achieved / 1e12:.2f} TFLOP/s ({"FP16" if args.mixed else "FP32"} MFMA)')
     out = dict(
         metric='radiance tables/hour/GPU at <=1e-4 rel err vs FP64; learned+batched vs plain',
         value=round(3600.0 / per_step, 2),
@@ -156,14 +183,25 @@ def main():
         higher_is_better=True,
         scaling='weak',
         vs_baseline=None,
-        dtype='fp16-sh' if args.mixed else 'f32',
+        dtype='f32+fp16-mfma' if args.mixed else 'f32',
         data='synthetic',
-        config=dict(workload=f'{args.config}: plain FP32 HIP, {s.geo.nh} levels x {s.geo.nc} chi, '
+        config=dict(workload=f'{args.config}: {"mixed FP16 SH" if args.mixed else "plain FP32"} HIP, {s.geo.nh} levels x {s.geo.nc} chi, '
                              f'{s.W} wavelengths, L={k.L}, nmu={s.geo.nmu} nphi={s.geo.nphi} path_n={s.geo.path_n}, '
                              f'{len(hs)} solar elevations x {len(el)} x {len(az)} views, I/Q/U, tol {tol:g}',
                     evaluations=evals, rays=solver.geometry.bundle.rays, prepare_seconds=round(prep, 1)),
         roofline=roofline,
+        launch_seconds_per_step=launch_s,
     )
+    if args.mixed:
+        # the same table from the plain FP32 contraction (same kernel, plans and state): per-entry error of I, Q, U
+        k.mixed = False
+        info32, table32 = step()
+        k.mixed = True
+        I32 = np.abs(table32[..., 0])
+        d = np.abs(table - table32) / I32[..., None]
+        out['error_vs_fp32'] = dict(max_I=float(d[..., 0].max()), rms_I=float(np.sqrt(np.mean(d[..., 0] ** 2))),
+                                    max_QU=float(d[..., 1:].max()), evaluations_fp32=info32['evaluations'])
+        log(f'mixed vs FP32 table: {out["error_vs_fp32"]}')
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(s, scope, int(np.mean(evals)))
     print(json.dumps(out), flush=True)

@@ -50,7 +50,6 @@ extern "C" {
 /* Transport flags */
 #define SOS_FIRST 1     /* first light (single scattering of the direct and ground-reflected sun) */
 #define SOS_BOUNDARY 2  /* add the attenuated ground boundary term at ray ends that hit the ground */
-#define SOS_MIXED 4     /* SH plane stored as FP16 (mixed precision) */
 #define SOS_RECORDS 8   /* read per-(node, corner) SH values from records_dev instead of reconstructing them */
 #define SOS_UNIFORM_PHASE 16 /* first light: every lane of a ray has the same aerosol and sulfate asymmetry */
 
@@ -127,10 +126,14 @@ int sos_ground_records(const sos_grid *grid, int64_t items, const int32_t *item_
 /* SH values of every visit as an MFMA contraction per source cell: records[id][w] = sum_k Z[id][k] plane[cell][k][w]
  * with Z the basis row of the visit (the replaced per-corner sum of numpy_reference.py:579-600). Work item i covers
  * the visits vis_dev[v0[i] .. v1[i]) of cell item_cell[i]; vis_dev holds visit ids node * 4 + corner, and
- * vis_mu_dev / vis_cp_dev the ray's mu and the node's cos(phi) in the same (sorted) order. */
-int sos_sh_records(const sos_grid *grid, const void *plane_dev, int32_t half, int64_t items, const int32_t *item_cell_dev,
-                   const int64_t *item_v0_dev, const int64_t *item_v1_dev, const int32_t *vis_dev,
-                   const float *vis_mu_dev, const float *vis_cp_dev, float *records_dev, void *stream);
+ * vis_mu_dev / vis_cp_dev the ray's mu and the node's cos(phi) in the same (sorted) order. half = 1: the plane is
+ * the FP16 plane of sos_make_plane with its per-(cell, lane) scale_dev, Z is rounded to FP16 and the products run
+ * on v_mfma_f32_16x16x16f16 with FP32 accumulation (mixed precision, lanes 16 or 32; one lane runs FP32 Z against
+ * the FP16 plane). */
+int sos_sh_records(const sos_grid *grid, const void *plane_dev, const float *scale_dev, int32_t half, int64_t items,
+                   const int32_t *item_cell_dev, const int64_t *item_v0_dev, const int64_t *item_v1_dev,
+                   const int32_t *vis_dev, const float *vis_mu_dev, const float *vis_cp_dev, float *records_dev,
+                   void *stream);
 
 /* Moments [groups][4 + K][lanes] and ground flux [groups][lanes] of the per-ray radiances rad_dev
  * [rays][lanes][4] (the bundle's groups are the destination cells). ground_only writes the flux only. */
@@ -142,9 +145,11 @@ int sos_project(const sos_grid *grid, const sos_bundle *bundle, const float *rad
 int sos_compose(const sos_grid *grid, int32_t rows, const float *moments_dev, const float *first_dev,
                 const float *factor_dev, float *source_dev, void *stream);
 
-/* plane[cells][K][lanes] = source[:, 10:10+K] * ha[ih] + source[:, 10+K:10+2K] * sd[ih] (FP32 or FP16). */
+/* plane[cells][K][lanes] = source[:, 10:10+K] * ha[ih] + source[:, 10+K:10+2K] * sd[ih], K in m-major order.
+ * half = 0: FP32, scale_dev unused. half = 1: FP16 plane / scale with scale_dev [cells][lanes] float the power of two
+ * just above the column's largest magnitude (1 for an all-zero column). */
 int sos_make_plane(const sos_grid *grid, int32_t rows, const float *source_dev, const float *ha_dev,
-                   const float *sd_dev, int32_t half, void *plane_dev, void *stream);
+                   const float *sd_dev, int32_t half, void *plane_dev, float *scale_dev, void *stream);
 
 /* Per lane: count of non-finite g, max |g|, max |g - x|; frozen lanes read x for g. out_dev [3][lanes] float
  * (count as float). */

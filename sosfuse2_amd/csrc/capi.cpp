@@ -38,7 +38,7 @@ int sh_records_dispatch(const ShArgs &a, bool half, hipStream_t s);
 int compose_launch(int64_t cells, int rows, int K, int G, const float *mom, const float *first, const float *factor,
                    float *out, hipStream_t s);
 int plane_launch(int nh, int nc, int L, int rows, int G, int strato, const float *src, const float *ha,
-                 const float *sd, int half, void *plane, hipStream_t s);
+                 const float *sd, int half, void *plane, float *scale, hipStream_t s);
 int metrics_launch(int64_t rows, int G, const float *g, const float *x, unsigned frozen, float *out, hipStream_t s);
 int residual_launch(int64_t cells, int rows, int G, const float *g, const float *x, const float *obs, int nobs,
                     float *work, float *f, hipStream_t s);
@@ -118,17 +118,19 @@ int sos_ground_records(const sos_grid *g, int64_t items, const int32_t *item_cel
     return ground_records_dispatch(a, (hipStream_t)stream);
 }
 
-int sos_sh_records(const sos_grid *g, const void *plane_dev, int32_t half, int64_t items, const int32_t *item_cell_dev,
-                   const int64_t *item_v0_dev, const int64_t *item_v1_dev, const int32_t *vis_dev,
-                   const float *vis_mu_dev, const float *vis_cp_dev, float *records_dev, void *stream) {
+int sos_sh_records(const sos_grid *g, const void *plane_dev, const float *scale_dev, int32_t half, int64_t items,
+                   const int32_t *item_cell_dev, const int64_t *item_v0_dev, const int64_t *item_v1_dev,
+                   const int32_t *vis_dev, const float *vis_mu_dev, const float *vis_cp_dev, float *records_dev,
+                   void *stream) {
     if (!g || !plane_dev || !item_cell_dev || !item_v0_dev || !item_v1_dev || !vis_dev || !vis_mu_dev ||
         !vis_cp_dev || !records_dev)
         return arg_error("sos_sh_records: null argument");
+    if (half && !scale_dev) return arg_error("sos_sh_records: FP16 plane without its scale");
     if (!lanes_ok(g->lanes) || !order_ok(g->L)) return arg_error("sos_sh_records: lanes %d / L %d", g->lanes, g->L);
     ShArgs a;
     a.L = g->L; a.G = g->lanes; a.K = (g->L + 1) * (g->L + 2) / 2; a.plane = plane_dev; a.items = items;
     a.item_cell = item_cell_dev; a.item_v0 = item_v0_dev; a.item_v1 = item_v1_dev; a.vis = vis_dev;
-    a.vis_mu = vis_mu_dev; a.vis_cp = vis_cp_dev; a.records = records_dev;
+    a.vis_mu = vis_mu_dev; a.vis_cp = vis_cp_dev; a.records = records_dev; a.scale = scale_dev;
     return sh_records_dispatch(a, half != 0, (hipStream_t)stream);
 }
 
@@ -198,14 +200,15 @@ int sos_compose(const sos_grid *g, int32_t rows, const float *moments_dev, const
 }
 
 int sos_make_plane(const sos_grid *g, int32_t rows, const float *source_dev, const float *ha_dev, const float *sd_dev,
-                   int32_t half, void *plane_dev, void *stream) {
+                   int32_t half, void *plane_dev, float *scale_dev, void *stream) {
     if (!g || !source_dev || !ha_dev || !plane_dev) return arg_error("sos_make_plane: null argument");
+    if (half && !scale_dev) return arg_error("sos_make_plane: FP16 plane without its scale");
     const int K = (g->L + 1) * (g->L + 2) / 2;
     const int strato = rows == 10 + 2 * K;
     if (rows != 10 + K && !strato) return arg_error("sos_make_plane: rows %d for K %d", rows, K);
     if (strato && !sd_dev) return arg_error("sos_make_plane: sulfate plane without its density");
     return plane_launch(g->nh, g->nc, g->L, rows, g->lanes, strato, source_dev, ha_dev, sd_dev, half, plane_dev,
-                        (hipStream_t)stream);
+                        scale_dev, (hipStream_t)stream);
 }
 
 int sos_source_metrics(int64_t rows, int32_t lanes, const float *g_dev, const float *x_dev, uint32_t frozen,

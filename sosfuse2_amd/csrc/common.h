@@ -110,6 +110,7 @@ struct ShArgs {
     const int32_t *vis;        // [visits] visit ids (node * 4 + corner), sorted by cell
     const float *vis_mu, *vis_cp;  // [visits] the ray's mu and the node's cos(phi), in sorted visit order
     float *records;            // [nodes * 4][G]
+    const float *scale;        // [cells][G] scale of an FP16 plane (mixed precision)
 };
 
 struct GroundArgs {

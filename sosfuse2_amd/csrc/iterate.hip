@@ -46,28 +46,72 @@ __global__ void compose_kernel(int64_t cells, int rows, int K, int G, const floa
 }
 
 // ------------------------------------------------------------------------------------------------- make_plane
-template <bool HALF>
+// Combined plane c[cell][j][w] = src[10 + k] ha(h) (+ src[10 + K + k] sd(h)), j the m-major position of k = (l, m).
+__device__ inline int plane_k(int L, int j) {
+    int m = 0, off = 0;
+    while (off + (L + 1 - m) <= j) { off += L + 1 - m; ++m; }
+    const int l = m + (j - off);
+    return l * (l + 1) / 2 + m;
+}
+
 __global__ void plane_kernel(int nh, int nc, int L, int rows, int G, int strato, const float *__restrict__ src,
-                             const float *__restrict__ ha, const float *__restrict__ sd, void *__restrict__ plane) {
+                             const float *__restrict__ ha, const float *__restrict__ sd, float *__restrict__ plane) {
     const int K = (L + 1) * (L + 2) / 2;
     const int64_t total = (int64_t)nh * nc * K * G;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         const int w = (int)(e % G);
         const int64_t ck = e / G;
-        const int j = (int)(ck % K);  // m-major position
+        const int k = plane_k(L, (int)(ck % K));
         const int64_t cell = ck / K;
-        // invert j -> (m, l): order m holds L + 1 - m degrees
-        int m = 0, off = 0;
-        while (off + (L + 1 - m) <= j) { off += L + 1 - m; ++m; }
-        const int l = m + (j - off);
-        const int k = l * (l + 1) / 2 + m;
         const int ih = (int)(cell / nc);
         const float *s = src + cell * rows * G + w;
         float v = s[(10 + k) * G] * ha[ih];
         if (strato) v = v + s[(10 + K + k) * G] * sd[ih];
-        if constexpr (HALF) reinterpret_cast<__half *>(plane)[e] = __float2half(v);
-        else reinterpret_cast<float *>(plane)[e] = v;
+        plane[e] = v;
     }
+}
+
+// FP16 plane for the mixed-precision contraction: one workgroup per cell. scale[cell][w] is the power of two just
+// above max_j |c[cell][j][w]| (1 for an all-zero column), and the plane holds c / scale in [-1, 1] rounded to
+// nearest FP16, so every coefficient keeps an absolute error of at most 2^-12 scale and no column overflows.
+__global__ __launch_bounds__(256) void plane_half_kernel(int nc, int L, int rows, int G, int strato,
+                                                         const float *__restrict__ src, const float *__restrict__ ha,
+                                                         const float *__restrict__ sd, _Float16 *__restrict__ plane,
+                                                         float *__restrict__ scale) {
+    constexpr int KMAX = 33 * 34 / 2;
+    __shared__ short kmap[KMAX];
+    __shared__ float colmax[256];
+    __shared__ float inv[32];
+    const int K = (L + 1) * (L + 2) / 2;
+    const int64_t cell = blockIdx.x;
+    const int ih = (int)(cell / nc);
+    const int tid = threadIdx.x, w = tid % G, q = tid / G, nq = 256 / G;
+    for (int j = tid; j < K; j += 256) kmap[j] = (short)plane_k(L, j);
+    __syncthreads();
+    const float *s = src + cell * rows * G + w;
+    const float h = ha[ih], d = strato ? sd[ih] : 0.0f;
+    auto value = [&](int j) {
+        const int k = kmap[j];
+        float v = s[(10 + k) * G] * h;
+        if (strato) v = v + s[(10 + K + k) * G] * d;
+        return v;
+    };
+    float mx = 0.0f;
+    for (int j = q; j < K; j += nq) mx = fmaxf(mx, fabsf(value(j)));
+    colmax[tid] = mx;
+    __syncthreads();
+    if (tid < G) {
+        for (int r = 1; r < nq; ++r) mx = fmaxf(mx, colmax[r * G + tid]);
+        int ex = 0;
+        frexpf(mx, &ex);  // mx = f 2^ex with f in [0.5, 1)
+        const float sc = mx > 0.0f ? ldexpf(1.0f, ex) : 1.0f;
+        scale[cell * G + tid] = sc;
+        inv[tid] = 1.0f / sc;  // exact: a power of two
+    }
+    __syncthreads();
+    const float iv = inv[w];
+    _Float16 *out = plane + cell * K * G + w;
+    for (int j = q; j < K; j += nq) out[(int64_t)j * G] = (_Float16)(value(j) * iv);
 }
 
 // ---------------------------------------------------------------------------------------------------- metrics
@@ -262,13 +306,16 @@ int compose_launch(int64_t cells, int rows, int K, int G, const float *mom, cons
 }
 
 int plane_launch(int nh, int nc, int L, int rows, int G, int strato, const float *src, const float *ha,
-                 const float *sd, int half, void *plane, hipStream_t s) {
+                 const float *sd, int half, void *plane, float *scale, hipStream_t s) {
     const int K = (L + 1) * (L + 2) / 2;
-    const unsigned b = blocks_for((int64_t)nh * nc * K * G);
-    if (half)
-        hipLaunchKernelGGL(plane_kernel<true>, dim3(b), dim3(256), 0, s, nh, nc, L, rows, G, strato, src, ha, sd, plane);
-    else
-        hipLaunchKernelGGL(plane_kernel<false>, dim3(b), dim3(256), 0, s, nh, nc, L, rows, G, strato, src, ha, sd, plane);
+    if (half) {
+        if (nh * nc > 0)
+            hipLaunchKernelGGL(plane_half_kernel, dim3((unsigned)(nh * nc)), dim3(256), 0, s, nc, L, rows, G, strato,
+                               src, ha, sd, reinterpret_cast<_Float16 *>(plane), scale);
+    } else {
+        hipLaunchKernelGGL(plane_kernel, dim3(blocks_for((int64_t)nh * nc * K * G)), dim3(256), 0, s, nh, nc, L, rows,
+                           G, strato, src, ha, sd, reinterpret_cast<float *>(plane));
+    }
     return check("make_plane");
 }
 

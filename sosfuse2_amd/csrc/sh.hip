@@ -59,8 +59,8 @@ __device__ inline void visit_seeds(int v, double mu, float cp, const double *dd,
 // m = p and m = L - p side by side (their lengths add up to L + 2, so tasks are balanced and each thread has two
 // independent dependency chains); each basis value is rounded to FP32 before the FP32 product with cos(m phi)
 // (the reference's FP64 basis, rounded).
-template <int L, int V, int KZ>
-__device__ inline void build_z(float (*Z)[KZ], const double *mus, const double (*seed)[V], const float (*cmv)[V],
+template <int L, int V, int KZ, typename T = float>
+__device__ inline void build_z(T (*Z)[KZ], const double *mus, const double (*seed)[V], const float (*cmv)[V],
                                const double *aa, const double *bb, int nv, int nt) {
     constexpr int PAIRS = L / 2 + 1;
     for (int task = threadIdx.x; task < V * PAIRS; task += nt) {
@@ -69,9 +69,9 @@ __device__ inline void build_z(float (*Z)[KZ], const double *mus, const double (
         const bool two = mb != ma;
         const int oa = moff<L>(ma), ob = moff<L>(mb);
         if (v >= nv) {
-            for (int l = ma; l <= L; ++l) Z[v][oa + l - ma] = 0.0f;
+            for (int l = ma; l <= L; ++l) Z[v][oa + l - ma] = (T)0.0f;
             if (two)
-                for (int l = mb; l <= L; ++l) Z[v][ob + l - mb] = 0.0f;
+                for (int l = mb; l <= L; ++l) Z[v][ob + l - mb] = (T)0.0f;
             continue;
         }
         const double mu = mus[v];
@@ -79,14 +79,14 @@ __device__ inline void build_z(float (*Z)[KZ], const double *mus, const double (
         const double xa = ma ? 1.41421356237309515 : 1.0, xb = 1.41421356237309515;
         // chain a: m = p (long); chain b: m = L - p (short, length p + 1)
         double a0 = seed[ma][v], b0 = seed[mb][v];
-        Z[v][oa] = (float)(xa * a0) * ca;
-        if (two) Z[v][ob] = (float)(xb * b0) * cb;
+        Z[v][oa] = (T)((float)(xa * a0) * ca);
+        if (two) Z[v][ob] = (T)((float)(xb * b0) * cb);
         double a1 = (sqrt(2.0 * ma + 3.0) * mu) * a0;  // ma < L always
-        Z[v][oa + 1] = (float)(xa * a1) * ca;
+        Z[v][oa + 1] = (T)((float)(xa * a1) * ca);
         double b1 = 0.0;
         if (two && mb < L) {
             b1 = (sqrt(2.0 * mb + 3.0) * mu) * b0;
-            Z[v][ob + 1] = (float)(xb * b1) * cb;
+            Z[v][ob + 1] = (T)((float)(xb * b1) * cb);
         }
         // both chains advance together while chain b lasts
         int la = ma + 2, lb = mb + 2;
@@ -95,8 +95,8 @@ __device__ inline void build_z(float (*Z)[KZ], const double *mus, const double (
             const int ka = kidx(la, ma), kb = kidx(lb, mb);
             const double a2 = (aa[ka] * mu) * a1 - bb[ka] * a0;
             const double b2 = (aa[kb] * mu) * b1 - bb[kb] * b0;
-            Z[v][oa + la - ma] = (float)(xa * a2) * ca;
-            Z[v][ob + lb - mb] = (float)(xb * b2) * cb;
+            Z[v][oa + la - ma] = (T)((float)(xa * a2) * ca);
+            Z[v][ob + lb - mb] = (T)((float)(xb * b2) * cb);
             a0 = a1; a1 = a2;
             b0 = b1; b1 = b2;
         }
@@ -104,13 +104,13 @@ __device__ inline void build_z(float (*Z)[KZ], const double *mus, const double (
         for (; la <= L; ++la) {
             const int ka = kidx(la, ma);
             const double a2 = (aa[ka] * mu) * a1 - bb[ka] * a0;
-            Z[v][oa + la - ma] = (float)(xa * a2) * ca;
+            Z[v][oa + la - ma] = (T)((float)(xa * a2) * ca);
             a0 = a1; a1 = a2;
         }
     }
 }
 
-template <int G, int L, bool HALF>
+template <int G, int L>
 __global__ __launch_bounds__(G * 4) __attribute__((amdgpu_waves_per_eu(2, 2))) void sh_records_kernel(ShArgs a) {
     constexpr int K = (L + 1) * (L + 2) / 2;
     constexpr int KZ = (K + 15) / 16 * 16;   // padded row
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(G * 4) __attribute__((amdgpu_waves_per_eu(2, 2))) v
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             const int k = 16 * (s >> 2) + 4 * (lane >> 4) + (s & 3);
-            b[s] = k < K ? plane_at<L, HALF>(a.plane, base + (int64_t)k * G) : 0.0f;
+            b[s] = k < K ? reinterpret_cast<const float *>(a.plane)[base + (int64_t)k * G] : 0.0f;
         }
     }
     // visit data of the next round is loaded one round ahead (contiguous: the plan stores mu and cos(phi) in sorted
@@ -194,7 +194,94 @@ __global__ __launch_bounds__(G * 4) __attribute__((amdgpu_waves_per_eu(2, 2))) v
     }
 }
 
-// W = 1 lanes: one thread per visit, dot product against the cell's coefficients staged in LDS.
+// Mixed precision: the plane is FP16 scaled per (cell, lane) (plane = c / scale[cell][lane]), Z rows are FP16, and
+// v_mfma_f32_16x16x16f16 accumulates in FP32; the epilogue multiplies by the scale. 32 visits per round
+// (two M-tiles); wavefront w takes M-tile w / (G/16) and N-tile w % (G/16).
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+
+template <int G, int L>
+__global__ __launch_bounds__(G * 8) __attribute__((amdgpu_waves_per_eu(2, 2))) void sh_records_half_kernel(ShArgs a) {
+    constexpr int K = (L + 1) * (L + 2) / 2;
+    constexpr int KZ = (K + 15) / 16 * 16;
+    constexpr int KS = KZ / 16;               // MFMA k-steps of 16
+    constexpr int NT = G * 8;
+    constexpr int V = 32;
+    constexpr int ZS = KZ + 8;                // halves: row stride = 4 (mod 32 banks) in 32-bit words
+    __shared__ _Float16 Z[V][ZS];
+    __shared__ double aa[K], bb[K];
+    __shared__ double seed[L + 1][V];
+    __shared__ float cmv[L + 1][V];
+    __shared__ double smu[V];
+    __shared__ double dd[L + 1];
+    __shared__ int sid[V];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int mt = wave / (G / 16), nt = wave % (G / 16);
+    const int64_t item = blockIdx.x;
+    const int cell = a.item_cell[item];
+    const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
+    stage_constants<L>(aa, bb, NT);
+    for (int m = tid; m <= L; m += NT) dd[m] = m ? -sqrt((2.0 * m + 1) / (2.0 * m)) : 0.0;
+    for (int e = tid; e < V * (KZ - K); e += NT) Z[e / (KZ - K)][K + e % (KZ - K)] = (_Float16)0.0f;
+    const int col = nt * 16 + (lane & 15);
+    const float scale = a.scale[(int64_t)cell * G + col];
+    // B operand: b[s] holds plane[cell][16 s + 4 (lane / 16) + 0..3][col]
+    half4_t b[KS];
+    {
+        const _Float16 *pl = reinterpret_cast<const _Float16 *>(a.plane) + (int64_t)cell * K * G + col;
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = 16 * s + 4 * (lane >> 4) + u;
+                b[s][u] = k < K ? pl[(int64_t)k * G] : (_Float16)0.0f;
+            }
+    }
+    int nid = -1;
+    float nmu = 0.0f, ncp = 1.0f;
+    if (tid < V && v0 + tid < v1) {
+        nid = a.vis[v0 + tid];
+        nmu = a.vis_mu[v0 + tid];
+        ncp = a.vis_cp[v0 + tid];
+    }
+    for (int64_t r0 = v0; r0 < v1; r0 += V) {
+        const int nv = (int)min((int64_t)V, v1 - r0);
+        if (tid < V) {
+            const int id = nid;
+            const double mu = (double)nmu;
+            const float cp = ncp;
+            const int64_t nx = r0 + V + tid;
+            nid = -1;
+            if (nx < v1) {
+                nid = a.vis[nx];
+                nmu = a.vis_mu[nx];
+                ncp = a.vis_cp[nx];
+            }
+            sid[tid] = tid < nv ? id : -1;
+            smu[tid] = tid < nv ? mu : 0.0;
+            visit_seeds<L, V>(tid, tid < nv ? mu : 0.0, tid < nv ? cp : 1.0f, dd, seed, cmv);
+        }
+        __syncthreads();
+        build_z<L, V, ZS, _Float16>(Z, smu, seed, cmv, aa, bb, nv, NT);
+        __syncthreads();
+        floatx4 acc[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
+        const _Float16 *zrow = &Z[mt * 16 + (lane & 15)][4 * (lane >> 4)];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const half4_t av = *reinterpret_cast<const half4_t *>(zrow + 16 * s);
+            acc[s & 1] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, b[s], acc[s & 1], 0, 0, 0);
+        }
+        const floatx4 d = acc[0] + acc[1];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = mt * 16 + 4 * (lane >> 4) + v;
+            if (row < nv) a.records[(int64_t)sid[row] * G + col] = d[v] * scale;
+        }
+        __syncthreads();
+    }
+}
+
+// W = 1 lanes: one thread per visit, dot product against the cell's coefficients staged in LDS (an FP16 plane is
+// scaled back to FP32 there; the basis stays FP32).
 template <int L, bool HALF>
 __global__ __launch_bounds__(256) void sh_records_scalar(ShArgs a) {
     constexpr int K = (L + 1) * (L + 2) / 2;
@@ -202,7 +289,8 @@ __global__ __launch_bounds__(256) void sh_records_scalar(ShArgs a) {
     __shared__ double aa[K], bb[K];
     const int64_t item = blockIdx.x;
     const int cell = a.item_cell[item];
-    for (int k = threadIdx.x; k < K; k += 256) c[k] = plane_at<L, HALF>(a.plane, (int64_t)cell * K + k);
+    const float sc = HALF ? a.scale[cell] : 1.0f;
+    for (int k = threadIdx.x; k < K; k += 256) c[k] = plane_at<L, HALF>(a.plane, (int64_t)cell * K + k) * sc;
     stage_constants<L>(aa, bb, 256);
     __syncthreads();
     for (int64_t r = a.item_v0[item] + threadIdx.x; r < a.item_v1[item]; r += 256) {
@@ -247,8 +335,8 @@ static int sh_launch(const ShArgs &a, bool half, hipStream_t s) {
         if (half) hipLaunchKernelGGL((sh_records_scalar<L, true>), dim3((unsigned)a.items), dim3(256), 0, s, a);
         else hipLaunchKernelGGL((sh_records_scalar<L, false>), dim3((unsigned)a.items), dim3(256), 0, s, a);
     } else {
-        if (half) hipLaunchKernelGGL((sh_records_kernel<G, L, true>), dim3((unsigned)a.items), dim3(G * 4), 0, s, a);
-        else hipLaunchKernelGGL((sh_records_kernel<G, L, false>), dim3((unsigned)a.items), dim3(G * 4), 0, s, a);
+        if (half) hipLaunchKernelGGL((sh_records_half_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 8), 0, s, a);
+        else hipLaunchKernelGGL((sh_records_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 4), 0, s, a);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error("sh_records launch: %s", hipGetErrorString(e));

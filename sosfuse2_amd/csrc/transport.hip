@@ -16,18 +16,10 @@
 
 namespace sos {
 
-
-
-template <bool HALF>
-__device__ inline float ld_plane(const void *p, int64_t i) {
-    if constexpr (HALF) return __half2float(reinterpret_cast<const __half *>(p)[i]);
-    else return reinterpret_cast<const float *>(p)[i];
-}
-
 // Azimuthal amplitudes of one corner cell for a ray with direction cosine mu = d_z (global frame):
 // out[m] = sqrt2^(m>0) * sum_{l>=m} P_lm(mu) c[cell][(l,m)], the recurrence of arithmetic.py:86-109 in FP32.
-template <int L, int G, bool HALF>
-__device__ inline void refill(const void *plane, int cell, int w, float mu, float st, float (&out)[L + 1]) {
+template <int L, int G>
+__device__ inline void refill(const float *plane, int cell, int w, float mu, float st, float (&out)[L + 1]) {
     constexpr int K = (L + 1) * (L + 2) / 2;
     const int64_t base = (int64_t)cell * K * G + w;
     float diag = 0.28209479177387814f;
@@ -36,16 +28,16 @@ __device__ inline void refill(const void *plane, int cell, int w, float mu, floa
         if (m) diag = (diag * ShConst<L>::dd()[m]) * st;
         float z0 = diag;
         const int64_t o = base + (int64_t)moff<L>(m) * G;
-        float acc = z0 * ld_plane<HALF>(plane, o);
+        float acc = z0 * plane[o];
         if (m < L) {
             float z1 = (ShConst<L>::sq()[m] * mu) * z0;
-            acc = acc + z1 * ld_plane<HALF>(plane, o + G);
+            acc = acc + z1 * plane[o + G];
             const float *aa = ShConst<L>::aa(), *bb = ShConst<L>::bb();
 #pragma unroll 4
             for (int l = m + 2; l <= L; ++l) {
                 const int k = kidx(l, m);
                 float z2 = (aa[k] * mu) * z1 - bb[k] * z0;
-                acc = acc + z2 * ld_plane<HALF>(plane, o + (int64_t)(l - m) * G);
+                acc = acc + z2 * plane[o + (int64_t)(l - m) * G];
                 z0 = z1;
                 z1 = z2;
             }
@@ -183,11 +175,10 @@ __global__ void plan_nodes_kernel(TransportArgs a, const int64_t *node_off, int3
     }
 }
 
-enum Aero { DIRECT32 = 0, DIRECT16 = 1, RECORDS = 2 };
+enum Aero { DIRECT = 0, RECORDS = 1 };
 
 template <int G, int L, int P, bool FIRST, bool BOUNDARY, int AERO, bool UNI = false>
 __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
-    constexpr bool HALF = AERO == DIRECT16;
     constexpr int RPW = 64 / G;
     const int lane = threadIdx.x & 63;
     const int w = lane % G;
@@ -465,7 +456,7 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
                             if (tag != cell) {
                                 ++refills;
                                 float fresh[L + 1];
-                                refill<L, G, HALF>(a.plane, cell, w, mu, st, fresh);
+                                refill<L, G>(reinterpret_cast<const float *>(a.plane), cell, w, mu, st, fresh);
     #pragma unroll
                                 for (int s = 0; s < 4; ++s)
                                     if (s == c) {
@@ -595,7 +586,7 @@ static int launch_one(const TransportArgs &a, hipStream_t s) {
 // First light and the records consumer do not depend on L; the direct (register cache) path does.
 template <int G, int P>
 static int launch_flags(const TransportArgs &a, int L, hipStream_t s) {
-    const bool first = a.flags & SOS_FIRST, bnd = a.flags & SOS_BOUNDARY, half = a.flags & SOS_MIXED;
+    const bool first = a.flags & SOS_FIRST, bnd = a.flags & SOS_BOUNDARY;
     const bool records = a.flags & SOS_RECORDS;
     if (first)
         return (a.flags & SOS_UNIFORM_PHASE) ? launch_one<G, 8, P, true, false, RECORDS, true>(a, s)
@@ -603,10 +594,7 @@ static int launch_flags(const TransportArgs &a, int L, hipStream_t s) {
     if (records)
         return bnd ? launch_one<G, 8, P, false, true, RECORDS>(a, s) : launch_one<G, 8, P, false, false, RECORDS>(a, s);
 #define SOS_DIRECT(LL)                                                                                   \
-    return bnd ? (half ? launch_one<G, LL, P, false, true, DIRECT16>(a, s)                               \
-                       : launch_one<G, LL, P, false, true, DIRECT32>(a, s))                              \
-               : (half ? launch_one<G, LL, P, false, false, DIRECT16>(a, s)                              \
-                       : launch_one<G, LL, P, false, false, DIRECT32>(a, s));
+    return bnd ? launch_one<G, LL, P, false, true, DIRECT>(a, s) : launch_one<G, LL, P, false, false, DIRECT>(a, s);
     switch (L) {
         case 8: SOS_DIRECT(8)
         case 16: SOS_DIRECT(16)

@@ -9,7 +9,7 @@ c_i32, c_i64, c_u32, c_f32 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32, ct
 P = ctypes.c_void_p
 
 LANE_CONSTANTS = 12
-FIRST, BOUNDARY, MIXED, RECORDS, UNIFORM_PHASE, GROUND_RECORDS = 1, 2, 4, 8, 16, 32
+FIRST, BOUNDARY, RECORDS, UNIFORM_PHASE, GROUND_RECORDS = 1, 2, 8, 16, 32
 
 
 class Bundle(ctypes.Structure):
@@ -37,10 +37,10 @@ _SIGNATURES = {
     'sos_plan_count': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, P],
     'sos_plan_nodes': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, P, P, P, P, P],
     'sos_ground_records': [ctypes.POINTER(Grid), c_i64, P, P, P, P, P, P, P],
-    'sos_sh_records': [ctypes.POINTER(Grid), P, c_i32, c_i64, P, P, P, P, P, P, P, P],
+    'sos_sh_records': [ctypes.POINTER(Grid), P, P, c_i32, c_i64, P, P, P, P, P, P, P, P],
     'sos_project': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, c_i32, P, P, P],
     'sos_compose': [ctypes.POINTER(Grid), c_i32, P, P, P, P, P],
-    'sos_make_plane': [ctypes.POINTER(Grid), c_i32, P, P, P, c_i32, P, P],
+    'sos_make_plane': [ctypes.POINTER(Grid), c_i32, P, P, P, c_i32, P, P, P],
     'sos_source_metrics': [c_i64, c_i32, P, P, c_u32, P, P],
     'sos_anderson_residual': [c_i64, c_i32, c_i32, P, P, P, c_i32, P, P, P],
     'sos_anderson_gram': [c_i64, c_i32, c_i32, c_i32, c_u32, P, P, P, P],

@@ -167,6 +167,8 @@ class Kernel:
             raise ValueError('E_SH_L: the HIP kernels are built for L = 8, 16 and 32')
         if not torch.cuda.is_available():
             raise RuntimeError('E_HIP_DEVICE: no GPU visible to torch')
+        if mixed and not records:
+            raise ValueError('E_MIXED: mixed precision runs the SH contraction as records (records=True)')
         hip.lib()
         self.L = L
         self.K = layout.count(L)
@@ -204,13 +206,15 @@ class Kernel:
 
     # ------------------------------------------------------------------------------------------------ launches
     def plane(self, ctx, field, dc):
-        """Combined SH plane of a source field (m-major per cell)."""
+        """Combined SH plane of a source field (m-major per cell): (FP32 plane, None), or in mixed precision the FP16
+        plane and its per-(cell, lane) power-of-two scale."""
         K = self.K
         dt = torch.float16 if self.mixed else torch.float32
         out = torch.empty((ctx.geo.ns, K, dc.G), dtype=dt, device=self.device)
+        scale = torch.empty((ctx.geo.ns, dc.G), dtype=torch.float32, device=self.device) if self.mixed else None
         hip.call('sos_make_plane', dc.struct, ctx.rows, hip.ptr(field), hip.ptr(dc.ha), hip.ptr(dc.sd),
-                 int(self.mixed), hip.ptr(out), self.stream())
-        return out
+                 int(self.mixed), hip.ptr(out), hip.ptr(scale), self.stream())
+        return out, scale
 
     def plan(self, db, dc):
         if db.plan is None:
@@ -247,12 +251,11 @@ class Kernel:
         if not first:
             if plane is None:
                 plane = self.plane(ctx, field, dc)
-            if self.mixed:
-                flags |= hip.MIXED
             if self.records:
                 pl = self.plan(db, dc)
                 e0 = self._event()
-                hip.call('sos_sh_records', dc.struct, hip.ptr(plane), int(self.mixed), pl.items, hip.ptr(pl.item_cell),
+                hip.call('sos_sh_records', dc.struct, hip.ptr(plane[0]), hip.ptr(plane[1]), int(self.mixed), pl.items,
+                         hip.ptr(pl.item_cell),
                          hip.ptr(pl.item_v0), hip.ptr(pl.item_v1), hip.ptr(pl.vis), hip.ptr(pl.vis_mu),
                          hip.ptr(pl.vis_cp), hip.ptr(pl.records), self.stream())
                 if e0 is not None:
@@ -262,7 +265,7 @@ class Kernel:
                 flags |= hip.RECORDS
         e0 = self._event()
         hip.call('sos_transport', dc.struct, db.struct, flags, pairs, ctx.rows,
-                 None if first else hip.ptr(field), None if first else hip.ptr(plane), hip.ptr(boundary),
+                 None if first else hip.ptr(field), None if first else hip.ptr(plane[0]), hip.ptr(boundary),
                  hip.ptr(records), hip.ptr(out), hip.ptr(self.counters), self.stream())
         if e0 is not None:
             self.timed.append((name, e0, self._event()))

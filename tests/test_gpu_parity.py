@@ -48,14 +48,14 @@ def physical_field(s, seed):
     return f
 
 
-def setup_case(W, L=16, nr=5, state=None, nmu=1, nphi=2, path_n=4, records=True):
+def setup_case(W, L=16, nr=5, state=None, nmu=1, nphi=2, path_n=4, records=True, mixed=False):
     from oracle.solve import OracleSolver
 
     cfg = dict(nr=nr, chi_grid='tiny', nmu=nmu, nphi=nphi, path_n=path_n, L=L, top_km=100.0, observer_km=0.1,
                query_n=6, anderson_depth=4)
     wl = atmosphere.log_spaced_wavelengths(W, 420, 680) if W > 1 else [550.0]
     st = state or atmosphere.config0_state()
-    s, scope = api.create_configured(None, cfg, wl, 'hip', st, records=records)
+    s, scope = api.create_configured(None, cfg, wl, 'hip', st, records=records, mixed=mixed)
     orc = OracleSolver(s, np.float32, depth=4, packs=scope.packs)
     return s, scope, orc
 
@@ -150,6 +150,70 @@ def test_fp32_table_within_1e4_of_fp64(gpu):
     err = np.max(np.abs(tab[..., 0] - t64[..., 0]) / np.abs(t64[..., 0]))
     print(f'max relative error of I vs FP64: {err:.3e}')
     assert err < 1e-4
+
+
+@pytest.mark.parametrize('W,L', [(3, 16), (20, 32), (1, 16)])
+def test_mixed_records_bound(gpu, W, L):
+    """FP16 SH records against the FP32 records of the same field. The plane stores c / s with s the power of two
+    above the column maximum, so |c - c16| <= 2^-12 s; Z16 carries a relative error <= 2^-11 (FP16 rounding, with
+    subnormal Z flushed adding at most 2^-25 each). Every record stays within
+    2^-11 sum_k |Z_k c_k| + 2^-12 s sum_k |Z_k| + 2^-24 K max|Z| s, which the test bounds from above by
+    (2^-11 + 2^-12 + 2^-20) s sum_k |Z_k| with sum_k |Z_k| <= K max|Z|, evaluated per visit from the FP32 basis."""
+    s, scope, orc = setup_case(W, L)
+    s16, _, _ = setup_case(W, L, mixed=True)
+    k, k16 = s.kernel, s16.kernel
+    solver = scope.solver(s, log=None)
+    db = solver.geometry.bundle
+    dc = k.constants(s, db.nq)
+    field = physical_field(s, 5)
+    fd = torch.from_numpy(np.ascontiguousarray(padded(field, dc.G))).cuda()
+    pl = k.plan(db, dc)
+    k.radiance(db, s, fd, False, None, 4)
+    rec = pl.records.clone()
+    dc16 = k16.constants(s16, db.nq)
+    plane16 = k16.plane(s16, fd, dc16)
+    hip.call('sos_sh_records', dc16.struct, hip.ptr(plane16[0]), hip.ptr(plane16[1]), 1, pl.items,
+             hip.ptr(pl.item_cell), hip.ptr(pl.item_v0), hip.ptr(pl.item_v1), hip.ptr(pl.vis), hip.ptr(pl.vis_mu),
+             hip.ptr(pl.vis_cp), hip.ptr(pl.records), k16.stream())
+    torch.cuda.synchronize()
+    rec16 = pl.records.cpu().numpy()
+    rec32 = rec.cpu().numpy()
+    # per-visit bound: K max|Z| <= K sqrt((2L + 1) / (4 pi)) sqrt(2) (normalised harmonics), s per (cell, lane)
+    zmax = math.sqrt((2 * L + 1) / (4 * math.pi)) * math.sqrt(2.0)
+    scale = plane16[1].cpu().numpy()
+    cell = np.empty(rec32.shape[0], 'i8')
+    ic, v0, v1, vis = (t.cpu().numpy() for t in (pl.item_cell, pl.item_v0, pl.item_v1, pl.vis))
+    for i in range(len(ic)):
+        cell[vis[v0[i]:v1[i]]] = ic[i]
+    used = np.zeros(rec32.shape[0], bool)
+    used[vis] = True
+    bound = (2.0 ** -11 + 2.0 ** -12 + 2.0 ** -20) * s.K * zmax * scale[cell[used]]
+    err = np.abs(rec16[used] - rec32[used])
+    print(f'W={W} L={L}: max FP16 record error / bound {np.max(err / bound):.3e}, '
+          f'max rel to the largest record {np.max(err) / np.abs(rec32[used]).max():.3e}')
+    assert np.all(err <= bound)
+    assert np.max(err) / np.abs(rec32[used]).max() < 2e-3
+
+
+def test_mixed_table_against_fp64(gpu):
+    """Mixed precision (FP16 SH contraction) end to end on a small grid: the converged table against the FP64 twin,
+    with the max and RMS relative error of I stated; the solve runs to the same tolerance as the FP32 method."""
+    from oracle.solve import OracleSolver
+
+    s, scope, orc = setup_case(2, 16, mixed=True)
+    solver = scope.solver(s, log=None)
+    x, info = solver.solve(40, 1e-5)
+    o64 = OracleSolver(s, np.float64, depth=4, packs=scope.packs)
+    x64, _ = o64.solve(60, 1e-7)
+    hs = [-3.0, -6.0]
+    el = 90 - (np.arange(4) + 0.5) * 90 / 4
+    az = (np.arange(6) + 0.5) * 180 / 6
+    tab, _ = api.make_table(s, x, hs, el, az, log=None)
+    t64 = o64.table(x64, hs, el, az)
+    r = np.abs(tab[..., 0] - t64[..., 0]) / np.abs(t64[..., 0])
+    print(f'mixed: max relative error of I vs FP64 {r.max():.3e}, RMS {np.sqrt(np.mean(r ** 2)):.3e}, '
+          f'{info["evaluations"]} evaluations')
+    assert r.max() < 1e-4
 
 
 def test_projection_matches_numpy(gpu):
@@ -278,3 +342,43 @@ def test_rejects_bad_arguments(gpu):
     with pytest.raises(RuntimeError, match='warm transport needs field'):
         hip.call('sos_transport', dc.struct, geo.bundle.struct, 0, 4, s.rows, None, None, None, None, hip.ptr(out), None,
                  s.kernel.stream())
+
+
+def test_device_iteration_known_affine_fixed_point(gpu):
+    """The reference's D1 check (src/sosfuse2/tests/test_iteration.py:25-61) through the device Anderson path of
+    Solver.solve: a diagonal affine map with a known fixed point, one wavelength settled at the first apply and
+    frozen afterwards, and the non-convergence signal."""
+    from types import SimpleNamespace
+
+    from sosfuse2_amd.kernel import Kernel
+    from sosfuse2_amd.solver import Solver
+
+    waves, G, ns, rows = 15, 16, 2, 571
+    q = np.linspace(0.25, 0.75, waves, dtype='f4')
+    q[0] = 0
+    q = padded(q, G)
+    forcing = padded(np.linspace(0.01, 0.2, ns * rows * waves, dtype='f4').reshape(ns, rows, waves), G)
+    expected = forcing / (1 - q)
+    qd = torch.from_numpy(np.ascontiguousarray(q)).cuda()
+    fd = torch.from_numpy(np.ascontiguousarray(forcing)).cuda()
+
+    def solver(apply):
+        sv = Solver.__new__(Solver)
+        sv.s = SimpleNamespace(W=waves, geo=SimpleNamespace(ns=ns), rows=rows, monitors=(2, -3, -6, -9, -12),
+                               directions=[None] * 7)
+        sv.kernel = Kernel(L=16)
+        sv.G, sv.depth, sv.log = G, 4, None
+        sv.apply = apply
+        sv.boxes = lambda field: np.broadcast_to(field[0, 0].double().cpu().numpy(), (5, 7, G)).copy()
+        return sv
+
+    field, stats = solver(lambda x: fd + qd * x).solve(16, 0.005)
+    got = field.cpu().numpy()[..., :waves]
+    relative = float(np.max(np.abs(got - expected[..., :waves]) / expected[..., :waves]))
+    assert relative <= float(q.max() / (1 - q.max()) * 0.005) + 1e-6
+    first_done = next(i for i, row in enumerate(stats['history']) if row['converged'][0])
+    assert first_done < len(stats['history']) - 1
+    assert all(row['source_relative'][0] == 0 and row['observer_relative'][0] == 0
+               for row in stats['history'][first_done + 1:])
+    with pytest.raises(RuntimeError, match='E_NOT_CONVERGED'):
+        solver(lambda x: x + fd).solve(2, 0.005)

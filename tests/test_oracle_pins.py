@@ -196,3 +196,116 @@ def test_stokes_of_rayleigh_single_scattering():
     np.testing.assert_allclose(dop, ref, rtol=1e-10)
     # the phase function: I = 3/(16 pi) D (1 + cos^2) + (1 - D)/(4 pi), normalised over the sphere
     np.testing.assert_allclose(s[:, 0], D * 3 / (16 * math.pi) * (1 + cos**2) + (1 - D) / (4 * math.pi), rtol=1e-12)
+
+
+def _cpu_context(L=8, nr=5, nmu=1, nphi=2, path_n=4, state=None, wavelengths=(450.0, 550.0, 650.0)):
+    """A small context without a GPU: the host geometry with a stand-in kernel that only carries L."""
+    from types import SimpleNamespace
+
+    from sosfuse2_amd import api, atmosphere
+
+    cfg = dict(nr=nr, chi_grid='tiny', nmu=nmu, nphi=nphi, path_n=path_n, L=L, top_km=100.0, observer_km=0.1,
+               query_n=6, anderson_depth=4, sun_scale=1, workers=1)
+    st = state or atmosphere.config1_state()
+    scope = api.Scope(cfg, st, SimpleNamespace(L=L))
+    return scope.context(atmosphere.lane_optics(st, np.asarray(wavelengths))), scope
+
+
+def _truth_tau(g, c, e0, e1):
+    """Independent FP64 optical depth along each ray between path coordinates e0 and e1: Gauss-Legendre 48 on ten
+    equal pieces, heights from |p + t d| - R in FP64, the same profile model."""
+    x, w = np.polynomial.legendre.leggauss(48)
+    out = np.zeros((len(e0), c['W']))
+    for piece in range(10):
+        a = e0 + (e1 - e0) * piece / 10
+        b = e0 + (e1 - e0) * (piece + 1) / 10
+        for xi, wi in zip(x, w):
+            t = (a + b) / 2 + (b - a) / 2 * xi
+            q = g['p'] + t[:, None] * g['d']
+            h = np.clip(np.linalg.norm(q, axis=1) - 6371.0, 0.0, c['top'])
+            prof, st = ot._profiles(h, c, np.float64)
+            out += ((b - a) / 2 * wi)[:, None] * ot._extinction(prof, st, c, np.float64)
+    return out
+
+
+def _prefix_errors(path_n):
+    ctx, scope = _cpu_context(path_n=path_n)
+    c = ctx.constants(path_n, np.float64)
+    g = geometry_concat(scope.packs[:6])
+    ops = ot._Ops(np.float64)
+    pre = ot.Simpson(g, c, ops)
+    st = np.sqrt(np.maximum(0.0, 1 - g['d'][:, 2] ** 2))
+    closure = np.zeros((len(g['d']), c['W']))
+    for seg in range(g['edges'].shape[1] - 1):
+        pre.begin(seg)
+        for step in range(len(c['u'])):
+            q = ot.sample(g, seg, step, c, ops, st, ot.PAIRS4)
+            if not q['valid'].any():
+                continue
+            mid = pre.node(q)
+            beta = ot._extinction(q['prof'], q['st'], c, np.float64)
+            closure += np.where(q['valid'][:, None], q['ds'][:, None] * beta * np.exp(-mid), 0.0)
+        pre.finish()
+    e = g['edges']
+    valid = (e[:, 1:] - e[:, :-1]) > 1e-8
+    truth = np.zeros_like(pre.tau)
+    for seg in range(e.shape[1] - 1):
+        truth += np.where(valid[:, seg, None], _truth_tau(g, c, e[:, seg], e[:, seg + 1]), 0.0)
+    tau_err = float(np.max(np.abs(pre.tau - truth) / np.maximum(truth, 1e-12)))
+    closure_err = float(np.max(np.abs(closure - (1.0 - np.exp(-pre.tau)))))
+    return tau_err, closure_err
+
+
+def test_simpson_prefix_against_fp64_integral_and_closure():
+    """The Simpson optical-depth prefix driven as the transport drives it (begin, node(sample), finish) against an
+    independent FP64 integral of the same profiles (the truth of the reference's test_simpson_prefix.py), and the
+    closure sum_k ds beta e^-tau_k = 1 - e^-tau_end of the attenuation weights (its E4 check). Composite Simpson
+    converges at fourth order: each doubling of the nodes per segment divides the error by about 16 (asserted > 10),
+    and 32 nodes reach the FP64 truth to 5e-6. A lost factor 4 on the midpoint or a wrong panel weight stops the
+    convergence at O(1) error; a wrong node position or ds breaks the closure."""
+    errs = [_prefix_errors(n) for n in (8, 16, 32)]
+    for (a, _), (b, _) in zip(errs, errs[1:]):
+        assert a / b > 10.0
+    assert errs[-1][0] < 5e-6
+    assert errs[-1][1] < 1e-5
+
+
+def geometry_concat(packs):
+    from sosfuse2_amd import geometry
+
+    return geometry.concat_packs(packs)
+
+
+def test_anderson_iteration_known_affine_fixed_point():
+    """The reference's D1 check (src/sosfuse2/tests/test_iteration.py:25-61) on the oracle's iteration: a diagonal
+    affine map forcing + q x with a known fixed point forcing / (1 - q), one wavelength settling at the first apply
+    (q = 0) and staying frozen, and the non-convergence signal."""
+    from types import SimpleNamespace
+
+    from oracle.solve import OracleSolver
+
+    waves, ns, rows = 15, 2, 571
+    q = np.linspace(0.25, 0.75, waves, dtype='f4')[None, None, :]
+    q[:, :, 0] = 0
+    forcing = np.linspace(0.01, 0.2, ns * rows * waves, dtype='f4').reshape(ns, rows, waves)
+    expected = forcing / (1 - q)
+
+    def solver(apply):
+        o = OracleSolver.__new__(OracleSolver)
+        o.ctx = SimpleNamespace(geo=SimpleNamespace(ns=ns), rows=rows)
+        o.c = dict(W=waves)
+        o.F = np.float32
+        o.depth = 4
+        o.apply = apply
+        o.boxes = lambda field: np.broadcast_to(field[0, 0], (5, 7, waves)).copy()
+        return o
+
+    field, stats = solver(lambda x: forcing + q * x).solve(16, 0.005)
+    relative = float(np.max(np.abs(field - expected) / expected))
+    assert relative <= float(q.max() / (1 - q.max()) * 0.005) + 1e-6
+    first_done = next(i for i, row in enumerate(stats['history']) if row['converged'][0])
+    assert first_done < len(stats['history']) - 1
+    assert all(row['source_relative'][0] == 0 and row['observer_relative'][0] == 0
+               for row in stats['history'][first_done + 1:])
+    with pytest.raises(RuntimeError, match='E_NOT_CONVERGED'):
+        solver(lambda x: x + forcing).solve(2, 0.005)
