@@ -97,8 +97,8 @@ typedef struct sos_grid {
 } sos_grid;
 
 /* Radiance tensor of every ray: out_dev [rays][lanes][pairs] (pairs 4: xx yy zz xz; 6: + xy yz), FP32.
- * field_dev: source [cells][rows][lanes] (rows 0..3 read); plane_dev: combined SH plane [cells][K][lanes]
- * (FP32, or FP16 with SOS_MIXED); boundary_dev: [nc][lanes] ground radiance (with SOS_BOUNDARY).
+ * field_dev: source [cells][rows][lanes] (rows 0..9 read); plane_dev: the FP32 combined SH plane [cells][K][lanes]
+ * of sos_make_plane (read without SOS_RECORDS); boundary_dev: [nc][lanes] ground radiance (with SOS_BOUNDARY).
  * counters_dev (may be NULL): uint64 [2] += quadrature nodes evaluated and SH corner refills, summed over rays.
  * records_dev (SOS_RECORDS): [nodes][4][lanes] SH values of every (node, corner) from sos_sh_records; the plane is
  * then not read. */
@@ -116,23 +116,26 @@ int sos_plan_nodes(const sos_grid *grid, const sos_bundle *bundle, const int64_t
                    float *node_mu_dev, float *node_cp_dev, float *node_dir_dev, void *stream);
 
 /* First-light ground records: rec[id][w] = asca[w] p_aer(h) sum_p PHt gval[p][w] + ssca[w] p_sul(h) sum_p PHs gval[p][w]
- * over the ground pools of the visited cell (the pool sums of numpy_reference.py:526-558), for lanes that share the
- * asymmetries. node_dir_dev: [nodes][4] (ray direction in the node frame: mu, ex, |ey|; node height) from
- * sos_plan_nodes. The records feed sos_transport's first light through ground_records_dev. */
-int sos_ground_records(const sos_grid *grid, int64_t items, const int32_t *item_cell_dev, const int64_t *item_v0_dev,
-                       const int64_t *item_v1_dev, const int32_t *vis_dev, const float *node_dir_dev,
-                       float *records_dev, void *stream);
+ * over the ground pools of the visited cell (the pool sums of numpy_reference.py:526-558). The lanes fall into
+ * `groups` phase groups (1 to 4), group i holding the lanes whose aerosol and sulfate asymmetries equal
+ * group_g[i] and group_sg[i] (host arrays); the phase tiles are evaluated once per group. node_dir_dev:
+ * [nodes][4] (ray direction in the node frame: mu, ex, |ey|; node height) from sos_plan_nodes. The records feed
+ * sos_transport's first light through records_dev with SOS_GROUND_RECORDS. */
+int sos_ground_records(const sos_grid *grid, int32_t groups, const float *group_g, const float *group_sg, int64_t items,
+                       const int32_t *item_cell_dev, const int64_t *item_v0_dev, const int64_t *item_v1_dev,
+                       const int32_t *vis_dev, const float *node_dir_dev, float *records_dev, void *stream);
 
-/* SH values of every visit as an MFMA contraction per source cell: records[id][w] = sum_k Z[id][k] plane[cell][k][w]
- * with Z the basis row of the visit (the replaced per-corner sum of numpy_reference.py:579-600). Work item i covers
- * the visits vis_dev[v0[i] .. v1[i]) of cell item_cell[i]; vis_dev holds visit ids node * 4 + corner, and
- * vis_mu_dev / vis_cp_dev the ray's mu and the node's cos(phi) in the same (sorted) order. half = 1: the plane is
- * the FP16 plane of sos_make_plane with its per-(cell, lane) scale_dev, Z is rounded to FP16 and the products run
- * on v_mfma_f32_16x16x16f16 with FP32 accumulation (mixed precision, lanes 16 or 32; one lane runs FP32 Z against
- * the FP16 plane). */
+/* SH values of every (node, corner) as an MFMA contraction: records[4 n + c][w] = sum_k Z[n][k] plane[cell_c][k][w]
+ * with Z the basis row of node n and cell_c its corners base, base + 1, base + nc, base + nc + 1 (the replaced
+ * per-corner sum of numpy_reference.py:579-600). Work item i covers the nodes node_dev[v0[i] .. v1[i]) whose base
+ * cell is item_cell[i]; node_dev holds node ids sorted by base cell and node_mu_dev / node_cp_dev the ray's mu and
+ * the node's cos(phi) in the same order. half = 1: the plane is the split FP16 plane of sos_make_plane with its
+ * per-(cell, lane) scale_dev; Z rows are split the same way and the products run as three v_mfma_f32_16x16x16f16
+ * (hi hi, hi lo, lo hi) with FP32 accumulation (mixed precision, FP32-level accuracy; one lane recombines the plane
+ * in FP32). */
 int sos_sh_records(const sos_grid *grid, const void *plane_dev, const float *scale_dev, int32_t half, int64_t items,
                    const int32_t *item_cell_dev, const int64_t *item_v0_dev, const int64_t *item_v1_dev,
-                   const int32_t *vis_dev, const float *vis_mu_dev, const float *vis_cp_dev, float *records_dev,
+                   const int32_t *node_dev, const float *node_mu_dev, const float *node_cp_dev, float *records_dev,
                    void *stream);
 
 /* Moments [groups][4 + K][lanes] and ground flux [groups][lanes] of the per-ray radiances rad_dev
@@ -146,8 +149,9 @@ int sos_compose(const sos_grid *grid, int32_t rows, const float *moments_dev, co
                 const float *factor_dev, float *source_dev, void *stream);
 
 /* plane[cells][K][lanes] = source[:, 10:10+K] * ha[ih] + source[:, 10+K:10+2K] * sd[ih], K in m-major order.
- * half = 0: FP32, scale_dev unused. half = 1: FP16 plane / scale with scale_dev [cells][lanes] float the power of two
- * just above the column's largest magnitude (1 for an all-zero column). */
+ * half = 0: FP32 [cells][K][lanes], scale_dev unused. half = 1: split FP16, plane_dev [2][cells][K][lanes] halves
+ * holding hi = fp16(x) and lo = fp16((x - hi) 2^11) of x = c / scale, with scale_dev [cells][lanes] float the power
+ * of two just above the column's largest magnitude (1 for an all-zero column). */
 int sos_make_plane(const sos_grid *grid, int32_t rows, const float *source_dev, const float *ha_dev,
                    const float *sd_dev, int32_t half, void *plane_dev, float *scale_dev, void *stream);
 

@@ -101,9 +101,11 @@ int sos_plan_nodes(const sos_grid *g, const sos_bundle *b, const int64_t *node_o
                              (hipStream_t)stream);
 }
 
-int sos_ground_records(const sos_grid *g, int64_t items, const int32_t *item_cell_dev, const int64_t *item_v0_dev,
-                       const int64_t *item_v1_dev, const int32_t *vis_dev, const float *node_dir_dev,
-                       float *records_dev, void *stream) {
+int sos_ground_records(const sos_grid *g, int32_t groups, const float *group_g, const float *group_sg, int64_t items,
+                       const int32_t *item_cell_dev, const int64_t *item_v0_dev, const int64_t *item_v1_dev,
+                       const int32_t *vis_dev, const float *node_dir_dev, float *records_dev, void *stream) {
+    if (groups < 1 || groups > 4 || !group_g || !group_sg)
+        return arg_error("sos_ground_records: %d phase groups (1 to 4 with their asymmetries)", groups);
     if (!g || !item_cell_dev || !item_v0_dev || !item_v1_dev || !vis_dev || !node_dir_dev || !records_dev ||
         !g->goff_dev || !g->gdir_dev || !g->gval_dev || !g->lane_dev)
         return arg_error("sos_ground_records: null argument");
@@ -115,22 +117,29 @@ int sos_ground_records(const sos_grid *g, int64_t items, const int32_t *item_cel
     a.items = items; a.item_cell = item_cell_dev; a.item_v0 = item_v0_dev; a.item_v1 = item_v1_dev; a.vis = vis_dev;
     a.node_dir = reinterpret_cast<const float4 *>(node_dir_dev); a.goff = g->goff_dev; a.gdir = g->gdir_dev;
     a.gval = g->gval_dev; a.lane = g->lane_dev; a.records = records_dev;
+    a.groups = groups;
+    for (int i = 0; i < 4; ++i) {
+        a.gg[i] = i < groups ? group_g[i] : 0.0f;
+        a.sgg[i] = i < groups ? group_sg[i] : 0.0f;
+    }
     return ground_records_dispatch(a, (hipStream_t)stream);
 }
 
 int sos_sh_records(const sos_grid *g, const void *plane_dev, const float *scale_dev, int32_t half, int64_t items,
                    const int32_t *item_cell_dev, const int64_t *item_v0_dev, const int64_t *item_v1_dev,
-                   const int32_t *vis_dev, const float *vis_mu_dev, const float *vis_cp_dev, float *records_dev,
+                   const int32_t *node_dev, const float *node_mu_dev, const float *node_cp_dev, float *records_dev,
                    void *stream) {
-    if (!g || !plane_dev || !item_cell_dev || !item_v0_dev || !item_v1_dev || !vis_dev || !vis_mu_dev ||
-        !vis_cp_dev || !records_dev)
+    if (!g || !plane_dev || !item_cell_dev || !item_v0_dev || !item_v1_dev || !node_dev || !node_mu_dev ||
+        !node_cp_dev || !records_dev)
         return arg_error("sos_sh_records: null argument");
     if (half && !scale_dev) return arg_error("sos_sh_records: FP16 plane without its scale");
     if (!lanes_ok(g->lanes) || !order_ok(g->L)) return arg_error("sos_sh_records: lanes %d / L %d", g->lanes, g->L);
+    if (g->nh < 2 || g->nc < 2) return arg_error("sos_sh_records: grid too small");
     ShArgs a;
-    a.L = g->L; a.G = g->lanes; a.K = (g->L + 1) * (g->L + 2) / 2; a.plane = plane_dev; a.items = items;
-    a.item_cell = item_cell_dev; a.item_v0 = item_v0_dev; a.item_v1 = item_v1_dev; a.vis = vis_dev;
-    a.vis_mu = vis_mu_dev; a.vis_cp = vis_cp_dev; a.records = records_dev; a.scale = scale_dev;
+    a.L = g->L; a.G = g->lanes; a.K = (g->L + 1) * (g->L + 2) / 2; a.nc = g->nc; a.plane = plane_dev; a.items = items;
+    a.item_cell = item_cell_dev; a.item_v0 = item_v0_dev; a.item_v1 = item_v1_dev; a.node = node_dev;
+    a.node_mu = node_mu_dev; a.node_cp = node_cp_dev; a.records = records_dev; a.scale = scale_dev;
+    a.plane_lo = (int64_t)g->nh * g->nc * a.K * a.G;
     return sh_records_dispatch(a, half != 0, (hipStream_t)stream);
 }
 

@@ -102,15 +102,16 @@ struct ProjectArgs {
 };
 
 struct ShArgs {
-    int L, G, K;
-    const void *plane;         // [cells][K][G] float or half
+    int L, G, K, nc;
+    const void *plane;         // [cells][K][G] float, or [2][cells][K][G] half (split hi, lo)
     int64_t items;
-    const int32_t *item_cell;  // [items]
-    const int64_t *item_v0, *item_v1;
-    const int32_t *vis;        // [visits] visit ids (node * 4 + corner), sorted by cell
-    const float *vis_mu, *vis_cp;  // [visits] the ray's mu and the node's cos(phi), in sorted visit order
-    float *records;            // [nodes * 4][G]
-    const float *scale;        // [cells][G] scale of an FP16 plane (mixed precision)
+    const int32_t *item_cell;  // [items] base cell (ir * nc + j) of the item's nodes
+    const int64_t *item_v0, *item_v1;  // node range of the item in the sorted order
+    const int32_t *node;       // [nodes] node ids sorted by base cell
+    const float *node_mu, *node_cp;  // [nodes] the ray's mu and the node's cos(phi), in the same order
+    float *records;            // [nodes][4][G]: corner c of node n at (4 n + c) G
+    const float *scale;        // [cells][G] scale of a split FP16 plane (mixed precision)
+    int64_t plane_lo;          // offset (elements) of the lo plane after the hi plane: cells * K * G
 };
 
 struct GroundArgs {
@@ -125,6 +126,8 @@ struct GroundArgs {
     const float *gdir, *gval;  // [pools][3], [pools][G]
     const float *lane;         // [SOS_LANE_CONSTANTS][G]
     float *records;            // [nodes * 4][G]
+    int groups;                // phase groups: distinct (aerosol g, sulfate g) among the lanes, at most 4
+    float gg[4], sgg[4];
 };
 
 }  // namespace sos

@@ -3,10 +3,11 @@
 // For every (node, corner) visit of a first-light launch, the sunlit-ground light scattered by the aerosol layers:
 //   rec[visit][w] = asca[w] p_aer(h) sum_p PHt[visit][p] gv[cell][p][w] + ssca[w] p_sul(h) sum_p PHs[visit][p] gv[p][w]
 // with PH the paired Henyey-Greenstein phases (hg(m0 + m1) + hg(m0 - m1)) / 2 between the ray direction in the node
-// frame and the cell's pool direction (src/sosfuse2/backends/numpy_reference.py:526-558). The phases do not depend
-// on the lane when every lane shares the asymmetries (one state per launch), so the pool sum is a dense
-// [visits x pools] x [pools x lanes] product: per cell the pool radiances are staged once in LDS, phase tiles of 16
-// visits are generated in LDS, and v_mfma_f32_16x16x4f32 accumulates them. The transport then adds
+// frame and the cell's pool direction (src/sosfuse2/backends/numpy_reference.py:526-558). The phases depend on the
+// lane only through its asymmetries (g, sulfate g), so for each of the launch's phase groups (lanes sharing both; one
+// per state, at most four) the pool sum is a dense [visits x pools] x [pools x lanes] product: per cell the pool
+// radiances are staged once in LDS, phase tiles of 16 visits are generated in LDS per group, and
+// v_mfma_f32_16x16x4f32 accumulates them; each lane keeps its own group's sums. The transport then adds
 // sum_corners sw * rec instead of walking the pools of four cells at every node.
 
 #include "common.h"
@@ -31,8 +32,14 @@ __global__ __launch_bounds__(G * 4) void ground_records_kernel(GroundArgs a) {
     const int64_t p0 = a.goff[cell];
     const int np = (int)(a.goff[cell + 1] - p0);
     const int chunks = (np + PC - 1) / PC;
-    const float g = a.lane[SOS_LANE_G * G], sg = a.lane[SOS_LANE_SG * G];
     const float asca = a.lane[SOS_LANE_ASCA * G + col], ssca = a.lane[SOS_LANE_SSCA * G + col];
+    // this lane's phase group
+    int mygroup = 0;
+    {
+        const float lg = a.lane[SOS_LANE_G * G + col], lsg = a.lane[SOS_LANE_SG * G + col];
+        for (int i = 0; i < a.groups; ++i)
+            if (a.gg[i] == lg && (!a.strato || a.sgg[i] == lsg)) { mygroup = i; break; }
+    }
     auto stage = [&](int ch) {
         for (int e = tid; e < PC * G; e += NT) {
             const int p = e / G, w = e % G;
@@ -66,29 +73,39 @@ __global__ __launch_bounds__(G * 4) void ground_records_kernel(GroundArgs a) {
                 __syncthreads();
             }
             const int n = min(PC, np - ch * PC);
-            for (int e = tid; e < V * PC; e += NT) {
-                const int v = e / PC, p = e % PC;
-                float pt = 0.0f, ps = 0.0f;
-                if (v < nv && p < n) {
-                    const float4 dv = sdir[v];
-                    const float m0 = dv.x * gd[p][2] + dv.y * gd[p][0];
-                    const float m1 = dv.z * gd[p][1];
-                    const float up = clampf(m0 + m1, -1.0f, 1.0f), dn = clampf(m0 - m1, -1.0f, 1.0f);
-                    pt = (hg(up, g) + hg(dn, g)) / 2.0f;
-                    if (a.strato) ps = (hg(up, sg) + hg(dn, sg)) / 2.0f;
+            for (int grp = 0; grp < a.groups; ++grp) {
+                const float g = a.gg[grp], sg = a.sgg[grp];
+                for (int e = tid; e < V * PC; e += NT) {
+                    const int v = e / PC, p = e % PC;
+                    float pt = 0.0f, ps = 0.0f;
+                    if (v < nv && p < n) {
+                        const float4 dv = sdir[v];
+                        const float m0 = dv.x * gd[p][2] + dv.y * gd[p][0];
+                        const float m1 = dv.z * gd[p][1];
+                        const float up = clampf(m0 + m1, -1.0f, 1.0f), dn = clampf(m0 - m1, -1.0f, 1.0f);
+                        pt = (hg(up, g) + hg(dn, g)) / 2.0f;
+                        if (a.strato) ps = (hg(up, sg) + hg(dn, sg)) / 2.0f;
+                    }
+                    PT[v][p] = pt;
+                    PU[v][p] = ps;
                 }
-                PT[v][p] = pt;
-                PU[v][p] = ps;
-            }
-            __syncthreads();
-            const int kk = lane >> 4, row = lane & 15;
+                __syncthreads();
+                // every wavefront runs the products (the MFMA needs the whole wavefront); lanes of other groups
+                // discard them
+                floatx4 t = floatx4{0.f, 0.f, 0.f, 0.f}, u = floatx4{0.f, 0.f, 0.f, 0.f};
+                const int kk = lane >> 4, row = lane & 15;
 #pragma unroll 8
-            for (int s = 0; s < PC / 4; ++s) {
-                const float b = B[4 * s + kk][col];
-                acct = __builtin_amdgcn_mfma_f32_16x16x4f32(PT[row][4 * s + kk], b, acct, 0, 0, 0);
-                if (a.strato) accs = __builtin_amdgcn_mfma_f32_16x16x4f32(PU[row][4 * s + kk], b, accs, 0, 0, 0);
+                for (int s = 0; s < PC / 4; ++s) {
+                    const float b = B[4 * s + kk][col];
+                    t = __builtin_amdgcn_mfma_f32_16x16x4f32(PT[row][4 * s + kk], b, t, 0, 0, 0);
+                    if (a.strato) u = __builtin_amdgcn_mfma_f32_16x16x4f32(PU[row][4 * s + kk], b, u, 0, 0, 0);
+                }
+                if (grp == mygroup) {
+                    acct = acct + t;
+                    accs = accs + u;
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {

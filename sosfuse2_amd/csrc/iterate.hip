@@ -71,9 +71,10 @@ __global__ void plane_kernel(int nh, int nc, int L, int rows, int G, int strato,
     }
 }
 
-// FP16 plane for the mixed-precision contraction: one workgroup per cell. scale[cell][w] is the power of two just
-// above max_j |c[cell][j][w]| (1 for an all-zero column), and the plane holds c / scale in [-1, 1] rounded to
-// nearest FP16, so every coefficient keeps an absolute error of at most 2^-12 scale and no column overflows.
+// Split FP16 plane for the mixed-precision contraction: one workgroup per cell. scale[cell][w] is the power of two
+// just above max_j |c[cell][j][w]| (1 for an all-zero column); x = c / scale lies in [-1, 1] and is stored as
+// hi = fp16(x) in the first plane and lo = fp16((x - hi) 2^11) in the second (cells x K x G further on), so
+// hi + lo 2^-11 carries x to 2^-23 and neither part leaves FP16's range.
 __global__ __launch_bounds__(256) void plane_half_kernel(int nc, int L, int rows, int G, int strato,
                                                          const float *__restrict__ src, const float *__restrict__ ha,
                                                          const float *__restrict__ sd, _Float16 *__restrict__ plane,
@@ -110,8 +111,14 @@ __global__ __launch_bounds__(256) void plane_half_kernel(int nc, int L, int rows
     }
     __syncthreads();
     const float iv = inv[w];
-    _Float16 *out = plane + cell * K * G + w;
-    for (int j = q; j < K; j += nq) out[(int64_t)j * G] = (_Float16)(value(j) * iv);
+    _Float16 *hi = plane + cell * K * G + w;
+    _Float16 *lo = hi + (int64_t)gridDim.x * K * G;
+    for (int j = q; j < K; j += nq) {
+        const float x = value(j) * iv;
+        const _Float16 h = (_Float16)x;
+        hi[(int64_t)j * G] = h;
+        lo[(int64_t)j * G] = (_Float16)((x - (float)h) * 2048.0f);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------- metrics
@@ -177,25 +184,47 @@ __global__ void norms_kernel(int64_t cells, int rows, int G, const float *__rest
     }
 }
 
-__global__ void residual_kernel(int64_t cells, int rows, int G, const float *__restrict__ g,
-                                const float *__restrict__ x, const float *__restrict__ norms,
-                                const unsigned *__restrict__ maxbits, const float *__restrict__ obs, int nobs,
-                                float *__restrict__ f) {
+// f[w][i] = (g - x)[i][w] / max(norm[cell(i)][w], 1e-8 max_cell norm[w]) for the source rows i < cells * rows and
+// f[w][nsrc + o] = obs[w][o]: the field is row-major over lanes and the residual lane-major, so each block moves a
+// tile of 64 rows through LDS (coalesced reads of g and x, coalesced writes of every lane's row).
+__global__ __launch_bounds__(256) void residual_kernel(int64_t cells, int rows, int G, const float *__restrict__ g,
+                                                       const float *__restrict__ x, const float *__restrict__ norms,
+                                                       const unsigned *__restrict__ maxbits,
+                                                       const float *__restrict__ obs, int nobs,
+                                                       float *__restrict__ f) {
+    constexpr int R = 64;
+    __shared__ float tile[32][R + 1];
+    __shared__ float floor_[32];
     const int64_t nsrc = cells * rows;
     const int64_t n = nsrc + nobs;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        if (i < nsrc) {
-            const int64_t cell = i / rows;
-            for (int w = 0; w < G; ++w) {
-                const float nrm = norms[cell * G + w];
-                const float scale = fmaxf(nrm, __uint_as_float(maxbits[w]) * 1e-8f);
-                f[(int64_t)w * n + i] = (g[i * G + w] - x[i * G + w]) / scale;
+    if (threadIdx.x < G) floor_[threadIdx.x] = __uint_as_float(maxbits[threadIdx.x]) * 1e-8f;
+    __syncthreads();
+    const int64_t tiles = (nsrc + R - 1) / R;
+    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int64_t r0 = t * R;
+        for (int e = threadIdx.x; e < R * G; e += blockDim.x) {
+            const int r = e / G, w = e % G;
+            const int64_t i = r0 + r;
+            float v = 0.0f;
+            if (i < nsrc) {
+                const float scale = fmaxf(norms[(i / rows) * G + w], floor_[w]);
+                v = (g[i * G + w] - x[i * G + w]) / scale;
             }
-        } else {
-            const int64_t o = i - nsrc;
-            for (int w = 0; w < G; ++w) f[(int64_t)w * n + i] = obs[(int64_t)w * nobs + o];
+            tile[w][r] = v;
         }
+        __syncthreads();
+        for (int e = threadIdx.x; e < R * G; e += blockDim.x) {
+            const int w = e / R, r = e % R;
+            const int64_t i = r0 + r;
+            if (i < nsrc) f[(int64_t)w * n + i] = tile[w][r];
+        }
+        __syncthreads();
     }
+    if (blockIdx.x == 0)
+        for (int e = threadIdx.x; e < G * nobs; e += blockDim.x) {
+            const int w = e / nobs, o = e % nobs;
+            f[(int64_t)w * n + nsrc + o] = obs[(int64_t)w * nobs + o];
+        }
 }
 
 // Block partial dot products: part[b][w][j] = sum over the block's range of f[slot][w][i] f[j][w][i].
@@ -335,8 +364,9 @@ int residual_launch(int64_t cells, int rows, int G, const float *g, const float 
     unsigned *maxbits = reinterpret_cast<unsigned *>(work + cells * G);
     hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, maxbits, G);
     hipLaunchKernelGGL(norms_kernel, dim3(blocks_for(cells * G)), dim3(256), 0, s, cells, rows, G, g, norms, maxbits);
-    hipLaunchKernelGGL(residual_kernel, dim3(blocks_for(cells * rows + nobs)), dim3(256), 0, s, cells, rows, G, g, x,
-                       norms, maxbits, obs, nobs, f);
+    const int64_t tiles = (cells * rows + 63) / 64;
+    hipLaunchKernelGGL(residual_kernel, dim3((unsigned)max((int64_t)1, min(tiles, (int64_t)8192))), dim3(256), 0, s,
+                       cells, rows, G, g, x, norms, maxbits, obs, nobs, f);
     return check("anderson_residual");
 }
 

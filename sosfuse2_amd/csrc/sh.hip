@@ -1,15 +1,16 @@
 // Spherical-harmonic reconstruction of the source at every (node, corner) visit as an MFMA contraction (gfx950).
 //
-// aero[visit][w] = sum_k Z[visit][k] c[cell][k][w],  Z[visit][k] = sqrt2^(m>0) P_lm(mu) cos(m phi)
-// where the visit belongs to a quadrature node of a ray (mu = the ray's direction cosine to the sun axis, cos(m phi)
-// the node's azimuth) and c is the combined SH plane of the visited cell. This is the per-corner SH sum of
-// src/sosfuse2/backends/numpy_reference.py:579-600 (cache[m] = sum_l Y c, aero = sum_m cm cache[m]), regrouped by
-// source cell: visits are sorted by cell once per bundle (the geometry is fixed), so each cell's K x lanes
-// coefficients are loaded once into registers as the B operand and every visiting row streams through
-// v_mfma_f32_16x16x4f32. The plan kernels walk the same node geometry as the transport.
+// rec[node][c][w] = sum_k Z[node][k] c[cell_c][k][w],  Z[node][k] = sqrt2^(m>0) P_lm(mu) cos(m phi)
+// with mu the ray's direction cosine to the sun axis and cos(m phi) the node's azimuth, cell_c the four bilinear
+// corner cells of the node. This is the per-corner SH sum of src/sosfuse2/backends/numpy_reference.py:579-600
+// (cache[m] = sum_l Y c, aero = sum_m cm cache[m]), regrouped: the basis row Z of a node is the same for its four
+// corners, so nodes are sorted by their base cell once per bundle (the geometry is fixed) and one workgroup takes a
+// run of nodes of one base cell. Every wavefront holds the K x 16 coefficients of one corner cell and one 16-lane
+// N-tile in registers as the MFMA B operand; tiles of 16 basis rows are generated once in LDS and stream past all
+// four corners. The plan kernels walk the same node geometry as the transport.
 //
-// K order (the plane layout): m-major, k' = moff(m) + (l - m). MFMA k-steps are permuted so that one ds_read_b128
-// feeds four consecutive MFMAs: step s covers k' = 16 (s/4) + 4 (lane/16) + (s%4).
+// K order (the plane layout): m-major, k' = moff(m) + (l - m). FP32 MFMA k-steps are permuted so that one
+// ds_read_b128 feeds four consecutive MFMAs: step s covers k' = 16 (s/4) + 4 (lane/16) + (s%4).
 
 #include "common.h"
 
@@ -37,7 +38,7 @@ __device__ inline void stage_constants(double *aa, double *bb, int nt) {
     }
 }
 
-// Per visit: the diagonal seeds 1/sqrt(4 pi) prod dd_j st (FP64, arithmetic.py:95-97) and cos(m phi) by the FP32
+// Per node: the diagonal seeds 1/sqrt(4 pi) prod dd_j st (FP64, arithmetic.py:95-97) and cos(m phi) by the FP32
 // Chebyshev recurrence (arithmetic.py:112-119).
 template <int L, int V>
 __device__ inline void visit_seeds(int v, double mu, float cp, const double *dd, double (*seed)[V], float (*cmv)[V]) {
@@ -55,7 +56,7 @@ __device__ inline void visit_seeds(int v, double mu, float cp, const double *dd,
     }
 }
 
-// Z rows of V visits into LDS. Task (visit, p) runs the Legendre recurrence over l in FP64 for the two orders
+// Z rows of V nodes into LDS. Task (node, p) runs the Legendre recurrence over l in FP64 for the two orders
 // m = p and m = L - p side by side (their lengths add up to L + 2, so tasks are balanced and each thread has two
 // independent dependency chains); each basis value is rounded to FP32 before the FP32 product with cos(m phi)
 // (the reference's FP64 basis, rounded).
@@ -110,13 +111,108 @@ __device__ inline void build_z(T (*Z)[KZ], const double *mus, const double (*see
     }
 }
 
+// Mixed precision: split-FP16 operands on v_mfma_f32_16x16x16f16 with FP32 accumulation, the three-product scheme
+// of Ootomo and Yokota (2022, "Recovering single precision accuracy from Tensor Cores while surpassing the FP32
+// theoretical peak performance"): x = hi + lo 2^-11 with hi = fp16(x) and lo = fp16((x - hi) 2^11), and
+// a b = ah bh + (ah bl + al bh) 2^-11 (the lo lo term, 2^-22 relative, is dropped). The plane is stored scaled per
+// (cell, lane) to [-1, 1] (plane = c / scale, scale a power of two) so hi and lo stay in FP16's normal range; Z rows
+// are generated by the FP32 recurrence and split on the way into LDS. Each k-step of 16 costs three 8-pass MFMAs
+// instead of four 32-pass FP32 MFMAs; the error per record is about 2^-21 of sum_k |Z_k c_k|, the FP32 level.
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+
+// Per node, FP32: the diagonal seeds 1/sqrt(4 pi) prod dd_j st and cos(m phi) by the Chebyshev recurrence. Threads
+// v < V run the seed chain, threads V <= t < 2V the cosine chain.
+template <int L, int V>
+__device__ inline void visit_seeds32(int t, float mu, float cp, const float *dd, float (*seed)[V], float (*cmv)[V]) {
+    if (t < V) {
+        const float st = sqrtf(fmaxf(0.0f, 1.0f - mu * mu));
+        float diag = 0.28209479177387814f;
+        for (int m = 0; m <= L; ++m) {
+            if (m) diag = (diag * dd[m]) * st;
+            seed[m][t] = diag;
+        }
+    } else {
+        const int v = t - V;
+        float c0 = 1.0f, c1 = cp;
+        for (int m = 0; m <= L; ++m) {
+            float cm;
+            if (m == 0) cm = 1.0f;
+            else if (m == 1) cm = cp;
+            else { cm = (2.0f * cp) * c1 - c0; c0 = c1; c1 = cm; }
+            cmv[m][v] = cm;
+        }
+    }
+}
+
+__device__ inline void split_store(_Float16 *hi, _Float16 *lo, float z) {
+    const _Float16 h = (_Float16)z;
+    *hi = h;
+    *lo = (_Float16)((z - (float)h) * 2048.0f);
+}
+
+// Z rows of V nodes, split into hi and lo planes: task (node, p) runs the FP32 recurrence of the orders m = p and
+// m = L - p side by side (as build_z).
+template <int L, int V, int ZS>
+__device__ inline void build_z_split(_Float16 (*Zh)[ZS], _Float16 (*Zl)[ZS], const float *mus, const float (*seed)[V],
+                                     const float (*cmv)[V], const float *aa, const float *bb, int nv, int nt) {
+    constexpr int PAIRS = L / 2 + 1;
+    for (int task = threadIdx.x; task < V * PAIRS; task += nt) {
+        const int v = task % V, p = task / V;
+        const int ma = p, mb = L - p;
+        const bool two = mb != ma;
+        const int oa = moff<L>(ma), ob = moff<L>(mb);
+        if (v >= nv) {
+            for (int l = ma; l <= L; ++l) Zh[v][oa + l - ma] = Zl[v][oa + l - ma] = (_Float16)0.0f;
+            if (two)
+                for (int l = mb; l <= L; ++l) Zh[v][ob + l - mb] = Zl[v][ob + l - mb] = (_Float16)0.0f;
+            continue;
+        }
+        const float mu = mus[v];
+        const float ca = ma ? 1.41421356237309515f * cmv[ma][v] : cmv[ma][v];
+        const float cb = 1.41421356237309515f * cmv[mb][v];
+        float a0 = seed[ma][v], b0 = seed[mb][v];
+        split_store(&Zh[v][oa], &Zl[v][oa], a0 * ca);
+        if (two) split_store(&Zh[v][ob], &Zl[v][ob], b0 * cb);
+        float a1 = (sqrtf(2.0f * ma + 3.0f) * mu) * a0;
+        split_store(&Zh[v][oa + 1], &Zl[v][oa + 1], a1 * ca);
+        float b1 = 0.0f;
+        if (two && mb < L) {
+            b1 = (sqrtf(2.0f * mb + 3.0f) * mu) * b0;
+            split_store(&Zh[v][ob + 1], &Zl[v][ob + 1], b1 * cb);
+        }
+        int la = ma + 2, lb = mb + 2;
+#pragma unroll 2
+        for (; two && lb <= L; ++la, ++lb) {
+            const int ka = kidx(la, ma), kb = kidx(lb, mb);
+            const float a2 = (aa[ka] * mu) * a1 - bb[ka] * a0;
+            const float b2 = (aa[kb] * mu) * b1 - bb[kb] * b0;
+            split_store(&Zh[v][oa + la - ma], &Zl[v][oa + la - ma], a2 * ca);
+            split_store(&Zh[v][ob + lb - mb], &Zl[v][ob + lb - mb], b2 * cb);
+            a0 = a1; a1 = a2;
+            b0 = b1; b1 = b2;
+        }
+#pragma unroll 4
+        for (; la <= L; ++la) {
+            const int ka = kidx(la, ma);
+            const float a2 = (aa[ka] * mu) * a1 - bb[ka] * a0;
+            split_store(&Zh[v][oa + la - ma], &Zl[v][oa + la - ma], a2 * ca);
+            a0 = a1; a1 = a2;
+        }
+    }
+}
+
+// Corner c of a node with base cell b (ir * nc + j): b, b + 1, b + nc, b + nc + 1, the transport's corner order.
+__device__ inline int corner_cell(int b, int c, int nc) { return b + (c & 1) + (c >> 1) * nc; }
+
+// FP32: 4 x G/16 wavefronts, wavefront (c, nt) holds corner c's coefficients of lanes 16 nt .. 16 nt + 15.
 template <int G, int L>
-__global__ __launch_bounds__(G * 4) __attribute__((amdgpu_waves_per_eu(2, 2))) void sh_records_kernel(ShArgs a) {
+__global__ __launch_bounds__(G * 16) __attribute__((amdgpu_waves_per_eu(2, 2))) void sh_records_kernel(ShArgs a) {
     constexpr int K = (L + 1) * (L + 2) / 2;
     constexpr int KZ = (K + 15) / 16 * 16;   // padded row
     constexpr int KS = KZ / 4;               // MFMA k-steps
-    constexpr int NT = G * 4;                // one wavefront per 16-lane N-tile
-    constexpr int V = 16;                    // visits per round (one M-tile)
+    constexpr int NW = G / 16;
+    constexpr int NT = 64 * 4 * NW;
+    constexpr int V = 16;                    // nodes per round (one M-tile)
     constexpr int ZS = KZ + 4;               // row stride = 4 (mod 32 banks): 16 rows spread over the LDS banks
     __shared__ float Z[V][ZS];
     __shared__ double aa[K], bb[K];
@@ -125,33 +221,32 @@ __global__ __launch_bounds__(G * 4) __attribute__((amdgpu_waves_per_eu(2, 2))) v
     __shared__ double smu[V];
     __shared__ double dd[L + 1];
     __shared__ int sid[V];
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int nt = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int corner = wave / NW, nt = wave % NW;
+    const int col = nt * 16 + (lane & 15);
     const int64_t item = blockIdx.x;
-    const int cell = a.item_cell[item];
+    const int cell = corner_cell(a.item_cell[item], corner, a.nc);
     const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
     stage_constants<L>(aa, bb, NT);
     for (int m = tid; m <= L; m += NT) dd[m] = m ? -sqrt((2.0 * m + 1) / (2.0 * m)) : 0.0;
     for (int e = tid; e < V * (KZ - K); e += NT) Z[e / (KZ - K)][K + e % (KZ - K)] = 0.0f;
-    // B operand in registers: b[s] = c[cell][k'(s, lane)][nt*16 + lane%16]
+    // B operand in registers: b[s] = c[cell][k'(s, lane)][col]
     float b[KS];
     {
-        const int col = nt * 16 + (lane & 15);
-        const int64_t base = (int64_t)cell * K * G + col;
+        const float *pl = reinterpret_cast<const float *>(a.plane) + (int64_t)cell * K * G + col;
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             const int k = 16 * (s >> 2) + 4 * (lane >> 4) + (s & 3);
-            b[s] = k < K ? reinterpret_cast<const float *>(a.plane)[base + (int64_t)k * G] : 0.0f;
+            b[s] = k < K ? pl[(int64_t)k * G] : 0.0f;
         }
     }
-    // visit data of the next round is loaded one round ahead (contiguous: the plan stores mu and cos(phi) in sorted
-    // visit order), so its memory latency overlaps the current round's work
+    // node data of the next round is loaded one round ahead (contiguous in the sorted order)
     int nid = -1;
     float nmu = 0.0f, ncp = 1.0f;
     if (tid < V && v0 + tid < v1) {
-        nid = a.vis[v0 + tid];
-        nmu = a.vis_mu[v0 + tid];
-        ncp = a.vis_cp[v0 + tid];
+        nid = a.node[v0 + tid];
+        nmu = a.node_mu[v0 + tid];
+        ncp = a.node_cp[v0 + tid];
     }
     for (int64_t r0 = v0; r0 < v1; r0 += V) {
         const int nv = (int)min((int64_t)V, v1 - r0);
@@ -162,9 +257,9 @@ __global__ __launch_bounds__(G * 4) __attribute__((amdgpu_waves_per_eu(2, 2))) v
             const int64_t nx = r0 + V + tid;
             nid = -1;
             if (nx < v1) {
-                nid = a.vis[nx];
-                nmu = a.vis_mu[nx];
-                ncp = a.vis_cp[nx];
+                nid = a.node[nx];
+                nmu = a.node_mu[nx];
+                ncp = a.node_cp[nx];
             }
             sid[tid] = tid < nv ? id : -1;
             smu[tid] = tid < nv ? mu : 0.0;
@@ -188,117 +283,159 @@ __global__ __launch_bounds__(G * 4) __attribute__((amdgpu_waves_per_eu(2, 2))) v
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int row = 4 * (lane >> 4) + v;
-            if (row < nv) a.records[(int64_t)sid[row] * G + nt * 16 + (lane & 15)] = d[v];
+            if (row < nv) a.records[((int64_t)sid[row] * 4 + corner) * G + col] = d[v];
         }
         __syncthreads();
     }
 }
 
-// Mixed precision: the plane is FP16 scaled per (cell, lane) (plane = c / scale[cell][lane]), Z rows are FP16, and
-// v_mfma_f32_16x16x16f16 accumulates in FP32; the epilogue multiplies by the scale. 32 visits per round
-// (two M-tiles); wavefront w takes M-tile w / (G/16) and N-tile w % (G/16).
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+// The split kernel takes one 16-lane N-tile per workgroup (blockIdx.y) and divides the K range over KP wavefronts
+// per corner (KP = 2 at L = 32), so that each keeps only its part of the hi / lo B operands in registers (about 170
+// VGPRs, two wavefronts per SIMD); the parts are summed through LDS after each round. The basis rows are generated
+// once per N-tile.
+template <int L>
+struct SplitK { static constexpr int KP = L >= 32 ? 2 : 1; };
 
 template <int G, int L>
-__global__ __launch_bounds__(G * 8) __attribute__((amdgpu_waves_per_eu(2, 2))) void sh_records_half_kernel(ShArgs a) {
+__global__ __launch_bounds__(256 * SplitK<L>::KP) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void sh_records_split_kernel(ShArgs a) {
     constexpr int K = (L + 1) * (L + 2) / 2;
     constexpr int KZ = (K + 15) / 16 * 16;
     constexpr int KS = KZ / 16;               // MFMA k-steps of 16
-    constexpr int NT = G * 8;
-    constexpr int V = 32;
+    constexpr int KP = SplitK<L>::KP;
+    constexpr int KSH = (KS + KP - 1) / KP;   // k-steps of one wavefront
+    constexpr int NT = 256 * KP;              // 4 corners x KP parts
+    constexpr int V = 16;                     // nodes per round (one M-tile)
     constexpr int ZS = KZ + 8;                // halves: row stride = 4 (mod 32 banks) in 32-bit words
-    __shared__ _Float16 Z[V][ZS];
-    __shared__ double aa[K], bb[K];
-    __shared__ double seed[L + 1][V];
+    __shared__ _Float16 Zh[V][ZS], Zl[V][ZS];
+    __shared__ float aa[K], bb[K];
+    __shared__ float seed[L + 1][V];
     __shared__ float cmv[L + 1][V];
-    __shared__ double smu[V];
-    __shared__ double dd[L + 1];
+    __shared__ float smu[V], scp[V];
+    __shared__ float dd[L + 1];
     __shared__ int sid[V];
+    __shared__ floatx4 red[KP > 1 ? 4 * 64 : 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int mt = wave / (G / 16), nt = wave % (G / 16);
+    const int corner = wave & 3, kh = wave >> 2;
+    const int col = blockIdx.y * 16 + (lane & 15);
     const int64_t item = blockIdx.x;
-    const int cell = a.item_cell[item];
+    const int cell = corner_cell(a.item_cell[item], corner, a.nc);
     const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
-    stage_constants<L>(aa, bb, NT);
-    for (int m = tid; m <= L; m += NT) dd[m] = m ? -sqrt((2.0 * m + 1) / (2.0 * m)) : 0.0;
-    for (int e = tid; e < V * (KZ - K); e += NT) Z[e / (KZ - K)][K + e % (KZ - K)] = (_Float16)0.0f;
-    const int col = nt * 16 + (lane & 15);
+    for (int k = tid; k < K; k += NT) {
+        int l = 0;
+        while ((l + 1) * (l + 2) / 2 <= k) ++l;
+        const int m = k - l * (l + 1) / 2;
+        const double fl = l, fm = m;
+        aa[k] = l > m ? (float)sqrt((4.0 * fl * fl - 1.0) / (fl * fl - fm * fm)) : 0.0f;
+        bb[k] = l > m + 1 ? (float)sqrt((2.0 * fl + 1.0) * ((fl - 1.0) * (fl - 1.0) - fm * fm) /
+                                        ((2.0 * fl - 3.0) * (fl * fl - fm * fm)))
+                          : 0.0f;
+    }
+    for (int m = tid; m <= L; m += NT) dd[m] = m ? -(float)sqrt((2.0 * m + 1) / (2.0 * m)) : 0.0f;
+    for (int e = tid; e < V * (KZ - K); e += NT) {
+        Zh[e / (KZ - K)][K + e % (KZ - K)] = (_Float16)0.0f;
+        Zl[e / (KZ - K)][K + e % (KZ - K)] = (_Float16)0.0f;
+    }
     const float scale = a.scale[(int64_t)cell * G + col];
-    // B operand: b[s] holds plane[cell][16 s + 4 (lane / 16) + 0..3][col]
-    half4_t b[KS];
+    // B operands: bh[i], bl[i] hold the hi / lo plane at k = 16 (kh KSH + i) + 4 (lane / 16) + 0..3, column col
+    half4_t bh[KSH], bl[KSH];
     {
-        const _Float16 *pl = reinterpret_cast<const _Float16 *>(a.plane) + (int64_t)cell * K * G + col;
+        const _Float16 *ph = reinterpret_cast<const _Float16 *>(a.plane) + (int64_t)cell * K * G + col;
+        const _Float16 *pl = ph + a.plane_lo;
 #pragma unroll
-        for (int s = 0; s < KS; ++s)
+        for (int i = 0; i < KSH; ++i)
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const int k = 16 * s + 4 * (lane >> 4) + u;
-                b[s][u] = k < K ? pl[(int64_t)k * G] : (_Float16)0.0f;
+                const int k = 16 * (kh * KSH + i) + 4 * (lane >> 4) + u;
+                bh[i][u] = k < K ? ph[(int64_t)k * G] : (_Float16)0.0f;
+                bl[i][u] = k < K ? pl[(int64_t)k * G] : (_Float16)0.0f;
             }
     }
     int nid = -1;
     float nmu = 0.0f, ncp = 1.0f;
     if (tid < V && v0 + tid < v1) {
-        nid = a.vis[v0 + tid];
-        nmu = a.vis_mu[v0 + tid];
-        ncp = a.vis_cp[v0 + tid];
+        nid = a.node[v0 + tid];
+        nmu = a.node_mu[v0 + tid];
+        ncp = a.node_cp[v0 + tid];
     }
     for (int64_t r0 = v0; r0 < v1; r0 += V) {
         const int nv = (int)min((int64_t)V, v1 - r0);
         if (tid < V) {
-            const int id = nid;
-            const double mu = (double)nmu;
-            const float cp = ncp;
             const int64_t nx = r0 + V + tid;
+            sid[tid] = tid < nv ? nid : -1;
+            smu[tid] = tid < nv ? nmu : 0.0f;
+            scp[tid] = tid < nv ? ncp : 1.0f;
             nid = -1;
             if (nx < v1) {
-                nid = a.vis[nx];
-                nmu = a.vis_mu[nx];
-                ncp = a.vis_cp[nx];
+                nid = a.node[nx];
+                nmu = a.node_mu[nx];
+                ncp = a.node_cp[nx];
             }
-            sid[tid] = tid < nv ? id : -1;
-            smu[tid] = tid < nv ? mu : 0.0;
-            visit_seeds<L, V>(tid, tid < nv ? mu : 0.0, tid < nv ? cp : 1.0f, dd, seed, cmv);
         }
         __syncthreads();
-        build_z<L, V, ZS, _Float16>(Z, smu, seed, cmv, aa, bb, nv, NT);
-        __syncthreads();
-        floatx4 acc[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
-        const _Float16 *zrow = &Z[mt * 16 + (lane & 15)][4 * (lane >> 4)];
-#pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            const half4_t av = *reinterpret_cast<const half4_t *>(zrow + 16 * s);
-            acc[s & 1] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, b[s], acc[s & 1], 0, 0, 0);
+        if (tid < 2 * V) {
+            const int v = tid % V;
+            visit_seeds32<L, V>(tid, smu[v], scp[v], dd, seed, cmv);
         }
-        const floatx4 d = acc[0] + acc[1];
+        __syncthreads();
+        build_z_split<L, V, ZS>(Zh, Zl, smu, seed, cmv, aa, bb, nv, NT);
+        __syncthreads();
+        floatx4 am = floatx4{0.f, 0.f, 0.f, 0.f}, c1 = am, c2 = am;
+        const _Float16 *hrow = &Zh[lane & 15][4 * (lane >> 4) + 16 * kh * KSH];
+        const _Float16 *lrow = &Zl[lane & 15][4 * (lane >> 4) + 16 * kh * KSH];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int row = mt * 16 + 4 * (lane >> 4) + v;
-            if (row < nv) a.records[(int64_t)sid[row] * G + col] = d[v] * scale;
+        for (int i = 0; i < KSH; ++i) {
+            if (kh * KSH + i < KS) {
+                const half4_t ah = *reinterpret_cast<const half4_t *>(hrow + 16 * i);
+                const half4_t al = *reinterpret_cast<const half4_t *>(lrow + 16 * i);
+                am = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh[i], am, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl[i], c1, 0, 0, 0);
+                c2 = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh[i], c2, 0, 0, 0);
+            }
+        }
+        floatx4 d = am + (c1 + c2) * (1.0f / 2048.0f);
+        if constexpr (KP > 1) {
+            if (kh == 1) red[corner * 64 + lane] = d;
+            __syncthreads();
+            if (kh == 0) d = d + red[corner * 64 + lane];
+        }
+        if (kh == 0) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 4 * (lane >> 4) + v;
+                if (row < nv) a.records[((int64_t)sid[row] * 4 + corner) * G + col] = d[v] * scale;
+            }
         }
         __syncthreads();
     }
 }
 
-// W = 1 lanes: one thread per visit, dot product against the cell's coefficients staged in LDS (an FP16 plane is
-// scaled back to FP32 there; the basis stays FP32).
+// One lane (W = 1): one thread per node, FP64 recurrence, dot products against the four corner cells' coefficients
+// staged in LDS (a split FP16 plane is recombined and scaled back to FP32 there).
 template <int L, bool HALF>
 __global__ __launch_bounds__(256) void sh_records_scalar(ShArgs a) {
     constexpr int K = (L + 1) * (L + 2) / 2;
-    __shared__ float c[K];
+    __shared__ float c[4][K];
     __shared__ double aa[K], bb[K];
     const int64_t item = blockIdx.x;
-    const int cell = a.item_cell[item];
-    const float sc = HALF ? a.scale[cell] : 1.0f;
-    for (int k = threadIdx.x; k < K; k += 256) c[k] = plane_at<L, HALF>(a.plane, (int64_t)cell * K + k) * sc;
+    const int base = a.item_cell[item];
+    for (int e = threadIdx.x; e < 4 * K; e += 256) {
+        const int cell = corner_cell(base, e / K, a.nc), k = e % K;
+        const int64_t i = (int64_t)cell * K + k;
+        if constexpr (HALF)
+            c[e / K][k] = (plane_at<L, true>(a.plane, i) + plane_at<L, true>(a.plane, i + a.plane_lo) * (1.0f / 2048.0f)) *
+                          a.scale[cell];
+        else
+            c[e / K][k] = plane_at<L, false>(a.plane, i);
+    }
     stage_constants<L>(aa, bb, 256);
     __syncthreads();
     for (int64_t r = a.item_v0[item] + threadIdx.x; r < a.item_v1[item]; r += 256) {
-        const int id = a.vis[r];
-        const double mu = a.vis_mu[r];
-        const float cp = a.vis_cp[r];
+        const int id = a.node[r];
+        const double mu = a.node_mu[r];
+        const float cp = a.node_cp[r];
         const double st = sqrt(fmax(0.0, 1.0 - mu * mu));
-        float sum = 0.0f;
+        float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         float cm0 = 1.0f, cm1 = cp;
         double diag = 0.28209479177387814;
         for (int m = 0; m <= L; ++m) {
@@ -310,21 +447,30 @@ __global__ __launch_bounds__(256) void sh_records_scalar(ShArgs a) {
             const int o = moff<L>(m);
             const double mult = m ? 1.41421356237309515 : 1.0;
             double z0 = diag;
-            float acc = ((float)(mult * z0) * cm) * c[o];
+            float acc[4];
+            float z = (float)(mult * z0) * cm;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = z * c[q][o];
             if (m < L) {
                 double z1 = (sqrt(2.0 * m + 3.0) * mu) * z0;
-                acc = acc + ((float)(mult * z1) * cm) * c[o + 1];
+                z = (float)(mult * z1) * cm;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] = acc[q] + z * c[q][o + 1];
                 for (int l = m + 2; l <= L; ++l) {
                     const int k = kidx(l, m);
                     const double z2 = (aa[k] * mu) * z1 - bb[k] * z0;
-                    acc = acc + ((float)(mult * z2) * cm) * c[o + l - m];
+                    z = (float)(mult * z2) * cm;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[q] = acc[q] + z * c[q][o + l - m];
                     z0 = z1;
                     z1 = z2;
                 }
             }
-            sum = sum + acc;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sum[q] = sum[q] + acc[q];
         }
-        a.records[id] = sum;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a.records[(int64_t)id * 4 + q] = sum[q];
     }
 }
 
@@ -335,8 +481,10 @@ static int sh_launch(const ShArgs &a, bool half, hipStream_t s) {
         if (half) hipLaunchKernelGGL((sh_records_scalar<L, true>), dim3((unsigned)a.items), dim3(256), 0, s, a);
         else hipLaunchKernelGGL((sh_records_scalar<L, false>), dim3((unsigned)a.items), dim3(256), 0, s, a);
     } else {
-        if (half) hipLaunchKernelGGL((sh_records_half_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 8), 0, s, a);
-        else hipLaunchKernelGGL((sh_records_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 4), 0, s, a);
+        if (half)
+            hipLaunchKernelGGL((sh_records_split_kernel<G, L>), dim3((unsigned)a.items, G / 16), dim3(256 * SplitK<L>::KP),
+                               0, s, a);
+        else hipLaunchKernelGGL((sh_records_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 16), 0, s, a);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error("sh_records launch: %s", hipGetErrorString(e));

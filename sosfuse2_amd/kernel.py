@@ -5,6 +5,7 @@ Device memory is PyTorch's (one process, one GPU); every launch runs on torch's 
 wavelengths runs on G = 1, 16 or 32 lanes per ray (W <= G), padded by repeating the last lane.
 """
 
+import ctypes
 import math
 
 import numpy as np
@@ -69,10 +70,12 @@ class DeviceBundle:
 
 
 class Plan:
-    """Visit plan of a bundle (fixed by its geometry): every quadrature node's cell, mu and cos(phi), the visits
-    (node, corner) sorted by their source cell, and work items of at most `chunk` visits of one cell."""
+    """Visit plan of a bundle (fixed by its geometry): every quadrature node's cell, mu, cos(phi) and direction; the
+    nodes sorted by their base cell with work items of at most `chunk` nodes of one base cell (SH records, four
+    corners per node); and the (node, corner) visits sorted by visited cell with items of at most `chunk` visits
+    (ground records of first light)."""
 
-    def __init__(self, kernel, dc, db, chunk=2048):
+    def __init__(self, kernel, dc, db, chunk=2048, node_chunk=1024):
         dev = kernel.device
         st = kernel.stream()
         counts = torch.empty(max(db.rays, 1), dtype=torch.int64, device=dev)
@@ -89,26 +92,42 @@ class Plan:
                  hip.ptr(self.node_mu), hip.ptr(self.node_cp), hip.ptr(self.node_dir), st)
         nc = dc.struct.nc
         cells = dc.struct.nh * nc
+        # nodes by base cell (SH records)
+        self.node = torch.argsort(self.node_cell[:N], stable=True).to(torch.int32)
+        self.sorted_mu = self.node_mu[self.node.long()]
+        self.sorted_cp = self.node_cp[self.node.long()]
+        self.sh_cell, self.sh_v0, self.sh_v1 = _items(self.node_cell[:N], cells, node_chunk, dev)
+        self.sh_items = len(self.sh_cell)
+        # visits by visited cell (ground records)
         offs = torch.tensor([0, 1, nc, nc + 1], dtype=torch.int32, device=dev)
         vcell = (self.node_cell[:N, None] + offs[None, :]).reshape(-1)
         self.vis = torch.argsort(vcell, stable=True).to(torch.int32)
-        node_of = (self.vis >> 2).long()
-        self.vis_mu = self.node_mu[node_of]
-        self.vis_cp = self.node_cp[node_of]
-        del node_of
-        per_cell = torch.bincount(vcell, minlength=cells)
+        self.item_cell, self.item_v0, self.item_v1 = _items(vcell, cells, chunk, dev)
         del vcell
-        cell_off = torch.zeros(cells + 1, dtype=torch.int64, device=dev)
-        cell_off[1:] = torch.cumsum(per_cell, 0)
-        nchunk = (per_cell + chunk - 1) // chunk
-        self.item_cell = torch.repeat_interleave(torch.arange(cells, device=dev, dtype=torch.int32), nchunk)
-        first_item = torch.cumsum(nchunk, 0) - nchunk
-        idx = torch.arange(len(self.item_cell), device=dev) - torch.repeat_interleave(first_item, nchunk)
-        self.item_v0 = cell_off[:-1][self.item_cell.long()] + idx * chunk
-        self.item_v1 = torch.minimum(self.item_v0 + chunk, cell_off[1:][self.item_cell.long()])
         self.items = len(self.item_cell)
         self.visits = 4 * N
         self.records = torch.empty((max(4 * N, 1), dc.G), dtype=torch.float32, device=dev)
+
+    def records_for(self, G):
+        """The records buffer [4 nodes, G] (reallocated when a chunk of another lane count uses the plan)."""
+        if self.records.shape[1] != G:
+            self.records = None
+            self.records = torch.empty((max(4 * self.nodes, 1), G), dtype=torch.float32, device=self.node_off.device)
+        return self.records
+
+
+def _items(key, cells, chunk, dev):
+    """Work items (cell, start, end) of at most `chunk` entries over the entries sorted by `key` (a cell index)."""
+    per_cell = torch.bincount(key, minlength=cells)
+    cell_off = torch.zeros(cells + 1, dtype=torch.int64, device=dev)
+    cell_off[1:] = torch.cumsum(per_cell, 0)
+    nchunk = (per_cell + chunk - 1) // chunk
+    item_cell = torch.repeat_interleave(torch.arange(cells, device=dev, dtype=torch.int32), nchunk)
+    first_item = torch.cumsum(nchunk, 0) - nchunk
+    idx = torch.arange(len(item_cell), device=dev) - torch.repeat_interleave(first_item, nchunk)
+    v0 = cell_off[:-1][item_cell.long()] + idx * chunk
+    v1 = torch.minimum(v0 + chunk, cell_off[1:][item_cell.long()])
+    return item_cell, v0, v1
 
 
 class DeviceConstants:
@@ -143,6 +162,16 @@ class DeviceConstants:
         self.factor = t(_pad_lanes(ctx.eigen(), G), 'f4')
         # one asymmetry per launch (a single-state chunk): first light may share phase evaluations across lanes
         self.uniform_phase = bool(np.all(lanes[6] == lanes[6][0]) and np.all(lanes[11] == lanes[11][0]))
+        # phase groups of the ground records: distinct (aerosol g, sulfate g) in lane order, None beyond four
+        keys = []
+        for w in range(G):
+            key = (lanes[6][w], lanes[11][w] if ctx.strato else np.float32(0))
+            if key not in keys:
+                keys.append(key)
+        self.phase_groups = None
+        if len(keys) <= 4:
+            self.phase_groups = (len(keys), (ctypes.c_float * 4)(*[k[0] for k in keys]),
+                                 (ctypes.c_float * 4)(*[k[1] for k in keys]))
         p = c['profile']
         g = ctx.geo
         self.struct = hip.Grid(
@@ -206,11 +235,13 @@ class Kernel:
 
     # ------------------------------------------------------------------------------------------------ launches
     def plane(self, ctx, field, dc):
-        """Combined SH plane of a source field (m-major per cell): (FP32 plane, None), or in mixed precision the FP16
-        plane and its per-(cell, lane) power-of-two scale."""
+        """Combined SH plane of a source field (m-major per cell): (FP32 plane, None), or in mixed precision the split
+        FP16 plane [2 (hi, lo), cells, K, G] and its per-(cell, lane) power-of-two scale."""
         K = self.K
-        dt = torch.float16 if self.mixed else torch.float32
-        out = torch.empty((ctx.geo.ns, K, dc.G), dtype=dt, device=self.device)
+        if self.mixed:
+            out = torch.empty((2, ctx.geo.ns, K, dc.G), dtype=torch.float16, device=self.device)
+        else:
+            out = torch.empty((ctx.geo.ns, K, dc.G), dtype=torch.float32, device=self.device)
         scale = torch.empty((ctx.geo.ns, dc.G), dtype=torch.float32, device=self.device) if self.mixed else None
         hip.call('sos_make_plane', dc.struct, ctx.rows, hip.ptr(field), hip.ptr(dc.ha), hip.ptr(dc.sd),
                  int(self.mixed), hip.ptr(out), hip.ptr(scale), self.stream())
@@ -236,16 +267,17 @@ class Kernel:
         records = None
         if first and dc.uniform_phase and self.uniform_phase:
             flags |= hip.UNIFORM_PHASE
-            if self.records and dc.G > 1:
-                pl = self.plan(db, dc)
-                e0 = self._event()
-                hip.call('sos_ground_records', dc.struct, pl.items, hip.ptr(pl.item_cell), hip.ptr(pl.item_v0),
-                         hip.ptr(pl.item_v1), hip.ptr(pl.vis), hip.ptr(pl.node_dir), hip.ptr(pl.records),
-                         self.stream())
-                if e0 is not None:
-                    self.timed.append(('ground_records', e0, self._event()))
-                records = pl.records
-                flags |= hip.GROUND_RECORDS
+        if first and self.records and dc.G > 1 and dc.phase_groups is not None:
+            pl = self.plan(db, dc)
+            ng, gg, sgg = dc.phase_groups
+            e0 = self._event()
+            records = pl.records_for(dc.G)
+            hip.call('sos_ground_records', dc.struct, ng, gg, sgg, pl.items, hip.ptr(pl.item_cell),
+                     hip.ptr(pl.item_v0), hip.ptr(pl.item_v1), hip.ptr(pl.vis), hip.ptr(pl.node_dir),
+                     hip.ptr(records), self.stream())
+            if e0 is not None:
+                self.timed.append(('ground_records', e0, self._event()))
+            flags |= hip.GROUND_RECORDS
         name = 'first' if first else ('warm' if boundary is not None else
                                       ('query' if pairs == 6 or db.groups != ctx.geo.nc else 'ground'))
         if not first:
@@ -254,10 +286,10 @@ class Kernel:
             if self.records:
                 pl = self.plan(db, dc)
                 e0 = self._event()
-                hip.call('sos_sh_records', dc.struct, hip.ptr(plane[0]), hip.ptr(plane[1]), int(self.mixed), pl.items,
-                         hip.ptr(pl.item_cell),
-                         hip.ptr(pl.item_v0), hip.ptr(pl.item_v1), hip.ptr(pl.vis), hip.ptr(pl.vis_mu),
-                         hip.ptr(pl.vis_cp), hip.ptr(pl.records), self.stream())
+                records = pl.records_for(dc.G)
+                hip.call('sos_sh_records', dc.struct, hip.ptr(plane[0]), hip.ptr(plane[1]), int(self.mixed),
+                         pl.sh_items, hip.ptr(pl.sh_cell), hip.ptr(pl.sh_v0), hip.ptr(pl.sh_v1), hip.ptr(pl.node),
+                         hip.ptr(pl.sorted_mu), hip.ptr(pl.sorted_cp), hip.ptr(records), self.stream())
                 if e0 is not None:
                     e1 = self._event()
                     self.timed.append(('sh_' + name, e0, e1))

@@ -309,3 +309,26 @@ def test_anderson_iteration_known_affine_fixed_point():
                for row in stats['history'][first_done + 1:])
     with pytest.raises(RuntimeError, match='E_NOT_CONVERGED'):
         solver(lambda x: x + forcing).solve(2, 0.005)
+
+
+@pytest.mark.parametrize('nmu,nphi,tol', [(4, 8, 1e-5), (8, 16, 1e-10)])
+def test_projection_of_isotropic_light(nmu, nphi, tol):
+    """Projection of a unit isotropic radiance (trace 1, tensor diag 1/3) through the incident quadrature: the solid
+    angle converges to 4 pi (the reference records the same sum as solid_angle_max_relative,
+    src/sosfuse2/solve/engine_core.py:247, for its identical sinh-graded quadrature), the (0, 0) moment to
+    4 pi Y00 = sqrt(4 pi), each diagonal tensor moment to 4 pi / 3, and the ground flux to
+    albedo / pi * integral of max(0, cos) = albedo. A wrong weight, a Y00 or trace slip, or the flux taken on the
+    wrong axis misses these by O(1)."""
+    ctx, scope = _cpu_context(nmu=nmu, nphi=nphi)
+    c = ctx.constants(ctx.geo.path_n, np.float64)
+    for cell in (0, 7, len(scope.packs) // 2, len(scope.packs) - 1):
+        g = scope.packs[cell]
+        rad = np.zeros((len(g['d']), c['W'], 4))
+        rad[:, :, :3] = 1.0 / 3.0
+        out, ground = ot.project(g, rad, c)
+        assert abs(g['weight'].sum() / (4 * math.pi) - 1) < tol
+        assert np.allclose(out[4], math.sqrt(4 * math.pi), rtol=tol, atol=0)
+        for k in range(3):
+            assert np.allclose(out[k], 4 * math.pi / 3, rtol=tol, atol=0)
+        assert np.allclose(out[3], 0.0, atol=1e-12)
+        assert np.allclose(ground, c['albedo'], rtol=max(tol, 1e-3), atol=0)
