@@ -56,6 +56,80 @@ def calibration_copy():
     del x, y
 
 
+def batched_variant(scope, n, el, az, tol, seed=2026):
+    """Tables/hour of the batched form: n states of the batched distribution (uneven wavelength and elevation
+    counts) packed into 32-lane chunks, solved and queried; the plain form's rate of the same states solved one by
+    one is estimated from its lanes (each state alone is one chunk)."""
+    import torch
+
+    from sosfuse2_amd import batch
+
+    items = batch.items_from_seed(n, seed, top_km=120.0)
+    chunks = batch.pack(items, 32)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    res = batch.solve(scope, items, el, az, tolerance=tol, max_eval=60, lanes=32, log=log)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t
+    return dict(states=n, seed=seed, chunks=len(chunks), lanes=sum(len(it.wavelengths) for it in items),
+                elevations=sum(len(it.elevations) for it in items), seconds=round(elapsed, 2),
+                tables_per_hour=round(3600.0 * n / elapsed, 2),
+                evaluations=[r[1]['evaluations'] for r in res])
+
+
+def learned_variant(scope, ntrain, ntest, el, az, tol):
+    """The learned warm start in the batched form: train the network on the solves of ntrain states, then solve
+    ntest held-out states cold and warm-started (both to the same tolerance) and compare evaluations and time."""
+    import numpy as np
+    import torch
+
+    from sosfuse2_amd import atmosphere, batch, learned
+
+    t0 = time.perf_counter()
+    train_items = batch.items_from_seed(ntrain, 7, top_km=120.0)
+    rng = np.random.default_rng(0)
+    Xs, Ts = [], []
+
+    for ch in batch.pack(train_items, 32):
+        optics = atmosphere.concat_optics([atmosphere.lane_optics(train_items[i].state, train_items[i].wavelengths)
+                                           for i in ch.members])
+        s = scope.context(optics)
+        sv = scope.solver(s, log=None)
+        x, info = sv.solve(60, tol)
+        X, T = learned.samples(s, sv, x, cells_per_lane=2048, rng=rng)
+        Xs.append(X)
+        Ts.append(T)
+        log(f'LEARN data chunk {ch.members}: {info["evaluations"]} evaluations')
+    X, T = torch.cat(Xs), torch.cat(Ts)
+    gen = time.perf_counter() - t0
+    net = learned.Network(learned.sizes_for(s.rows), seed=0)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    hist = learned.train(net, X, T, 1500, batch=8192, lr=1e-3, log=log)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t1
+    del X, T, Xs, Ts
+    test_items = batch.items_from_seed(ntest, 1234, top_km=120.0)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    cold = batch.solve(scope, test_items, el, az, tolerance=tol, max_eval=60, lanes=32)
+    torch.cuda.synchronize()
+    t_cold = time.perf_counter() - t
+    t = time.perf_counter()
+    warm = batch.solve(scope, test_items, el, az, tolerance=tol, max_eval=60, lanes=32,
+                       x0=lambda s, sv: learned.warm_start(net, s, sv))
+    torch.cuda.synchronize()
+    t_warm = time.perf_counter() - t
+    worst = max(float(np.max(np.abs(w[0][..., 0] - c[0][..., 0]) / np.abs(c[0][..., 0]))) for w, c in zip(warm, cold))
+    return dict(train_states=ntrain, test_states=ntest, data_seconds=round(gen, 1), train_seconds=round(train_s, 1),
+                loss_first=hist[0], loss_last=hist[-1],
+                cold=dict(seconds=round(t_cold, 2), tables_per_hour=round(3600.0 * ntest / t_cold, 2),
+                          evaluations=[r[1]['evaluations'] for r in cold]),
+                warm=dict(seconds=round(t_warm, 2), tables_per_hour=round(3600.0 * ntest / t_warm, 2),
+                          evaluations=[r[1]['evaluations'] for r in warm]),
+                max_I_difference_warm_vs_cold=worst)
+
+
 def cpu_baseline(s, scope, evaluations, seconds_budget=20.0):
     """Time the FP32 oracle (NumPy, one core) on a bounded sample of cells of the same workload and extrapolate to
     one table (evaluations x warm+ground transport and projection, plus first light)."""
@@ -102,7 +176,13 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--config', default='config1', choices=['config0', 'config1'])
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--mixed', action='store_true', help='FP16 SH plane (mixed precision variant)')
+    ap.add_argument('--mixed', action='store_true', help='headline in the mixed-precision form (split-FP16 SH)')
+    ap.add_argument('--variants', default='mixed,batched',
+                    help='forms measured beside the headline: mixed, batched (comma list, or "none")')
+    ap.add_argument('--batched-states', type=int, default=8, help='states of the batched sample')
+    ap.add_argument('--learned', type=str, default='',
+                    help='TRAIN,TEST: train the warm-start network on TRAIN batched states, compare TEST states '
+                         'warm-started against cold (adds minutes)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); '
@@ -192,16 +272,33 @@ def main():
         roofline=roofline,
         launch_seconds_per_step=launch_s,
     )
-    if args.mixed:
-        # the same table from the plain FP32 contraction (same kernel, plans and state): per-entry error of I, Q, U
-        k.mixed = False
-        info32, table32 = step()
-        k.mixed = True
-        I32 = np.abs(table32[..., 0])
-        d = np.abs(table - table32) / I32[..., None]
-        out['error_vs_fp32'] = dict(max_I=float(d[..., 0].max()), rms_I=float(np.sqrt(np.mean(d[..., 0] ** 2))),
-                                    max_QU=float(d[..., 1:].max()), evaluations_fp32=info32['evaluations'])
-        log(f'mixed vs FP32 table: {out["error_vs_fp32"]}')
+    variants = [v for v in args.variants.split(',') if v and v != 'none']
+    if args.mixed or 'mixed' in variants:
+        # the same state's table in the other form (same kernel, plans and state): per-entry error of I, Q, U of the
+        # mixed table against the FP32 one
+        k.mixed = not args.mixed
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        info_o, table_o = step()
+        torch.cuda.synchronize()
+        t_other = time.perf_counter() - t
+        k.mixed = args.mixed
+        t16, t32 = (table, table_o) if args.mixed else (table_o, table)
+        I32 = np.abs(t32[..., 0])
+        d = np.abs(t16 - t32) / I32[..., None]
+        err = dict(max_I=float(d[..., 0].max()), rms_I=float(np.sqrt(np.mean(d[..., 0] ** 2))),
+                   max_QU=float(d[..., 1:].max()))
+        if args.mixed:
+            out['error_vs_fp32'] = dict(err, evaluations_fp32=info_o['evaluations'])
+        else:
+            out['mixed'] = dict(tables_per_hour=round(3600.0 / t_other, 2), evaluations=info_o['evaluations'],
+                                error_vs_fp32=err, form='split-FP16 SH contraction (v_mfma_f32_16x16x16f16)')
+        log(f'mixed vs FP32 table: {err}')
+    if 'batched' in variants and args.batched_states > 0:
+        out['batched'] = batched_variant(scope, args.batched_states, el, az, tol)
+    if args.learned:
+        ntrain, ntest = (int(v) for v in args.learned.split(','))
+        out['learned'] = learned_variant(scope, ntrain, ntest, el, az, tol)
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(s, scope, int(np.mean(evals)))
     print(json.dumps(out), flush=True)

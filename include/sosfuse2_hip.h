@@ -71,11 +71,12 @@ typedef struct sos_bundle {
     const float *end_col_dev;     /* [rays][6] slant columns to the ground (5 components + sulfate) */
     const uint8_t *hits_dev;      /* [rays] 1 when the ray ends on the ground */
     const int64_t *node_off_dev;  /* [rays + 1] first quadrature node of every ray (records mode, from the plan) */
+    const float *node_geom_dev;   /* [nodes][32] static node geometry (SOS_NODE_GEOMETRY, from sos_plan_geometry) */
 } sos_bundle;
 
 typedef struct sos_grid {
     int32_t nh, nc;            /* heights, solar zenith nodes; cells = nh * nc */
-    int32_t L;                 /* SH order: 8, 16, 24 or 32 */
+    int32_t L;                 /* SH order: 8, 16 or 32 */
     int32_t lanes;             /* lanes per ray: 1, 16 or 32 (>= active wavelengths) */
     int32_t nq;                /* Gauss-Legendre nodes per segment */
     int32_t strato;            /* sulfate layer present */
@@ -94,6 +95,10 @@ typedef struct sos_grid {
     const float *gdir_dev;     /* [pools][3] cell-frame directions */
     const float *gval_dev;     /* [pools][lanes] */
     const float *gm_dev;       /* [cells][4][lanes] Rayleigh tensor of the ground-reflected sun */
+    int32_t blocks_per_cu;     /* transport launch: persistent workgroups per CU (0: 8); a tuning knob, results do
+                                  not depend on it */
+    int32_t sh_tile;           /* sos_sh_records FP32 workgroup shape: 0 all lanes of a node run per workgroup,
+                                  1 one 16-lane tile per workgroup; a tuning knob, results do not depend on it */
 } sos_grid;
 
 /* Radiance tensor of every ray: out_dev [rays][lanes][pairs] (pairs 4: xx yy zz xz; 6: + xy yz), FP32.
@@ -107,6 +112,8 @@ int sos_transport(const sos_grid *grid, const sos_bundle *bundle, int32_t flags,
                   float *out_dev, uint64_t *counters_dev, void *stream);
 /* first light with the ground term read from ground records [nodes][4][lanes] (sos_ground_records) */
 #define SOS_GROUND_RECORDS 32
+/* warm records consumer on the bundle's static node geometry (node_geom_dev; pairs 4, with SOS_RECORDS) */
+#define SOS_NODE_GEOMETRY 64
 
 /* Visit plan of a bundle (fixed by its geometry): counts_dev [rays] = quadrature nodes of every ray; then with
  * node_off_dev = exclusive prefix sum of the counts ([rays + 1]), every node's cell (ir * nc + j, its corner 0),
@@ -114,6 +121,12 @@ int sos_transport(const sos_grid *grid, const sos_bundle *bundle, int32_t flags,
 int sos_plan_count(const sos_grid *grid, const sos_bundle *bundle, int64_t *counts_dev, void *stream);
 int sos_plan_nodes(const sos_grid *grid, const sos_bundle *bundle, const int64_t *node_off_dev, int32_t *node_cell_dev,
                    float *node_mu_dev, float *node_cp_dev, float *node_dir_dev, void *stream);
+/* Static geometry of every node for the warm consumer, geom_dev [nodes][32] floats: the Simpson panel ending at the
+ * node and (last node of a segment) the segment's closing panel as weights of the six profiles (five components and
+ * the sulfate layer), ds, the bilinear weights, the base cell with a last-of-segment bit, and the tensor rotation
+ * (the per-node part of numpy_reference.py:294-351 and :236-291, computed once per bundle). */
+int sos_plan_geometry(const sos_grid *grid, const sos_bundle *bundle, const int64_t *node_off_dev, float *geom_dev,
+                      void *stream);
 
 /* First-light ground records: rec[id][w] = asca[w] p_aer(h) sum_p PHt gval[p][w] + ssca[w] p_sul(h) sum_p PHs gval[p][w]
  * over the ground pools of the visited cell (the pool sums of numpy_reference.py:526-558). The lanes fall into
@@ -149,9 +162,10 @@ int sos_compose(const sos_grid *grid, int32_t rows, const float *moments_dev, co
                 const float *factor_dev, float *source_dev, void *stream);
 
 /* plane[cells][K][lanes] = source[:, 10:10+K] * ha[ih] + source[:, 10+K:10+2K] * sd[ih], K in m-major order.
- * half = 0: FP32 [cells][K][lanes], scale_dev unused. half = 1: split FP16, plane_dev [2][cells][K][lanes] halves
- * holding hi = fp16(x) and lo = fp16((x - hi) 2^11) of x = c / scale, with scale_dev [cells][lanes] float the power
- * of two just above the column's largest magnitude (1 for an all-zero column). */
+ * half = 0: FP32 [cells][K][lanes], scale_dev unused. half = 1: split FP16, plane_dev [2][cells][KZ/4][lanes][4]
+ * halves (KZ = K rounded up to 16, zeros beyond K; four consecutive K positions of a lane together) holding
+ * hi = fp16(x) and lo = fp16((x - hi) 2^11) of x = c / scale, with scale_dev [cells][lanes] float the power of two
+ * just above the column's largest magnitude (1 for an all-zero column). */
 int sos_make_plane(const sos_grid *grid, int32_t rows, const float *source_dev, const float *ha_dev,
                    const float *sd_dev, int32_t half, void *plane_dev, float *scale_dev, void *stream);
 
@@ -182,6 +196,28 @@ int sos_anderson_solve(int32_t lanes, int32_t depth, int32_t m, const int32_t *o
  * other lanes are left as they are. g_ring: [depth][cells][rows][lanes]. */
 int sos_anderson_mix(int64_t cells, int32_t rows, int32_t lanes, int32_t depth, int32_t m, const float *ring_dev,
                      const int32_t *order_dev, const float *w_dev, float *x_dev, void *stream);
+
+/* Learned warm start: no counterpart in the reference (the network only seeds Solver.solve's iteration, which still
+ * runs to its tolerance). All arrays FP32 in device memory; products round their operands to BF16 and accumulate
+ * in FP32.
+ * sos_mlp_gemm: Z[i][j] (row stride ldz) = sum_k A(i, k) B(k, j) (+ bias[j]) with A(i, k) = a[i sa_i + k sa_k],
+ * B(k, j) = b[k sb_k + j sb_j]; act 1 also writes H = gelu(Z) (exact erf form). splits > 1 divides K over
+ * workgroups (work_dev: splits * M * N floats) and sums the parts in a fixed order. */
+int sos_mlp_gemm(int64_t M, int64_t N, int64_t K, const float *a_dev, int64_t sa_i, int64_t sa_k, const float *b_dev,
+                 int64_t sb_k, int64_t sb_j, const float *bias_dev, int32_t act, float *z_dev, int64_t ldz,
+                 float *h_dev, int64_t ldh, int32_t splits, float *work_dev, void *stream);
+/* dz = dh * gelu'(z), elementwise over rows x cols with row strides ldd, ldz, ldo (dz may be dh). */
+int sos_mlp_gelu_backward(int64_t rows, int64_t cols, const float *dh_dev, int64_t ldd, const float *z_dev,
+                          int64_t ldz, float *dz_dev, int64_t ldo, void *stream);
+/* loss_dev[0] = mean over rows x cols of w[j] (y - t)^2 (w_dev may be NULL = 1); dy_dev (may be NULL) = its
+ * gradient 2 w (y - t) / (rows cols). work_dev: 1024 doubles. */
+int sos_mlp_mse(int64_t rows, int64_t cols, const float *y_dev, int64_t ldy, const float *t_dev, int64_t ldt,
+                const float *w_dev, float *dy_dev, int64_t lddy, float *loss_dev, double *work_dev, void *stream);
+/* db[j] = sum_i dz[i][j] (fixed order). */
+int sos_mlp_bias_grad(int64_t rows, int64_t cols, const float *dz_dev, int64_t ld, float *db_dev, void *stream);
+/* One Adam step (bias-corrected, step counts from 1) on n parameters in place. */
+int sos_mlp_adam(int64_t n, float *p_dev, const float *g_dev, float *m_dev, float *v_dev, int32_t step, float lr,
+                 float beta1, float beta2, float eps, void *stream);
 
 const char *sos_last_error(void);
 int sos_device_info(int32_t *cus, int32_t *xcds, char *arch, int32_t arch_len);

@@ -70,10 +70,10 @@ class Scope:
         return engine.Context(self.grid, optics, self.kernel, self.sun, self.pools, self.cfg.get('observer_km', 0.1),
                               self.state)
 
-    def solver(self, s, log=print):
+    def solver(self, s, log=print, graph=False):
         if self.geometry is None:
             self.geometry = Geometry(s, self.packs, self.kernel)
-        return Solver(s, self.geometry, depth=int(self.cfg.get('anderson_depth', 4)), log=log)
+        return Solver(s, self.geometry, depth=int(self.cfg.get('anderson_depth', 4)), log=log, graph=graph)
 
 
 def create_configured(output, cfg, wavelengths, backend, state, block_size=256, geometry_cache=None, *,

@@ -30,6 +30,7 @@ static int arg_error(const char *fmt, ...) {
 int transport_dispatch(const TransportArgs &a, int G, int L, hipStream_t s);
 int project_dispatch(const ProjectArgs &a, int G, int L, hipStream_t s);
 int plan_count_launch(const TransportArgs &a, int64_t *counts, hipStream_t s);
+int plan_geometry_launch(const TransportArgs &a, const int64_t *node_off, float *geom, hipStream_t s);
 int plan_nodes_launch(const TransportArgs &a, const int64_t *node_off, int32_t *cell, float *mu, float *cp,
                       float *dir, hipStream_t s);
 int ground_records_dispatch(const GroundArgs &a, hipStream_t s);
@@ -52,6 +53,14 @@ int mix_launch(int64_t cells, int rows, int G, int depth, int m, const float *ri
 static bool lanes_ok(int G) { return G == 1 || G == 16 || G == 32; }
 static bool order_ok(int L) { return L == 8 || L == 16 || L == 32; }
 
+int mlp_gemm_launch(const GemmArgs &in, float *work, hipStream_t s);
+int mlp_gelu_backward_launch(int64_t rows, int64_t cols, const float *dh, int64_t ldd, const float *z, int64_t ldz,
+                             float *dz, int64_t ldo, hipStream_t s);
+int mlp_mse_launch(int64_t rows, int64_t cols, const float *y, int64_t ldy, const float *t, int64_t ldt,
+                   const float *w, float *dy, int64_t lddy, float *loss, double *work, hipStream_t s);
+int mlp_bias_grad_launch(int64_t rows, int64_t cols, const float *dz, int64_t ld, float *db, hipStream_t s);
+int mlp_adam_launch(int64_t n, float *p, const float *g, float *m, float *v, int step, float lr, float b1, float b2,
+                    float eps, hipStream_t s);
 }  // namespace sos
 
 using namespace sos;
@@ -101,6 +110,17 @@ int sos_plan_nodes(const sos_grid *g, const sos_bundle *b, const int64_t *node_o
                              (hipStream_t)stream);
 }
 
+int sos_plan_geometry(const sos_grid *g, const sos_bundle *b, const int64_t *node_off_dev, float *geom_dev,
+                      void *stream) {
+    if (!g || !b || !node_off_dev || !geom_dev) return arg_error("sos_plan_geometry: null argument");
+    TransportArgs a;
+    fill_args(a, g, b);
+    a.shape = ProfileShape{1.0f / g->air_H, 1.0f / g->aer_H, g->o3_c, 1.0f / g->o3_s,
+                           g->no2_c, 1.0f / g->no2_s, g->st_c, 1.0f / g->st_s};
+    a.targets = b->targets_dev; a.dloc = b->dloc_dev; a.nc = g->nc; a.nh = g->nh;
+    return plan_geometry_launch(a, node_off_dev, geom_dev, (hipStream_t)stream);
+}
+
 int sos_ground_records(const sos_grid *g, int32_t groups, const float *group_g, const float *group_sg, int64_t items,
                        const int32_t *item_cell_dev, const int64_t *item_v0_dev, const int64_t *item_v1_dev,
                        const int32_t *vis_dev, const float *node_dir_dev, float *records_dev, void *stream) {
@@ -139,7 +159,8 @@ int sos_sh_records(const sos_grid *g, const void *plane_dev, const float *scale_
     a.L = g->L; a.G = g->lanes; a.K = (g->L + 1) * (g->L + 2) / 2; a.nc = g->nc; a.plane = plane_dev; a.items = items;
     a.item_cell = item_cell_dev; a.item_v0 = item_v0_dev; a.item_v1 = item_v1_dev; a.node = node_dev;
     a.node_mu = node_mu_dev; a.node_cp = node_cp_dev; a.records = records_dev; a.scale = scale_dev;
-    a.plane_lo = (int64_t)g->nh * g->nc * a.K * a.G;
+    a.plane_lo = (int64_t)g->nh * g->nc * ((a.K + 15) / 16 * 16) * a.G;
+    a.tile = g->sh_tile != 0;
     return sh_records_dispatch(a, half != 0, (hipStream_t)stream);
 }
 
@@ -166,6 +187,7 @@ int sos_transport(const sos_grid *g, const sos_bundle *b, int32_t flags, int32_t
     TransportArgs a;
     memset(&a, 0, sizeof(a));
     a.nh = g->nh; a.nc = g->nc; a.nq = g->nq; a.strato = g->strato; a.flags = flags; a.pairs = pairs; a.rows = rows;
+    a.blocks_per_cu = g->blocks_per_cu;
     a.sun_nh = g->sun_nh; a.sun_nx = g->sun_nx; a.top = g->top_km;
     a.shape = ProfileShape{1.0f / g->air_H, 1.0f / g->aer_H, g->o3_c, 1.0f / g->o3_s,
                            g->no2_c, 1.0f / g->no2_s, g->st_c, 1.0f / g->st_s};
@@ -184,6 +206,11 @@ int sos_transport(const sos_grid *g, const sos_bundle *b, int32_t flags, int32_t
     a.ground_records = ((flags & SOS_FIRST) && (flags & SOS_GROUND_RECORDS)) ? records_dev : nullptr;
     a.node_off = b->node_off_dev;
     if (a.ground_records && !a.node_off) return arg_error("sos_transport: ground records need the node offsets");
+    if (flags & SOS_NODE_GEOMETRY) {
+        if (first || !records || pairs != 4 || !b->node_geom_dev)
+            return arg_error("sos_transport: node geometry needs the warm records path, pairs 4 and the geometry");
+        a.node_geom = reinterpret_cast<const float4 *>(b->node_geom_dev);
+    }
     return transport_dispatch(a, g->lanes, g->L, (hipStream_t)stream);
 }
 
@@ -257,3 +284,40 @@ int sos_anderson_mix(int64_t cells, int32_t rows, int32_t lanes, int32_t depth, 
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------------------------- learned warm start
+int sos_mlp_gemm(int64_t M, int64_t N, int64_t K, const float *a_dev, int64_t sa_i, int64_t sa_k, const float *b_dev,
+                 int64_t sb_k, int64_t sb_j, const float *bias_dev, int32_t act, float *z_dev, int64_t ldz,
+                 float *h_dev, int64_t ldh, int32_t splits, float *work_dev, void *stream) {
+    if (M < 0 || N < 0 || K < 1 || !a_dev || !b_dev || !z_dev || ldz < N || (act == 1 && (!h_dev || ldh < N)) ||
+        act < 0 || act > 1 || splits < 1 || splits > 256 || (splits > 1 && !work_dev))
+        return arg_error("sos_mlp_gemm: bad argument");
+    GemmArgs a;
+    a.M = M; a.N = N; a.K = K; a.A = a_dev; a.sa_i = sa_i; a.sa_k = sa_k; a.B = b_dev; a.sb_k = sb_k; a.sb_j = sb_j;
+    a.bias = bias_dev; a.act = act; a.Z = z_dev; a.ldz = ldz; a.H = h_dev; a.ldh = ldh; a.splits = splits; a.kchunk = K;
+    return mlp_gemm_launch(a, work_dev, (hipStream_t)stream);
+}
+
+int sos_mlp_gelu_backward(int64_t rows, int64_t cols, const float *dh_dev, int64_t ldd, const float *z_dev,
+                          int64_t ldz, float *dz_dev, int64_t ldo, void *stream) {
+    if (rows < 0 || cols < 0 || !dh_dev || !z_dev || !dz_dev) return arg_error("sos_mlp_gelu_backward: bad argument");
+    return mlp_gelu_backward_launch(rows, cols, dh_dev, ldd, z_dev, ldz, dz_dev, ldo, (hipStream_t)stream);
+}
+
+int sos_mlp_mse(int64_t rows, int64_t cols, const float *y_dev, int64_t ldy, const float *t_dev, int64_t ldt,
+                const float *w_dev, float *dy_dev, int64_t lddy, float *loss_dev, double *work_dev, void *stream) {
+    if (rows < 1 || cols < 1 || !y_dev || !t_dev || !loss_dev || !work_dev) return arg_error("sos_mlp_mse: bad argument");
+    return mlp_mse_launch(rows, cols, y_dev, ldy, t_dev, ldt, w_dev, dy_dev, lddy, loss_dev, work_dev,
+                          (hipStream_t)stream);
+}
+
+int sos_mlp_bias_grad(int64_t rows, int64_t cols, const float *dz_dev, int64_t ld, float *db_dev, void *stream) {
+    if (rows < 0 || cols < 0 || !dz_dev || !db_dev) return arg_error("sos_mlp_bias_grad: bad argument");
+    return mlp_bias_grad_launch(rows, cols, dz_dev, ld, db_dev, (hipStream_t)stream);
+}
+
+int sos_mlp_adam(int64_t n, float *p_dev, const float *g_dev, float *m_dev, float *v_dev, int32_t step, float lr,
+                 float beta1, float beta2, float eps, void *stream) {
+    if (n < 0 || !p_dev || !g_dev || !m_dev || !v_dev || step < 1) return arg_error("sos_mlp_adam: bad argument");
+    return mlp_adam_launch(n, p_dev, g_dev, m_dev, v_dev, step, lr, beta1, beta2, eps, (hipStream_t)stream);
+}

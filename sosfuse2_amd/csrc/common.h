@@ -61,6 +61,7 @@ __device__ inline void profiles(const ProfileShape &s, float h, float *p, float 
 struct TransportArgs {
     // grid
     int nh, nc, nq, strato, flags, pairs, rows, sun_nh, sun_nx;
+    int blocks_per_cu;  // persistent workgroups per CU (0: 8)
     float top;
     ProfileShape shape;
     const double *gl_u, *gl_w;
@@ -90,6 +91,7 @@ struct TransportArgs {
     const float *records;    // [nodes][4][G] SH values per (node, corner) (records mode)
     const int64_t *node_off;  // [rays + 1] first node of every ray (records mode)
     const float *ground_records;  // [nodes][4][G] scattered ground light per (node, corner) (first light)
+    const float4 *node_geom;  // [nodes][8] static node geometry (SOS_NODE_GEOMETRY)
 };
 
 struct ProjectArgs {
@@ -103,7 +105,7 @@ struct ProjectArgs {
 
 struct ShArgs {
     int L, G, K, nc;
-    const void *plane;         // [cells][K][G] float, or [2][cells][K][G] half (split hi, lo)
+    const void *plane;         // [cells][K][G] float, or [2][cells][KZ/4][G][4] half (split hi, lo)
     int64_t items;
     const int32_t *item_cell;  // [items] base cell (ir * nc + j) of the item's nodes
     const int64_t *item_v0, *item_v1;  // node range of the item in the sorted order
@@ -111,7 +113,8 @@ struct ShArgs {
     const float *node_mu, *node_cp;  // [nodes] the ray's mu and the node's cos(phi), in the same order
     float *records;            // [nodes][4][G]: corner c of node n at (4 n + c) G
     const float *scale;        // [cells][G] scale of a split FP16 plane (mixed precision)
-    int64_t plane_lo;          // offset (elements) of the lo plane after the hi plane: cells * K * G
+    int64_t plane_lo;          // offset (elements) of the lo plane after the hi plane: cells * KZ * G
+    int tile;                  // FP32 workgroup shape: one 16-lane tile per workgroup (1) or all lanes (0)
 };
 
 struct GroundArgs {
@@ -128,6 +131,22 @@ struct GroundArgs {
     float *records;            // [nodes * 4][G]
     int groups;                // phase groups: distinct (aerosol g, sulfate g) among the lanes, at most 4
     float gg[4], sgg[4];
+};
+
+struct GemmArgs {
+    int64_t M, N, K;
+    const float *A;
+    int64_t sa_i, sa_k;
+    const float *B;
+    int64_t sb_k, sb_j;
+    const float *bias;  // [N] or null
+    int act;            // 0: none, 1: GELU
+    float *Z;           // [M][ldz] pre-activation (or the product), or the split partials
+    int64_t ldz;
+    float *H;           // [M][ldh] activation (act 1) or null
+    int64_t ldh;
+    int splits;
+    int64_t kchunk;     // K range of one split (multiple of 32)
 };
 
 }  // namespace sos

@@ -73,8 +73,10 @@ __global__ void plane_kernel(int nh, int nc, int L, int rows, int G, int strato,
 
 // Split FP16 plane for the mixed-precision contraction: one workgroup per cell. scale[cell][w] is the power of two
 // just above max_j |c[cell][j][w]| (1 for an all-zero column); x = c / scale lies in [-1, 1] and is stored as
-// hi = fp16(x) in the first plane and lo = fp16((x - hi) 2^11) in the second (cells x K x G further on), so
-// hi + lo 2^-11 carries x to 2^-23 and neither part leaves FP16's range.
+// hi = fp16(x) in the first plane and lo = fp16((x - hi) 2^11) in the second (cells x KZ x G further on), so
+// hi + lo 2^-11 carries x to 2^-23 and neither part leaves FP16's range. Layout of each plane: quads of four
+// consecutive K positions per lane, [cell][KZ / 4][G][4] with KZ = K rounded up to 16 and zeros beyond K, so that
+// an MFMA B operand (four K positions of one lane) is one 8-byte load.
 __global__ __launch_bounds__(256) void plane_half_kernel(int nc, int L, int rows, int G, int strato,
                                                          const float *__restrict__ src, const float *__restrict__ ha,
                                                          const float *__restrict__ sd, _Float16 *__restrict__ plane,
@@ -84,6 +86,7 @@ __global__ __launch_bounds__(256) void plane_half_kernel(int nc, int L, int rows
     __shared__ float colmax[256];
     __shared__ float inv[32];
     const int K = (L + 1) * (L + 2) / 2;
+    const int KZ = (K + 15) / 16 * 16;
     const int64_t cell = blockIdx.x;
     const int ih = (int)(cell / nc);
     const int tid = threadIdx.x, w = tid % G, q = tid / G, nq = 256 / G;
@@ -111,13 +114,14 @@ __global__ __launch_bounds__(256) void plane_half_kernel(int nc, int L, int rows
     }
     __syncthreads();
     const float iv = inv[w];
-    _Float16 *hi = plane + cell * K * G + w;
-    _Float16 *lo = hi + (int64_t)gridDim.x * K * G;
-    for (int j = q; j < K; j += nq) {
-        const float x = value(j) * iv;
-        const _Float16 h = (_Float16)x;
-        hi[(int64_t)j * G] = h;
-        lo[(int64_t)j * G] = (_Float16)((x - (float)h) * 2048.0f);
+    _Float16 *hi = plane + cell * KZ * G;
+    _Float16 *lo = hi + (int64_t)gridDim.x * KZ * G;
+    for (int j = q; j < KZ; j += nq) {
+        const float x = j < K ? value(j) * iv : 0.0f;
+        const _Float16 hv = (_Float16)x;
+        const int64_t o = ((int64_t)(j >> 2) * G + w) * 4 + (j & 3);
+        hi[o] = hv;
+        lo[o] = (_Float16)((x - (float)hv) * 2048.0f);
     }
 }
 

@@ -10,7 +10,7 @@
 // four corners. The plan kernels walk the same node geometry as the transport.
 //
 // K order (the plane layout): m-major, k' = moff(m) + (l - m). FP32 MFMA k-steps are permuted so that one
-// ds_read_b128 feeds four consecutive MFMAs: step s covers k' = 16 (s/4) + 4 (lane/16) + (s%4).
+// ds_read_b128 feeds four consecutive MFMAs: step 4 i + u covers k' = 16 i + 4 (lane/16) + u.
 
 #include "common.h"
 
@@ -181,7 +181,7 @@ __device__ inline void build_z_split(_Float16 (*Zh)[ZS], _Float16 (*Zl)[ZS], con
             split_store(&Zh[v][ob + 1], &Zl[v][ob + 1], b1 * cb);
         }
         int la = ma + 2, lb = mb + 2;
-#pragma unroll 2
+#pragma unroll 1
         for (; two && lb <= L; ++la, ++lb) {
             const int ka = kidx(la, ma), kb = kidx(lb, mb);
             const float a2 = (aa[ka] * mu) * a1 - bb[ka] * a0;
@@ -191,7 +191,7 @@ __device__ inline void build_z_split(_Float16 (*Zh)[ZS], _Float16 (*Zl)[ZS], con
             a0 = a1; a1 = a2;
             b0 = b1; b1 = b2;
         }
-#pragma unroll 4
+#pragma unroll 1
         for (; la <= L; ++la) {
             const int ka = kidx(la, ma);
             const float a2 = (aa[ka] * mu) * a1 - bb[ka] * a0;
@@ -204,14 +204,31 @@ __device__ inline void build_z_split(_Float16 (*Zh)[ZS], _Float16 (*Zl)[ZS], con
 // Corner c of a node with base cell b (ir * nc + j): b, b + 1, b + nc, b + nc + 1, the transport's corner order.
 __device__ inline int corner_cell(int b, int c, int nc) { return b + (c & 1) + (c >> 1) * nc; }
 
-// FP32: 4 x G/16 wavefronts, wavefront (c, nt) holds corner c's coefficients of lanes 16 nt .. 16 nt + 15.
-template <int G, int L>
-__global__ __launch_bounds__(G * 16) __attribute__((amdgpu_waves_per_eu(2, 2))) void sh_records_kernel(ShArgs a) {
+// FP32, two workgroup shapes (a tuning choice; the records do not depend on it):
+//   TILE = false: one workgroup per work item, 4 x G/16 wavefronts, wavefront (c, nt) holds corner c's
+//                 coefficients of lanes 16 nt .. 16 nt + 15 (whole K, about 250 VGPRs, one workgroup per CU);
+//   TILE = true:  one workgroup per (work item, 16-lane tile), the K range split over KP wavefronts per corner
+//                 (KP = 2 at L = 32, at most 128 VGPRs) so that two workgroups share a CU and one's basis generation
+//                 overlaps the other's MFMA phase; the basis rows are then generated once per tile.
+template <int G, int L, bool TILE>
+struct ShShape {
+    static constexpr int KP = (TILE && L >= 32) ? 2 : 1;
+    static constexpr int THREADS = TILE ? 256 * KP : G * 16;
+    static constexpr int WAVES_PER_EU = TILE ? 4 : 2;
+};
+
+template <int G, int L, bool TILE>
+__global__ __launch_bounds__((ShShape<G, L, TILE>::THREADS))
+__attribute__((amdgpu_waves_per_eu((ShShape<G, L, TILE>::WAVES_PER_EU), (ShShape<G, L, TILE>::WAVES_PER_EU))))
+void sh_records_kernel(ShArgs a) {
     constexpr int K = (L + 1) * (L + 2) / 2;
     constexpr int KZ = (K + 15) / 16 * 16;   // padded row
-    constexpr int KS = KZ / 4;               // MFMA k-steps
-    constexpr int NW = G / 16;
-    constexpr int NT = 64 * 4 * NW;
+    constexpr int KQ = KZ / 16;              // groups of four MFMA k-steps (one ds_read_b128 each)
+    constexpr int KP = ShShape<G, L, TILE>::KP;
+    constexpr int KQH = (KQ + KP - 1) / KP;  // groups of one wavefront
+    constexpr int NWB = TILE ? 1 : G / 16;   // 16-lane tiles per workgroup
+    constexpr int NC = 4 * NWB;              // (corner, tile) wavefronts per K part
+    constexpr int NT = 64 * NC * KP;
     constexpr int V = 16;                    // nodes per round (one M-tile)
     constexpr int ZS = KZ + 4;               // row stride = 4 (mod 32 banks): 16 rows spread over the LDS banks
     __shared__ float Z[V][ZS];
@@ -221,8 +238,10 @@ __global__ __launch_bounds__(G * 16) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     __shared__ double smu[V];
     __shared__ double dd[L + 1];
     __shared__ int sid[V];
+    __shared__ floatx4 red[KP > 1 ? NC * 64 : 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int corner = wave / NW, nt = wave % NW;
+    const int wc = wave % NC, kh = wave / NC;
+    const int corner = wc / NWB, nt = TILE ? (int)blockIdx.y : wc % NWB;
     const int col = nt * 16 + (lane & 15);
     const int64_t item = blockIdx.x;
     const int cell = corner_cell(a.item_cell[item], corner, a.nc);
@@ -230,15 +249,17 @@ __global__ __launch_bounds__(G * 16) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     stage_constants<L>(aa, bb, NT);
     for (int m = tid; m <= L; m += NT) dd[m] = m ? -sqrt((2.0 * m + 1) / (2.0 * m)) : 0.0;
     for (int e = tid; e < V * (KZ - K); e += NT) Z[e / (KZ - K)][K + e % (KZ - K)] = 0.0f;
-    // B operand in registers: b[s] = c[cell][k'(s, lane)][col]
-    float b[KS];
+    // B operand in registers: b[4 i + u] = c[cell][k][col] at k = 16 (kh KQH + i) + 4 (lane / 16) + u
+    float b[4 * KQH];
     {
         const float *pl = reinterpret_cast<const float *>(a.plane) + (int64_t)cell * K * G + col;
 #pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            const int k = 16 * (s >> 2) + 4 * (lane >> 4) + (s & 3);
-            b[s] = k < K ? pl[(int64_t)k * G] : 0.0f;
-        }
+        for (int i = 0; i < KQH; ++i)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = 16 * (kh * KQH + i) + 4 * (lane >> 4) + u;
+                b[4 * i + u] = k < K ? pl[(int64_t)k * G] : 0.0f;
+            }
     }
     // node data of the next round is loaded one round ahead (contiguous in the sorted order)
     int nid = -1;
@@ -271,40 +292,50 @@ __global__ __launch_bounds__(G * 16) __attribute__((amdgpu_waves_per_eu(2, 2))) 
         floatx4 acc[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
-        const float *zrow = &Z[lane & 15][4 * (lane >> 4)];
+        const float *zrow = &Z[lane & 15][4 * (lane >> 4) + 16 * kh * KQH];
 #pragma unroll
-        for (int q = 0; q < KS / 4; ++q) {
-            const floatx4 av = *reinterpret_cast<const floatx4 *>(zrow + 16 * q);
+        for (int i = 0; i < KQH; ++i) {
+            if (kh * KQH + i < KQ) {
+                const floatx4 av = *reinterpret_cast<const floatx4 *>(zrow + 16 * i);
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], b[4 * q + u], acc[u], 0, 0, 0);
+                for (int u = 0; u < 4; ++u)
+                    acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], b[4 * i + u], acc[u], 0, 0, 0);
+                if constexpr (TILE) __builtin_amdgcn_sched_barrier(0);
+            }
         }
-        const floatx4 d = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        floatx4 d = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        if constexpr (KP > 1) {
+            if (kh == 1) red[wc * 64 + lane] = d;
+            __syncthreads();
+            if (kh == 0) d = d + red[wc * 64 + lane];
+        }
+        if (kh == 0) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int row = 4 * (lane >> 4) + v;
-            if (row < nv) a.records[((int64_t)sid[row] * 4 + corner) * G + col] = d[v];
+            for (int v = 0; v < 4; ++v) {
+                const int row = 4 * (lane >> 4) + v;
+                if (row < nv) a.records[((int64_t)sid[row] * 4 + corner) * G + col] = d[v];
+            }
         }
         __syncthreads();
     }
 }
 
-// The split kernel takes one 16-lane N-tile per workgroup (blockIdx.y) and divides the K range over KP wavefronts
-// per corner (KP = 2 at L = 32), so that each keeps only its part of the hi / lo B operands in registers (about 170
-// VGPRs, two wavefronts per SIMD); the parts are summed through LDS after each round. The basis rows are generated
-// once per N-tile.
+// The split kernel divides the K range over KP wavefronts per (corner, N-tile) (KP = 2 at L = 32), so that each
+// keeps only its part of the hi / lo B operands in registers (4 x G/16 x KP wavefronts, at most 128 VGPRs each);
+// the parts are summed through LDS after each round. The basis rows are generated once per node.
 template <int L>
 struct SplitK { static constexpr int KP = L >= 32 ? 2 : 1; };
 
 template <int G, int L>
-__global__ __launch_bounds__(256 * SplitK<L>::KP) __attribute__((amdgpu_waves_per_eu(2, 2)))
-void sh_records_split_kernel(ShArgs a) {
+__global__ __launch_bounds__(G * 16 * SplitK<L>::KP) void sh_records_split_kernel(ShArgs a) {
     constexpr int K = (L + 1) * (L + 2) / 2;
     constexpr int KZ = (K + 15) / 16 * 16;
     constexpr int KS = KZ / 16;               // MFMA k-steps of 16
     constexpr int KP = SplitK<L>::KP;
     constexpr int KSH = (KS + KP - 1) / KP;   // k-steps of one wavefront
-    constexpr int NT = 256 * KP;              // 4 corners x KP parts
+    constexpr int NW = G / 16;                // N-tiles
+    constexpr int NC = 4 * NW;                // (corner, N-tile) wavefronts per K part
+    constexpr int NT = 64 * NC * KP;
     constexpr int V = 16;                     // nodes per round (one M-tile)
     constexpr int ZS = KZ + 8;                // halves: row stride = 4 (mod 32 banks) in 32-bit words
     __shared__ _Float16 Zh[V][ZS], Zl[V][ZS];
@@ -314,10 +345,11 @@ void sh_records_split_kernel(ShArgs a) {
     __shared__ float smu[V], scp[V];
     __shared__ float dd[L + 1];
     __shared__ int sid[V];
-    __shared__ floatx4 red[KP > 1 ? 4 * 64 : 1];
+    __shared__ floatx4 red[KP > 1 ? NC * 64 : 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int corner = wave & 3, kh = wave >> 2;
-    const int col = blockIdx.y * 16 + (lane & 15);
+    const int wc = wave % NC, kh = wave / NC;
+    const int corner = wc / NW, nt = wc % NW;
+    const int col = nt * 16 + (lane & 15);
     const int64_t item = blockIdx.x;
     const int cell = corner_cell(a.item_cell[item], corner, a.nc);
     const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
@@ -337,19 +369,18 @@ void sh_records_split_kernel(ShArgs a) {
         Zl[e / (KZ - K)][K + e % (KZ - K)] = (_Float16)0.0f;
     }
     const float scale = a.scale[(int64_t)cell * G + col];
-    // B operands: bh[i], bl[i] hold the hi / lo plane at k = 16 (kh KSH + i) + 4 (lane / 16) + 0..3, column col
+    // B operands from the quad layout [cell][KZ / 4][G] (half4): bh[i], bl[i] hold K positions
+    // 16 (kh KSH + i) + 4 (lane / 16) + 0..3 of column col
     half4_t bh[KSH], bl[KSH];
     {
-        const _Float16 *ph = reinterpret_cast<const _Float16 *>(a.plane) + (int64_t)cell * K * G + col;
-        const _Float16 *pl = ph + a.plane_lo;
+        const half4_t *ph = reinterpret_cast<const half4_t *>(a.plane) + (int64_t)cell * (KZ / 4) * G + col;
+        const half4_t *pl = ph + a.plane_lo / 4;
 #pragma unroll
-        for (int i = 0; i < KSH; ++i)
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int k = 16 * (kh * KSH + i) + 4 * (lane >> 4) + u;
-                bh[i][u] = k < K ? ph[(int64_t)k * G] : (_Float16)0.0f;
-                bl[i][u] = k < K ? pl[(int64_t)k * G] : (_Float16)0.0f;
-            }
+        for (int i = 0; i < KSH; ++i) {
+            const int q = 4 * (kh * KSH + i) + (lane >> 4);
+            bh[i] = q < KZ / 4 ? ph[(int64_t)q * G] : half4_t{0, 0, 0, 0};
+            bl[i] = q < KZ / 4 ? pl[(int64_t)q * G] : half4_t{0, 0, 0, 0};
+        }
     }
     int nid = -1;
     float nmu = 0.0f, ncp = 1.0f;
@@ -380,7 +411,7 @@ void sh_records_split_kernel(ShArgs a) {
         __syncthreads();
         build_z_split<L, V, ZS>(Zh, Zl, smu, seed, cmv, aa, bb, nv, NT);
         __syncthreads();
-        floatx4 am = floatx4{0.f, 0.f, 0.f, 0.f}, c1 = am, c2 = am;
+        floatx4 am = floatx4{0.f, 0.f, 0.f, 0.f}, ac = am;
         const _Float16 *hrow = &Zh[lane & 15][4 * (lane >> 4) + 16 * kh * KSH];
         const _Float16 *lrow = &Zl[lane & 15][4 * (lane >> 4) + 16 * kh * KSH];
 #pragma unroll
@@ -389,15 +420,16 @@ void sh_records_split_kernel(ShArgs a) {
                 const half4_t ah = *reinterpret_cast<const half4_t *>(hrow + 16 * i);
                 const half4_t al = *reinterpret_cast<const half4_t *>(lrow + 16 * i);
                 am = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh[i], am, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl[i], c1, 0, 0, 0);
-                c2 = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh[i], c2, 0, 0, 0);
+                ac = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl[i], ac, 0, 0, 0);
+                ac = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh[i], ac, 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);  // one step at a time: keeps the LDS operands out of registers
             }
         }
-        floatx4 d = am + (c1 + c2) * (1.0f / 2048.0f);
+        floatx4 d = am + ac * (1.0f / 2048.0f);
         if constexpr (KP > 1) {
-            if (kh == 1) red[corner * 64 + lane] = d;
+            if (kh == 1) red[wc * 64 + lane] = d;
             __syncthreads();
-            if (kh == 0) d = d + red[corner * 64 + lane];
+            if (kh == 0) d = d + red[wc * 64 + lane];
         }
         if (kh == 0) {
 #pragma unroll
@@ -422,9 +454,11 @@ __global__ __launch_bounds__(256) void sh_records_scalar(ShArgs a) {
     for (int e = threadIdx.x; e < 4 * K; e += 256) {
         const int cell = corner_cell(base, e / K, a.nc), k = e % K;
         const int64_t i = (int64_t)cell * K + k;
-        if constexpr (HALF)
-            c[e / K][k] = (plane_at<L, true>(a.plane, i) + plane_at<L, true>(a.plane, i + a.plane_lo) * (1.0f / 2048.0f)) *
+        if constexpr (HALF) {
+            const int64_t j = (int64_t)cell * ((K + 15) / 16 * 16) + k;  // quad layout with one lane
+            c[e / K][k] = (plane_at<L, true>(a.plane, j) + plane_at<L, true>(a.plane, j + a.plane_lo) * (1.0f / 2048.0f)) *
                           a.scale[cell];
+        }
         else
             c[e / K][k] = plane_at<L, false>(a.plane, i);
     }
@@ -482,9 +516,12 @@ static int sh_launch(const ShArgs &a, bool half, hipStream_t s) {
         else hipLaunchKernelGGL((sh_records_scalar<L, false>), dim3((unsigned)a.items), dim3(256), 0, s, a);
     } else {
         if (half)
-            hipLaunchKernelGGL((sh_records_split_kernel<G, L>), dim3((unsigned)a.items, G / 16), dim3(256 * SplitK<L>::KP),
+            hipLaunchKernelGGL((sh_records_split_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 16 * SplitK<L>::KP),
                                0, s, a);
-        else hipLaunchKernelGGL((sh_records_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 16), 0, s, a);
+        else if (a.tile)
+            hipLaunchKernelGGL((sh_records_kernel<G, L, true>), dim3((unsigned)a.items, G / 16),
+                               dim3(ShShape<G, L, true>::THREADS), 0, s, a);
+        else hipLaunchKernelGGL((sh_records_kernel<G, L, false>), dim3((unsigned)a.items), dim3(G * 16), 0, s, a);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error("sh_records launch: %s", hipGetErrorString(e));

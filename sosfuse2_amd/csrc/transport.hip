@@ -175,6 +175,237 @@ __global__ void plan_nodes_kernel(TransportArgs a, const int64_t *node_off, int3
     }
 }
 
+// Static per-node geometry of a bundle for the warm consumer (one thread per ray), 8 float4 per node:
+//   [0] piece[0..3]  [1] piece[4], piece[5], post[4], post[5]  [2] post[0..3]  [3] ds, ah, ac, cell | last << 31
+//   [4..7] the tensor rotation T[q][u] (pairs xx, yy, zz, xz)
+// piece[k] = w6 ((p_prev + p_node) + 4 p_mid)[k] is the Simpson panel ending at the node with the five component
+// profiles and the sulfate profile in place of the lane's extinction (the lane takes piece . (ext, sext)); post is
+// the segment's closing panel (last node of a segment only). Everything is computed exactly as the transport kernel
+// computes it per lane (FP64 positions, the same node geometry), once per bundle instead of once per evaluation and
+// lane.
+__global__ void plan_geometry_kernel(TransportArgs a, const int64_t *node_off, float4 *geom) {
+    const int64_t ray = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (ray >= a.rays) return;
+    const int grp = a.ray_group[ray];
+    const double p[3] = {a.starts[3 * grp], a.starts[3 * grp + 1], a.starts[3 * grp + 2]};
+    const double d[3] = {a.dirs[3 * ray], a.dirs[3 * ray + 1], a.dirs[3 * ray + 2]};
+    float B[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) B[k] = a.targets[9 * grp + k];
+    const float df0 = (float)d[0], df1 = (float)d[1], df2 = (float)d[2];
+    const float dl[3] = {a.dloc[3 * ray], a.dloc[3 * ray + 1], a.dloc[3 * ray + 2]};
+    const int64_t e0 = a.edge_off[ray];
+    const int nseg = (int)(a.edge_off[ray + 1] - e0) - 1;
+    const unsigned bits = a.lower[ray];
+    int64_t n = node_off[ray];
+    auto prof6 = [&](double t, float *o) {
+        double r, qz, qx, qy;
+        node_position(p, d, t, r, qz, qx, qy);
+        float st;
+        profiles(a.shape, node_height(r, a.top), o, st);
+        o[5] = st;
+    };
+    for (int seg = 0; seg < nseg; ++seg) {
+        const double sa = a.edges[e0 + seg], sb = a.edges[e0 + seg + 1];
+        if (!(sb - sa > 1e-8)) continue;
+        const bool lower = (bits >> seg) & 1u;
+        double tprev = sa;
+        float pprev[6];
+        prof6(sa, pprev);
+        for (int step = 0; step < a.nq; ++step, ++n) {
+            double t;
+            float ds;
+            node_t(sa, sb, lower, a.gl_u[step], a.gl_w[step], t, ds);
+            double r, qz, qx, qy;
+            node_position(p, d, t, r, qz, qx, qy);
+            const NodeGeom ng = node_geometry(qx, qy, qz, r, a.h, a.nh, a.chi, a.nc, a.top);
+            float pn[6], pm[6], piece[6], post[6];
+            {
+                float st;
+                profiles(a.shape, ng.h, pn, st);
+                pn[5] = st;
+            }
+            prof6(0.5 * (tprev + t), pm);
+            const float w6 = (float)(t - tprev) / 6.0f;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) piece[k] = w6 * ((pprev[k] + pn[k]) + 4.0f * pm[k]);
+            const bool last = step == a.nq - 1;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) post[k] = 0.0f;
+            if (last) {
+                float pe[6], pme[6];
+                prof6(0.5 * (t + sb), pme);
+                prof6(sb, pe);
+                const float w6e = (float)(sb - t) / 6.0f;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) post[k] = w6e * ((pn[k] + pe[k]) + 4.0f * pme[k]);
+            }
+            const float cx = ng.cx, sx = ng.sx, sn = ng.sn, cc = ng.cc;
+            const float Q[9] = {-cc * cx, sx, sn * cx, -cc * sx, -cx, sn * sx, sn, 0.0f, cc};
+            float T[4][4];
+            {
+                float A[3][3], dt[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int jj = 0; jj < 3; ++jj) A[i][jj] = (B[i] * Q[jj] + B[3 + i] * Q[3 + jj]) + B[6 + i] * Q[6 + jj];
+#pragma unroll
+                for (int jj = 0; jj < 3; ++jj) dt[jj] = (df0 * Q[jj] + df1 * Q[3 + jj]) + df2 * Q[6 + jj];
+                float Kx[3][3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int jj = 0; jj < 3; ++jj) Kx[i][jj] = A[i][jj] - dl[i] * dt[jj];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = PairIdx<4>::i[q], k = PairIdx<4>::k[q];
+                    T[q][0] = Kx[i][0] * Kx[k][0];
+                    T[q][1] = Kx[i][1] * Kx[k][1];
+                    T[q][2] = Kx[i][2] * Kx[k][2];
+                    T[q][3] = Kx[i][0] * Kx[k][2] + Kx[i][2] * Kx[k][0];
+                }
+            }
+            float4 *o = geom + n * 8;
+            const unsigned cellbits = (unsigned)(ng.ir * a.nc + ng.j) | (last ? 0x80000000u : 0u);
+            o[0] = make_float4(piece[0], piece[1], piece[2], piece[3]);
+            o[1] = make_float4(piece[4], piece[5], post[4], post[5]);
+            o[2] = make_float4(post[0], post[1], post[2], post[3]);
+            o[3] = make_float4(ds, ng.ah, ng.ac, __uint_as_float(cellbits));
+#pragma unroll
+            for (int q = 0; q < 4; ++q) o[4 + q] = make_float4(T[q][0], T[q][1], T[q][2], T[q][3]);
+            pprev[0] = pn[0]; pprev[1] = pn[1]; pprev[2] = pn[2]; pprev[3] = pn[3]; pprev[4] = pn[4]; pprev[5] = pn[5];
+            tprev = t;
+        }
+    }
+}
+
+// Warm consumer on the static node geometry (records mode, pairs xx yy zz xz): per node and lane only the lane's
+// optical depth, attenuation, the four corners (Rayleigh tensor from the field, SH value from the records) with the
+// log-space trace blend, and the Kahan radiance sum remain. Same operations as transport_kernel's warm path.
+template <int G, bool BOUNDARY>
+__global__ __launch_bounds__(256) void transport_geom_kernel(TransportArgs a) {
+    constexpr int RPW = 64 / G;
+    constexpr int P = 4;
+    const int lane = threadIdx.x & 63;
+    const int w = lane % G;
+    const int sub = lane / G;
+    float lc[SOS_LANE_CONSTANTS];
+#pragma unroll
+    for (int k = 0; k < SOS_LANE_CONSTANTS; ++k) lc[k] = a.lane[k * G + w];
+    const float mix = lc[SOS_LANE_MIX];
+    const float sext = a.strato ? lc[SOS_LANE_SEXT] : 0.0f;
+    const float ext6[6] = {lc[0], lc[1], lc[2], lc[3], lc[4], sext};
+    const int64_t nwg = gridDim.x;
+    const int xcd = blockIdx.x & 7, idx = blockIdx.x >> 3;
+    const int64_t per_xcd_wg = (nwg + 7 - xcd) >> 3;
+    const int64_t waves_xcd = per_xcd_wg * 4;
+    const int64_t chunk = (a.tiles + 7) >> 3;
+    const int64_t t0 = xcd * chunk, t1 = min(a.tiles, t0 + chunk);
+    const int nc = a.nc;
+    for (int64_t tile = t0 + idx * 4 + (threadIdx.x >> 6); tile < t1; tile += waves_xcd) {
+        const int64_t ray = tile * RPW + sub;
+        if (ray >= a.rays) continue;
+        const int grp = a.ray_group[ray];
+        const float dl[3] = {a.dloc[3 * ray], a.dloc[3 * ray + 1], a.dloc[3 * ray + 2]};
+        float pu[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int i = PairIdx<P>::i[q], k = PairIdx<P>::k[q];
+            pu[q] = (i == k ? 1.0f : 0.0f) - dl[i] * dl[k];
+        }
+        (void)grp;
+        float total[P], corr[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) total[q] = corr[q] = 0.0f;
+        float tau = 0.0f, tc = 0.0f, acc = 0.0f;
+        const int64_t n0 = a.node_off[ray], n1 = a.node_off[ray + 1];
+        for (int64_t n = n0; n < n1; ++n) {
+            const float4 *g = a.node_geom + n * 8;
+            const float4 g0 = g[0], g1 = g[1], g3 = g[3];
+            acc = acc + (((((g0.x * ext6[0] + g0.y * ext6[1]) + g0.z * ext6[2]) + g0.w * ext6[3]) + g1.x * ext6[4]) +
+                         g1.y * ext6[5]);
+            const float mid = (tau - tc) + acc;
+            const float att = g3.x * expf(-mid);
+            const float ah = g3.y, ac = g3.z;
+            const unsigned cb = __float_as_uint(g3.w);
+            const int cell0 = (int)(cb & 0x7fffffffu);
+            const int ir = cell0 / nc;
+            const int cells[4] = {cell0, cell0 + 1, cell0 + nc, cell0 + nc + 1};
+            const float sw[4] = {(1.0f - ah) * (1.0f - ac), (1.0f - ah) * ac, ah * (1.0f - ac), ah * ac};
+            float T[P][4];
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const float4 t4 = g[4 + q];
+                T[q][0] = t4.x; T[q][1] = t4.y; T[q][2] = t4.z; T[q][3] = t4.w;
+            }
+            const float *rec = a.records + n * 4 * G + w;
+            float val[P];
+#pragma unroll
+            for (int q = 0; q < P; ++q) val[q] = 0.0f;
+            float logs = 0.0f, wg = 0.0f, am = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float aero = rec[c * G];
+                const float *f = a.field + (int64_t)cells[c] * a.rows * G + w;
+                const float rm[4] = {f[0], f[G], f[2 * G], f[3 * G]};
+                const float trace = (rm[0] + rm[1]) + rm[2];
+                const float density = a.hr[ir + (c >> 1)];
+                float tens[P];
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    float rot = 0.0f;
+#pragma unroll
+                    for (int uu = 0; uu < 4; ++uu) rot = rot + T[q][uu] * rm[uu];
+                    tens[q] = density * (mix * rot + ((1.0f - mix) / 3.0f * trace) * pu[q]);
+                    tens[q] = tens[q] + (aero * pu[q]) / 2.0f;
+                    val[q] = val[q] + sw[c] * tens[q];
+                }
+                const float tr = (tens[0] + tens[1]) + tens[2];
+                if (tr >= SOS_FLT_MIN) {
+                    logs = logs + sw[c] * logf(tr);
+                    wg = wg + sw[c];
+                    am = am + sw[c] * tr;
+                }
+            }
+            const bool ok = (wg > 0.0f) && (am >= SOS_FLT_MIN);
+            const float mean = ok ? am / wg : 1.0f;
+            const float scale = expf(ok ? logs - wg * logf(mean) : 0.0f);
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const float y = (val[q] * scale) * att - corr[q];
+                const float tn = total[q] + y;
+                corr[q] = (tn - total[q]) - y;
+                total[q] = tn;
+            }
+            if (cb & 0x80000000u) {
+                const float4 g2 = g[2];
+                acc = acc + (((((g2.x * ext6[0] + g2.y * ext6[1]) + g2.z * ext6[2]) + g2.w * ext6[3]) + g1.z * ext6[4]) +
+                             g1.w * ext6[5]);
+                const float y = acc - tc;
+                const float tn = tau + y;
+                tc = (tn - tau) - y;
+                tau = tn;
+                acc = 0.0f;
+            }
+        }
+        if constexpr (BOUNDARY) {
+            const float ef = a.end_f[ray];
+            const int ej = a.end_j[ray];
+            float E = (1.0f - ef) * a.boundary[ej * G + w] + ef * a.boundary[(ej + 1) * G + w];
+            float optical = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) optical = optical + a.end_col[6 * ray + k] * lc[k];
+            optical = optical + a.end_col[6 * ray + 5] * sext;
+            E = E * (expf(-optical) * (a.hits[ray] ? 1.0f : 0.0f));
+#pragma unroll
+            for (int q = 0; q < P; ++q) total[q] = total[q] + (E * pu[q]) / 2.0f;
+        }
+        float *o = a.out + (ray * G + w) * P;
+#pragma unroll
+        for (int q = 0; q < P; ++q) o[q] = total[q];
+    }
+}
+
 enum Aero { DIRECT = 0, RECORDS = 1 };
 
 template <int G, int L, int P, bool FIRST, bool BOUNDARY, int AERO, bool UNI = false>
@@ -573,11 +804,27 @@ static int launch_one(const TransportArgs &a, hipStream_t s) {
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const int64_t wgs_needed = (a.tiles + 3) / 4;
-    int64_t grid = (int64_t)cus * 8;
+    int64_t grid = (int64_t)cus * (a.blocks_per_cu > 0 ? a.blocks_per_cu : 8);
     if (grid > wgs_needed) grid = wgs_needed;
     grid = ((grid + 7) / 8) * 8;
     if (grid < 8) grid = 8;
     hipLaunchKernelGGL((transport_kernel<G, L, P, FIRST, BOUNDARY, AERO, UNI>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error("transport launch: %s", hipGetErrorString(e));
+    return SOS_OK;
+}
+
+template <int G, bool BOUNDARY>
+static int launch_geom(const TransportArgs &a, hipStream_t s) {
+    int dev = 0, cus = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int64_t wgs_needed = (a.tiles + 3) / 4;
+    int64_t grid = (int64_t)cus * (a.blocks_per_cu > 0 ? a.blocks_per_cu : 8);
+    if (grid > wgs_needed) grid = wgs_needed;
+    grid = ((grid + 7) / 8) * 8;
+    if (grid < 8) grid = 8;
+    hipLaunchKernelGGL((transport_geom_kernel<G, BOUNDARY>), dim3((unsigned)grid), dim3(256), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error("transport launch: %s", hipGetErrorString(e));
     return SOS_OK;
@@ -591,6 +838,10 @@ static int launch_flags(const TransportArgs &a, int L, hipStream_t s) {
     if (first)
         return (a.flags & SOS_UNIFORM_PHASE) ? launch_one<G, 8, P, true, false, RECORDS, true>(a, s)
                                              : launch_one<G, 8, P, true, false, RECORDS, false>(a, s);
+    if (records && (a.flags & SOS_NODE_GEOMETRY)) {
+        if constexpr (P == 4) return bnd ? launch_geom<G, true>(a, s) : launch_geom<G, false>(a, s);
+        else return set_error("node geometry: pairs must be 4");
+    }
     if (records)
         return bnd ? launch_one<G, 8, P, false, true, RECORDS>(a, s) : launch_one<G, 8, P, false, false, RECORDS>(a, s);
 #define SOS_DIRECT(LL)                                                                                   \
@@ -635,6 +886,14 @@ int plan_nodes_launch(const TransportArgs &a, const int64_t *node_off, int32_t *
                            reinterpret_cast<float4 *>(dir));
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? SOS_OK : set_error("plan_nodes: %s", hipGetErrorString(e));
+}
+
+int plan_geometry_launch(const TransportArgs &a, const int64_t *node_off, float *geom, hipStream_t s) {
+    const unsigned b = (unsigned)((a.rays + 255) / 256);
+    if (b)
+        hipLaunchKernelGGL(plan_geometry_kernel, dim3(b), dim3(256), 0, s, a, node_off, reinterpret_cast<float4 *>(geom));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SOS_OK : set_error("plan_geometry: %s", hipGetErrorString(e));
 }
 
 }  // namespace sos

@@ -9,7 +9,7 @@ c_i32, c_i64, c_u32, c_f32 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32, ct
 P = ctypes.c_void_p
 
 LANE_CONSTANTS = 12
-FIRST, BOUNDARY, RECORDS, UNIFORM_PHASE, GROUND_RECORDS = 1, 2, 8, 16, 32
+FIRST, BOUNDARY, RECORDS, UNIFORM_PHASE, GROUND_RECORDS, NODE_GEOMETRY = 1, 2, 8, 16, 32, 64
 
 
 class Bundle(ctypes.Structure):
@@ -17,7 +17,7 @@ class Bundle(ctypes.Structure):
         ('groups', c_i64), ('rays', c_i64), ('group_off_dev', P), ('starts_dev', P), ('targets_dev', P),
         ('ray_group_dev', P), ('dirs_dev', P), ('dloc_dev', P), ('weights_dev', P), ('edge_off_dev', P),
         ('edges_dev', P), ('lower_dev', P), ('end_j_dev', P), ('end_f_dev', P), ('end_col_dev', P), ('hits_dev', P),
-        ('node_off_dev', P),
+        ('node_off_dev', P), ('node_geom_dev', P),
     ]
 
 
@@ -28,7 +28,7 @@ class Grid(ctypes.Structure):
         ('no2_s', c_f32), ('st_c', c_f32), ('st_s', c_f32),
         ('gl_u_dev', P), ('gl_w_dev', P), ('h_dev', P), ('chi_dev', P), ('hr_dev', P), ('lane_dev', P),
         ('sun_nh', c_i32), ('sun_nx', c_i32), ('sun_h_dev', P), ('sun_col_dev', P), ('goff_dev', P),
-        ('gdir_dev', P), ('gval_dev', P), ('gm_dev', P),
+        ('gdir_dev', P), ('gval_dev', P), ('gm_dev', P), ('blocks_per_cu', c_i32), ('sh_tile', c_i32),
     ]
 
 
@@ -36,6 +36,7 @@ _SIGNATURES = {
     'sos_transport': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), c_i32, c_i32, c_i32, P, P, P, P, P, P, P],
     'sos_plan_count': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, P],
     'sos_plan_nodes': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, P, P, P, P, P],
+    'sos_plan_geometry': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, P, P],
     'sos_ground_records': [ctypes.POINTER(Grid), c_i32, ctypes.POINTER(c_f32), ctypes.POINTER(c_f32), c_i64, P, P, P, P,
                            P, P, P],
     'sos_sh_records': [ctypes.POINTER(Grid), P, P, c_i32, c_i64, P, P, P, P, P, P, P, P],
@@ -47,6 +48,11 @@ _SIGNATURES = {
     'sos_anderson_gram': [c_i64, c_i32, c_i32, c_i32, c_u32, P, P, P, P],
     'sos_anderson_solve': [c_i32, c_i32, c_i32, P, P, c_u32, P, P],
     'sos_anderson_mix': [c_i64, c_i32, c_i32, c_i32, c_i32, P, P, P, P, P],
+    'sos_mlp_gemm': [c_i64, c_i64, c_i64, P, c_i64, c_i64, P, c_i64, c_i64, P, c_i32, P, c_i64, P, c_i64, c_i32, P, P],
+    'sos_mlp_gelu_backward': [c_i64, c_i64, P, c_i64, P, c_i64, P, c_i64, P],
+    'sos_mlp_mse': [c_i64, c_i64, P, c_i64, P, c_i64, P, P, c_i64, P, P, P],
+    'sos_mlp_bias_grad': [c_i64, c_i64, P, c_i64, P, P],
+    'sos_mlp_adam': [c_i64, P, P, P, P, c_i32, c_f32, c_f32, c_f32, c_f32, P],
     'sos_device_info': [ctypes.POINTER(c_i32), ctypes.POINTER(c_i32), ctypes.c_char_p, c_i32],
 }
 

@@ -61,7 +61,7 @@ class DeviceBundle:
         self.struct = hip.Bundle(
             self.groups, self.rays, *(hip.ptr(x) for x in (
                 self.group_off, self.starts, self.targets, self.ray_group, self.dirs, self.dloc, self.weights,
-                self.edge_off, self.edges, self.lower, self.end_j, self.end_f, self.end_col, self.hits)), None)
+                self.edge_off, self.edges, self.lower, self.end_j, self.end_f, self.end_col, self.hits)), None, None)
 
     def nbytes(self):
         return sum(x.numel() * x.element_size() for x in (
@@ -71,11 +71,12 @@ class DeviceBundle:
 
 class Plan:
     """Visit plan of a bundle (fixed by its geometry): every quadrature node's cell, mu, cos(phi) and direction; the
-    nodes sorted by their base cell with work items of at most `chunk` nodes of one base cell (SH records, four
-    corners per node); and the (node, corner) visits sorted by visited cell with items of at most `chunk` visits
-    (ground records of first light)."""
+    nodes sorted by their base cell with work items of at most `node_chunk` nodes of one base cell (SH records,
+    four corners per node); and the (node, corner) visits sorted by visited cell with items of at most `chunk`
+    visits (ground records of first light)."""
 
     def __init__(self, kernel, dc, db, chunk=2048, node_chunk=1024):
+        self.node_chunk = node_chunk
         dev = kernel.device
         st = kernel.stream()
         counts = torch.empty(max(db.rays, 1), dtype=torch.int64, device=dev)
@@ -96,8 +97,8 @@ class Plan:
         self.node = torch.argsort(self.node_cell[:N], stable=True).to(torch.int32)
         self.sorted_mu = self.node_mu[self.node.long()]
         self.sorted_cp = self.node_cp[self.node.long()]
-        self.sh_cell, self.sh_v0, self.sh_v1 = _items(self.node_cell[:N], cells, node_chunk, dev)
-        self.sh_items = len(self.sh_cell)
+        self._cells = cells
+        self.set_node_chunk(node_chunk)
         # visits by visited cell (ground records)
         offs = torch.tensor([0, 1, nc, nc + 1], dtype=torch.int32, device=dev)
         vcell = (self.node_cell[:N, None] + offs[None, :]).reshape(-1)
@@ -107,6 +108,24 @@ class Plan:
         self.items = len(self.item_cell)
         self.visits = 4 * N
         self.records = torch.empty((max(4 * N, 1), dc.G), dtype=torch.float32, device=dev)
+        self.geom = None  # static node geometry of the warm consumer, built on first use (geometry())
+
+    def geometry(self, kernel, dc, db):
+        """[nodes, 32] static node geometry (sos_plan_geometry), built once."""
+        if self.geom is None:
+            self.geom = torch.empty((max(self.nodes, 1), 32), dtype=torch.float32, device=self.node_off.device)
+            hip.call('sos_plan_geometry', dc.struct, db.struct, hip.ptr(self.node_off), hip.ptr(self.geom),
+                     kernel.stream())
+            db.struct.node_geom_dev = hip.ptr(self.geom)
+        return self.geom
+
+    def set_node_chunk(self, node_chunk):
+        """Work items of the SH records: runs of at most node_chunk nodes of one base cell (a tuning knob; the
+        records do not depend on it)."""
+        self.node_chunk = node_chunk
+        self.sh_cell, self.sh_v0, self.sh_v1 = _items(self.node_cell[: self.nodes], self._cells, node_chunk,
+                                                      self.node_off.device)
+        self.sh_items = len(self.sh_cell)
 
     def records_for(self, G):
         """The records buffer [4 nodes, G] (reallocated when a chunk of another lane count uses the plan)."""
@@ -206,10 +225,13 @@ class Kernel:
         self.mixed = mixed
         self.records = records  # SH by per-cell MFMA records (True) or per-ray register caches (False)
         self.uniform_phase = True  # use the shared-phase first light when a chunk's lanes share the asymmetry
+        self.node_geometry = True  # warm records consumer on the plan's static node geometry
         self.xp = torch
         self._constants = {}
         self.metrics = dict(samples=0, launches=0)
         self.timed = None  # list of (name, start event, end event) while timing is on
+        # launch shapes chosen by the autotuner (sosfuse2_amd.tune); results do not depend on them
+        self.tuning = dict(blocks_per_cu=0, node_chunk=1024, sh_tile=0)
         self.counters = None  # device uint64 [2] while counting is on
 
     def sync(self):
@@ -235,11 +257,13 @@ class Kernel:
 
     # ------------------------------------------------------------------------------------------------ launches
     def plane(self, ctx, field, dc):
-        """Combined SH plane of a source field (m-major per cell): (FP32 plane, None), or in mixed precision the split
-        FP16 plane [2 (hi, lo), cells, K, G] and its per-(cell, lane) power-of-two scale."""
+        """Combined SH plane of a source field (m-major per cell): (FP32 plane [cells, K, G], None), or in mixed
+        precision the split FP16 plane [2 (hi, lo), cells, KZ/4, G, 4] (K padded to KZ, quads of four K positions per
+        lane) and its per-(cell, lane) power-of-two scale."""
         K = self.K
         if self.mixed:
-            out = torch.empty((2, ctx.geo.ns, K, dc.G), dtype=torch.float16, device=self.device)
+            KZ = (K + 15) // 16 * 16
+            out = torch.empty((2, ctx.geo.ns, KZ // 4, dc.G, 4), dtype=torch.float16, device=self.device)
         else:
             out = torch.empty((ctx.geo.ns, K, dc.G), dtype=torch.float32, device=self.device)
         scale = torch.empty((ctx.geo.ns, dc.G), dtype=torch.float32, device=self.device) if self.mixed else None
@@ -249,7 +273,9 @@ class Kernel:
 
     def plan(self, db, dc):
         if db.plan is None:
-            db.plan = Plan(self, dc, db)
+            db.plan = Plan(self, dc, db, node_chunk=self.tuning['node_chunk'])
+        elif db.plan.node_chunk != self.tuning['node_chunk']:
+            db.plan.set_node_chunk(self.tuning['node_chunk'])
         return db.plan
 
     def _event(self):
@@ -262,6 +288,8 @@ class Kernel:
     def radiance(self, db, ctx, field, first=False, boundary=None, pairs=4, plane=None):
         """Per-ray radiance tensor [rays, G, pairs]."""
         dc = self.constants(ctx, db.nq)
+        dc.struct.blocks_per_cu = self.tuning['blocks_per_cu']
+        dc.struct.sh_tile = self.tuning['sh_tile']
         out = torch.empty((db.rays, dc.G, pairs), dtype=torch.float32, device=self.device)
         flags = (hip.FIRST if first else 0) | (hip.BOUNDARY if boundary is not None else 0)
         records = None
@@ -295,6 +323,9 @@ class Kernel:
                     self.timed.append(('sh_' + name, e0, e1))
                 records = pl.records
                 flags |= hip.RECORDS
+                if pairs == 4 and self.node_geometry:
+                    pl.geometry(self, dc, db)
+                    flags |= hip.NODE_GEOMETRY
         e0 = self._event()
         hip.call('sos_transport', dc.struct, db.struct, flags, pairs, ctx.rows,
                  None if first else hip.ptr(field), None if first else hip.ptr(plane[0]), hip.ptr(boundary),

@@ -57,12 +57,14 @@ def observer_bundle(s, hs, el, az, kernel=None):
 class Solver:
     """Fixed-point iteration of the source field of one chunk."""
 
-    def __init__(self, s, geometry_, depth=4, log=print):
+    def __init__(self, s, geometry_, depth=4, log=print, graph=False):
         self.s = s
         self.geometry = geometry_
         self.kernel = s.kernel
         self.depth = depth
         self.log = log
+        self.graph = graph  # replay the evaluation A(x) as one captured HIP graph
+        self._graph = None
         self.first_moments = None
         self.first_flux = None
         self.G = lane_count(s.W)
@@ -86,7 +88,35 @@ class Solver:
             self.timings['setup_seconds'] = time.perf_counter() - t
 
     def apply(self, field):
-        """g = A(x) (`source_iteration.py:161-216`)."""
+        """g = A(x) (`source_iteration.py:161-216`). With graph=True the launches of one evaluation (plane, SH
+        records, ground and warm transports, projections, composition) are captured once into a HIP graph and
+        replayed on a static input buffer; the result is a fresh tensor either way."""
+        if self.graph:
+            return self._apply_graph(field)
+        return self._apply(field)
+
+    def _apply_graph(self, field):
+        if self._graph is None:
+            self._apply(field)  # eager warm-up: first light, plans and constants exist before the capture
+            self.kernel.sync()
+            self._gx = torch.empty_like(field)
+            self._gx.copy_(field)
+            timed, self.kernel.timed = self.kernel.timed, None
+            g = torch.cuda.CUDAGraph()
+            side = torch.cuda.Stream(self.device)
+            side.wait_stream(torch.cuda.current_stream(self.device))
+            with torch.cuda.stream(side):
+                self._apply(self._gx)  # a second eager pass on the capture stream settles its allocator state
+            torch.cuda.current_stream(self.device).wait_stream(side)
+            with torch.cuda.graph(g):
+                self._gy = self._apply(self._gx)
+            self.kernel.timed = timed
+            self._graph = g
+        self._gx.copy_(field)
+        self._graph.replay()
+        return self._gy.clone()
+
+    def _apply(self, field):
         self.initialize()
         t = time.perf_counter()
         s, k = self.s, self.kernel

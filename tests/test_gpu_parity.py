@@ -171,7 +171,7 @@ def test_mixed_records_match_fp32(gpu, W, L):
     rec32 = pl.records.clone()
     dc16 = k16.constants(s16, db.nq)
     plane16 = k16.plane(s16, fd, dc16)
-    assert plane16[0].shape == (2, s.geo.ns, s.K, dc.G)
+    assert plane16[0].shape == (2, s.geo.ns, (s.K + 15) // 16 * 4, dc.G, 4)
     hip.call('sos_sh_records', dc16.struct, hip.ptr(plane16[0]), hip.ptr(plane16[1]), 1, pl.sh_items,
              hip.ptr(pl.sh_cell), hip.ptr(pl.sh_v0), hip.ptr(pl.sh_v1), hip.ptr(pl.node), hip.ptr(pl.sorted_mu),
              hip.ptr(pl.sorted_cp), hip.ptr(pl.records), k16.stream())
@@ -412,8 +412,11 @@ def test_batched_solve_matches_per_state_solves(gpu):
     from sosfuse2_amd import batch
 
     s0, scope, _ = setup_case(3, 16)
-    items = batch.items_from_seed(5, seed=4, top_km=100.0, all_elevations=[-2.0, -4.0, -6.0], max_wavelengths=9,
-                                  max_elevations=3)
+    # the 5-level test grid resolves the 1.2 km aerosol layer with one 2.3 km cell: states of large optical depth
+    # stall there (the FP32 oracle stalls the same way), so the test keeps AOD < 0.2
+    items = [it for it in batch.items_from_seed(16, seed=4, top_km=100.0, all_elevations=[-2.0, -4.0, -6.0],
+                                                max_wavelengths=9, max_elevations=3) if it.state.aod550 < 0.2][:5]
+    assert len(items) == 5
     el = 90 - (np.arange(3) + 0.5) * 30
     az = (np.arange(4) + 0.5) * 45
     got = batch.solve(scope, items, el, az, tolerance=1e-5, max_eval=40, lanes=16)
@@ -424,3 +427,62 @@ def test_batched_solve_matches_per_state_solves(gpu):
         alone, _ = api.make_table(s, x, it.elevations, el, az, log=None)
         assert table.shape == alone.shape
         assert np.max(np.abs(table[..., 0] - alone[..., 0]) / np.abs(alone[..., 0])) < 2e-5
+
+
+def test_graph_replay_matches_eager(gpu):
+    """Solver(graph=True) captures one evaluation into a HIP graph and replays it on every evaluation, with the eager
+    Anderson update, monitors and host checks between the replays; every replay must reproduce the eager
+    evaluation bit for bit, so the whole solve matches (same evaluations, identical source)."""
+    s, scope, _ = setup_case(3, 16)
+    x0, i0 = scope.solver(s, log=None).solve(40, 1e-5)
+    sv = scope.solver(s, log=None, graph=True)
+    x1, i1 = sv.solve(40, 1e-5)
+    assert sv._graph is not None
+    assert i1['evaluations'] == i0['evaluations']
+    assert torch.equal(x0, x1)
+    # replays on changing inputs against eager evaluations
+    rng = np.random.default_rng(8)
+    for k in range(3):
+        f = torch.from_numpy(np.ascontiguousarray(padded(physical_field(s, 30 + k), 16))).cuda()
+        assert torch.equal(sv.apply(f), sv._apply(f))
+
+
+def test_autotuner_keeps_results(gpu):
+    """The autotuner picks launch shapes by timing; results do not depend on them (bitwise)."""
+    from sosfuse2_amd import tune
+
+    s, scope, _ = setup_case(3, 16)
+    sv = scope.solver(s, log=None)
+    f = torch.from_numpy(np.ascontiguousarray(padded(physical_field(s, 3), 16))).cuda()
+    ref = sv.apply(f)
+    best, timings = tune.autotune(sv, node_chunks=(256, 1024), blocks_per_cu=(4, 16))
+    assert (best['node_chunk'], best['blocks_per_cu'], best['sh_tile']) in timings
+    assert len(timings) == 8
+    assert s.kernel.tuning == best
+    for nc, b, st in timings:
+        s.kernel.tuning.update(node_chunk=nc, blocks_per_cu=b, sh_tile=st)
+        assert torch.equal(sv.apply(f), ref)
+    s.kernel.tuning.update(best)
+    again, t2 = tune.autotune(sv)
+    assert again == best and t2 == {}
+
+
+@pytest.mark.parametrize('W,L', [(3, 16), (20, 32)])
+def test_node_geometry_consumer_matches_per_lane_path(gpu, W, L):
+    """The warm consumer on the plan's static node geometry (Simpson panels as profile weights, rotation, corners and
+    weights computed once per bundle) against the per-lane consumer that recomputes them at every node: the
+    same radiances to FP32 rounding (the Simpson panels sum in another order)."""
+    s, scope, _ = setup_case(W, L)
+    k = s.kernel
+    solver = scope.solver(s, log=None)
+    db = solver.geometry.bundle
+    dc = k.constants(s, db.nq)
+    f = torch.from_numpy(np.ascontiguousarray(padded(physical_field(s, 12), dc.G))).cuda()
+    bnd = torch.from_numpy(np.abs(np.random.default_rng(2).normal(size=(s.geo.nc, dc.G))).astype('f4') * 1e-3).cuda()
+    k.node_geometry = True
+    a = k.radiance(db, s, f, False, bnd, 4).cpu().numpy()
+    k.node_geometry = False
+    b = k.radiance(db, s, f, False, bnd, 4).cpu().numpy()
+    k.node_geometry = True
+    assert db.plan.geom is not None
+    assert rel(a, b) < 2e-6

@@ -1,0 +1,110 @@
+"""Learned warm start: the HIP training and inference kernels against the CPU oracle, and what the trained network
+gains over a cold start. The network only seeds the iteration; the solve still runs to its tolerance."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp as om
+from sosfuse2_amd import api, atmosphere, hip, learned
+
+pytestmark = pytest.mark.gpu
+
+
+def _net_and_data(sizes, n, seed):
+    rng = np.random.default_rng(seed)
+    net = learned.Network(sizes, seed=seed)
+    # non-zero biases so that their gradients and products are exercised
+    net.flat.copy_(net.flat + torch.from_numpy(rng.normal(size=net.flat.numel()).astype('f4') * 0.01).cuda())
+    x = rng.normal(size=(n, sizes[0])).astype('f4')
+    t = rng.normal(size=(n, sizes[-1])).astype('f4')
+    params = [(net.W(i).cpu().numpy().astype('f8'), net.b(i).cpu().numpy().astype('f8')) for i in range(net.layers)]
+    return net, x, t, params
+
+
+@pytest.mark.parametrize('n', [300, 1100])
+def test_network_kernels_match_oracle(gpu, n):
+    """Forward, loss, every weight and bias gradient (n = 1100 takes the split-K weight-gradient path) and one Adam
+    step against the oracle evaluated with the same BF16 rounding points in FP64. Tolerances: 2e-3 of the largest
+    output for the forward pass, 2e-2 of the largest gradient entry per tensor (a BF16 flip of one activation moves
+    a gradient entry by up to 2^-8 of its size), 1e-6 for Adam."""
+    sizes = [37, 64, 64, 64, 29]
+    net, x, t, params = _net_and_data(sizes, n, 7)
+    xd, td = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
+    y = net.forward(xd, keep=True)
+    loss, dy = net.loss_and_grad(y, td)
+    net.backward(dy)
+    torch.cuda.synchronize()
+    yo, cache = om.forward(params, x.astype('f8'), round_bf16=True)
+    yh = y.cpu().numpy()
+    assert np.max(np.abs(yh - yo)) / np.max(np.abs(yo)) < 2e-3
+    assert abs(float(loss.item()) - om.loss(yo, t)) / om.loss(yo, t) < 2e-3
+    # gradients of the oracle at the HIP outputs (the same dy up to FP32), with the same rounding points
+    go = om.backward(params, cache, yh.astype('f8'), t.astype('f8'), round_bf16=True)
+    for i in range(net.layers):
+        dW = net.W(i, net.grad).cpu().numpy()
+        db = net.b(i, net.grad).cpu().numpy()
+        assert np.max(np.abs(dW - go[i][0])) / np.max(np.abs(go[i][0])) < 2e-2, i
+        assert np.max(np.abs(db - go[i][1])) / np.max(np.abs(go[i][1])) < 2e-2, i
+        assert np.median(np.abs(dW - go[i][0])) / np.max(np.abs(go[i][0])) < 2e-3, i
+    # one Adam step
+    p0, g0 = net.flat.cpu().numpy().astype('f8'), net.grad.cpu().numpy().astype('f8')
+    m, v = np.zeros_like(p0), np.zeros_like(p0)
+    net.adam(lr=1e-3)
+    om.adam(p0, g0, m, v, 1, lr=1e-3)
+    assert np.max(np.abs(net.flat.cpu().numpy() - p0)) < 1e-6
+
+
+def test_training_lowers_the_loss(gpu):
+    """Adam on the HIP kernels fits a smooth map (the loss falls by 10x in 300 steps)."""
+    rng = np.random.default_rng(3)
+    A = rng.normal(size=(20, 12)) / 4
+    x = rng.normal(size=(4096, 20)).astype('f4')
+    t = np.tanh(x @ A).astype('f4')
+    net = learned.Network([20, 64, 64, 64, 12], seed=1)
+    hist = learned.train(net, torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda(), 300, batch=512, lr=2e-3)
+    assert hist[-1] < hist[0] / 10
+
+
+def test_warm_start_saves_evaluations(gpu):
+    """Train on solves of 12 seeded states of the batched distribution (small grid, three wavelengths), then solve 4
+    held-out states cold and warm-started: the warm starts need fewer evaluations on average and reach the same
+    tables (both iterate to 1e-5)."""
+    cfg = dict(nr=5, chi_grid='tiny', nmu=1, nphi=2, path_n=4, L=16, top_km=100.0, observer_km=0.1, query_n=6,
+               anderson_depth=4)
+    wl = atmosphere.log_spaced_wavelengths(3, 420, 680)
+    # AOD < 0.2: the 5-level test grid cannot resolve thicker aerosol layers (the iteration stalls there, FP32
+    # oracle included)
+    pick = lambda n, seed: [st for st, _, _ in atmosphere.sample_states(4 * n, seed=seed, top_km=100.0)
+                            if st.aod550 < 0.2][:n]
+    train_states, test_states = pick(12, 21), pick(4, 99)
+    assert len(train_states) == 12 and len(test_states) == 4
+    s0, scope = api.create_configured(None, cfg, wl, 'hip', train_states[0])
+    Xs, Ts = [], []
+    for st in train_states:
+        s = scope.context(atmosphere.lane_optics(st, wl))
+        solver = scope.solver(s, log=None)
+        x, _ = solver.solve(40, 1e-5)
+        X, T = learned.samples(s, solver, x)
+        Xs.append(X)
+        Ts.append(T)
+    X, T = torch.cat(Xs), torch.cat(Ts)
+    net = learned.Network(learned.sizes_for(s0.rows), seed=0)
+    hist = learned.train(net, X, T, 400, batch=1024, lr=1e-3)
+    assert hist[-1] < hist[0]
+    hs = [-3.0, -6.0]
+    el = 90 - (np.arange(3) + 0.5) * 30
+    az = (np.arange(4) + 0.5) * 45
+    cold, warm = [], []
+    for st in test_states:
+        s = scope.context(atmosphere.lane_optics(st, wl))
+        xc, ic = scope.solver(s, log=None).solve(40, 1e-5)
+        sv = scope.solver(s, log=None)
+        xw, iw = sv.solve(40, 1e-5, x0=learned.warm_start(net, s, sv))
+        cold.append(ic['evaluations'])
+        warm.append(iw['evaluations'])
+        tc, _ = api.make_table(s, xc, hs, el, az, log=None)
+        tw, _ = api.make_table(s, xw, hs, el, az, log=None)
+        assert np.max(np.abs(tw[..., 0] - tc[..., 0]) / tc[..., 0]) < 1e-4
+    print(f'evaluations cold {cold} warm {warm}')
+    assert np.mean(warm) < np.mean(cold)

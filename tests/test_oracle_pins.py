@@ -332,3 +332,61 @@ def test_projection_of_isotropic_light(nmu, nphi, tol):
             assert np.allclose(out[k], 4 * math.pi / 3, rtol=tol, atol=0)
         assert np.allclose(out[3], 0.0, atol=1e-12)
         assert np.allclose(ground, c['albedo'], rtol=max(tol, 1e-3), atol=0)
+
+
+def test_mlp_oracle_gradients_by_finite_differences():
+    """The network oracle's backward pass (FP64, no rounding) against central differences of its loss, for every
+    weight and bias of a small 4-layer GELU network; and gelu' against a difference quotient. A transposed operand,
+    a lost GELU derivative or a wrong 2/n factor fails by O(1)."""
+    from oracle import mlp
+
+    rng = np.random.default_rng(5)
+    sizes = [7, 6, 5, 6, 4]
+    params = [(rng.normal(size=(o, i)) / math.sqrt(i), rng.normal(size=o) * 0.1) for i, o in zip(sizes, sizes[1:])]
+    x = rng.normal(size=(9, sizes[0]))
+    t = rng.normal(size=(9, sizes[-1]))
+    wt = rng.uniform(0.5, 2.0, size=sizes[-1])
+    y, cache = mlp.forward(params, x)
+    grads = mlp.backward(params, cache, y, t, wt)
+    h = 1e-6
+    for li, (W, b) in enumerate(params):
+        for arr, g in ((W, grads[li][0]), (b, grads[li][1])):
+            num = np.zeros_like(arr)
+            for idx in np.ndindex(arr.shape):
+                old = arr[idx]
+                arr[idx] = old + h
+                lp = mlp.loss(mlp.forward(params, x)[0], t, wt)
+                arr[idx] = old - h
+                lm = mlp.loss(mlp.forward(params, x)[0], t, wt)
+                arr[idx] = old
+                num[idx] = (lp - lm) / (2 * h)
+            assert np.allclose(g, num, rtol=1e-6, atol=1e-9)
+    z = np.linspace(-4, 4, 81)
+    assert np.allclose(mlp.gelu_grad(z), (mlp.gelu(z + 1e-6) - mlp.gelu(z - 1e-6)) / 2e-6, rtol=1e-7, atol=1e-9)
+    # gelu closed values: gelu(0) = 0, gelu(z) -> z for large z, gelu(-z) = gelu(z) - z
+    assert mlp.gelu(0.0) == 0.0 and abs(mlp.gelu(10.0) - 10.0) < 1e-12
+    assert np.allclose(mlp.gelu(-z), mlp.gelu(z) - z, atol=1e-15)
+
+
+def test_bf16_rounding_ties_to_even():
+    """bf16 keeps the top 16 bits with round-to-nearest-even: 1 + 2^-8 (a tie) rounds to 1, 1 + 3 2^-8 to 1 + 2^-6,
+    and the result always has zero low bits."""
+    from oracle import mlp
+
+    x = np.array([1.0, 1 + 2.0 ** -8, 1 + 3 * 2.0 ** -8, -2.5, 3.0e38, 1e-30], np.float32)
+    r = mlp.bf16(x)
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1 + 2.0 ** -6 and r[3] == -2.5
+    assert np.all((r.view(np.uint32) & 0xFFFF) == 0)
+    assert np.all(np.abs(r - x) <= np.abs(x) * 2.0 ** -8)
+
+
+def test_adam_first_step_moves_by_the_learning_rate():
+    """Adam's bias-corrected first step is lr * g / (|g| + eps): every parameter moves by about lr against its
+    gradient's sign, whatever the gradient's scale."""
+    from oracle import mlp
+
+    p = np.array([1.0, -2.0, 0.5])
+    g = np.array([1e-3, -5.0, 2e2])
+    m, v = np.zeros(3), np.zeros(3)
+    mlp.adam(p, g, m, v, 1, lr=0.01)
+    assert np.allclose(p, [1.0 - 0.01, -2.0 + 0.01, 0.5 - 0.01], rtol=0, atol=1e-7)
