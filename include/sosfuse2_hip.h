@@ -72,6 +72,7 @@ typedef struct sos_bundle {
     const uint8_t *hits_dev;      /* [rays] 1 when the ray ends on the ground */
     const int64_t *node_off_dev;  /* [rays + 1] first quadrature node of every ray (records mode, from the plan) */
     const float *node_geom_dev;   /* [nodes][32] static node geometry (SOS_NODE_GEOMETRY, from sos_plan_geometry) */
+    const float *node_sun_dev;    /* [nodes][12] sun geometry of first light (SOS_NODE_GEOMETRY with SOS_FIRST) */
 } sos_bundle;
 
 typedef struct sos_grid {
@@ -112,7 +113,8 @@ int sos_transport(const sos_grid *grid, const sos_bundle *bundle, int32_t flags,
                   float *out_dev, uint64_t *counters_dev, void *stream);
 /* first light with the ground term read from ground records [nodes][4][lanes] (sos_ground_records) */
 #define SOS_GROUND_RECORDS 32
-/* warm records consumer on the bundle's static node geometry (node_geom_dev; pairs 4, with SOS_RECORDS) */
+/* transport on the bundle's static node geometry (node_geom_dev; pairs 4): the warm records consumer (with
+ * SOS_RECORDS), or first light (with SOS_FIRST and SOS_GROUND_RECORDS; also node_sun_dev) */
 #define SOS_NODE_GEOMETRY 64
 
 /* Visit plan of a bundle (fixed by its geometry): counts_dev [rays] = quadrature nodes of every ray; then with
@@ -127,6 +129,11 @@ int sos_plan_nodes(const sos_grid *grid, const sos_bundle *bundle, const int64_t
  * (the per-node part of numpy_reference.py:294-351 and :236-291, computed once per bundle). */
 int sos_plan_geometry(const sos_grid *grid, const sos_bundle *bundle, const int64_t *node_off_dev, float *geom_dev,
                       void *stream);
+/* Sun geometry of every node for first light, sun_dev [nodes][12] floats: the six bilinear sun-table column
+ * integrals at the node (FP64 lit test near the terminator, numpy_reference.py:431-458), the lit flag and the air,
+ * aerosol and sulfate profiles there. */
+int sos_plan_sun(const sos_grid *grid, const sos_bundle *bundle, const int64_t *node_off_dev, float *sun_dev,
+                 void *stream);
 
 /* First-light ground records: rec[id][w] = asca[w] p_aer(h) sum_p PHt gval[p][w] + ssca[w] p_sul(h) sum_p PHs gval[p][w]
  * over the ground pools of the visited cell (the pool sums of numpy_reference.py:526-558). The lanes fall into

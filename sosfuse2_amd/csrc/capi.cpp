@@ -31,6 +31,7 @@ int transport_dispatch(const TransportArgs &a, int G, int L, hipStream_t s);
 int project_dispatch(const ProjectArgs &a, int G, int L, hipStream_t s);
 int plan_count_launch(const TransportArgs &a, int64_t *counts, hipStream_t s);
 int plan_geometry_launch(const TransportArgs &a, const int64_t *node_off, float *geom, hipStream_t s);
+int plan_sun_launch(const TransportArgs &a, const int64_t *node_off, float *sun, hipStream_t s);
 int plan_nodes_launch(const TransportArgs &a, const int64_t *node_off, int32_t *cell, float *mu, float *cp,
                       float *dir, hipStream_t s);
 int ground_records_dispatch(const GroundArgs &a, hipStream_t s);
@@ -121,6 +122,17 @@ int sos_plan_geometry(const sos_grid *g, const sos_bundle *b, const int64_t *nod
     return plan_geometry_launch(a, node_off_dev, geom_dev, (hipStream_t)stream);
 }
 
+int sos_plan_sun(const sos_grid *g, const sos_bundle *b, const int64_t *node_off_dev, float *sun_dev, void *stream) {
+    if (!g || !b || !node_off_dev || !sun_dev) return arg_error("sos_plan_sun: null argument");
+    if (!g->sun_col_dev || g->sun_nh < 2 || g->sun_nx < 2) return arg_error("sos_plan_sun: no sun table");
+    TransportArgs a;
+    fill_args(a, g, b);
+    a.shape = ProfileShape{1.0f / g->air_H, 1.0f / g->aer_H, g->o3_c, 1.0f / g->o3_s,
+                           g->no2_c, 1.0f / g->no2_s, g->st_c, 1.0f / g->st_s};
+    a.sun_nh = g->sun_nh; a.sun_nx = g->sun_nx; a.sun_h = g->sun_h_dev; a.sun_col = g->sun_col_dev;
+    return plan_sun_launch(a, node_off_dev, sun_dev, (hipStream_t)stream);
+}
+
 int sos_ground_records(const sos_grid *g, int32_t groups, const float *group_g, const float *group_sg, int64_t items,
                        const int32_t *item_cell_dev, const int64_t *item_v0_dev, const int64_t *item_v1_dev,
                        const int32_t *vis_dev, const float *node_dir_dev, float *records_dev, void *stream) {
@@ -207,9 +219,13 @@ int sos_transport(const sos_grid *g, const sos_bundle *b, int32_t flags, int32_t
     a.node_off = b->node_off_dev;
     if (a.ground_records && !a.node_off) return arg_error("sos_transport: ground records need the node offsets");
     if (flags & SOS_NODE_GEOMETRY) {
-        if (first || !records || pairs != 4 || !b->node_geom_dev)
-            return arg_error("sos_transport: node geometry needs the warm records path, pairs 4 and the geometry");
+        if (pairs != 4 || !b->node_geom_dev || !b->node_off_dev)
+            return arg_error("sos_transport: node geometry needs pairs 4, the node offsets and the geometry");
+        if (first && (!a.ground_records || !b->node_sun_dev))
+            return arg_error("sos_transport: first light on node geometry needs ground records and the sun geometry");
+        if (!first && !records) return arg_error("sos_transport: warm node geometry needs the records path");
         a.node_geom = reinterpret_cast<const float4 *>(b->node_geom_dev);
+        a.node_sun = reinterpret_cast<const float4 *>(b->node_sun_dev);
     }
     return transport_dispatch(a, g->lanes, g->L, (hipStream_t)stream);
 }

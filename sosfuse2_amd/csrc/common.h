@@ -40,6 +40,18 @@ __device__ inline float hg(float u, float g) {
     return (1.0f - g * g) / (12.566370614359172f * (den * sqrtf(den)));
 }
 
+// Henyey-Greenstein with the constants of one asymmetry hoisted: c = (1 + g^2, 2 g, (1 - g^2) / (4 pi)); den^-3/2
+// as rsq^3 (v_rsq_f32, about 1 ulp: the value agrees with hg to a few 1e-7 relative) instead of a correctly rounded
+// square root and division. For the inner loops that evaluate many phases of one asymmetry.
+struct HgConst {
+    float a, b, norm;
+};
+__device__ inline HgConst hg_const(float g) { return HgConst{1.0f + g * g, 2.0f * g, (1.0f - g * g) / 12.566370614359172f}; }
+__device__ inline float hg_fast(float u, const HgConst &c) {
+    const float r = rsqrtf(fmaxf(1e-20f, c.a - c.b * u));
+    return c.norm * ((r * r) * r);
+}
+
 struct ProfileShape {
     float inv_air_H, inv_aer_H, o3_c, inv_o3_s, no2_c, inv_no2_s, st_c, inv_st_s;
 };
@@ -92,6 +104,7 @@ struct TransportArgs {
     const int64_t *node_off;  // [rays + 1] first node of every ray (records mode)
     const float *ground_records;  // [nodes][4][G] scattered ground light per (node, corner) (first light)
     const float4 *node_geom;  // [nodes][8] static node geometry (SOS_NODE_GEOMETRY)
+    const float4 *node_sun;   // [nodes][3] sun geometry of first light (SOS_NODE_GEOMETRY with SOS_FIRST)
 };
 
 struct ProjectArgs {

@@ -74,7 +74,7 @@ __global__ __launch_bounds__(G * 4) void ground_records_kernel(GroundArgs a) {
             }
             const int n = min(PC, np - ch * PC);
             for (int grp = 0; grp < a.groups; ++grp) {
-                const float g = a.gg[grp], sg = a.sgg[grp];
+                const HgConst ht = hg_const(a.gg[grp]), hs = hg_const(a.sgg[grp]);
                 for (int e = tid; e < V * PC; e += NT) {
                     const int v = e / PC, p = e % PC;
                     float pt = 0.0f, ps = 0.0f;
@@ -83,8 +83,8 @@ __global__ __launch_bounds__(G * 4) void ground_records_kernel(GroundArgs a) {
                         const float m0 = dv.x * gd[p][2] + dv.y * gd[p][0];
                         const float m1 = dv.z * gd[p][1];
                         const float up = clampf(m0 + m1, -1.0f, 1.0f), dn = clampf(m0 - m1, -1.0f, 1.0f);
-                        pt = (hg(up, g) + hg(dn, g)) / 2.0f;
-                        if (a.strato) ps = (hg(up, sg) + hg(dn, sg)) / 2.0f;
+                        pt = (hg_fast(up, ht) + hg_fast(dn, ht)) * 0.5f;
+                        if (a.strato) ps = (hg_fast(up, hs) + hg_fast(dn, hs)) * 0.5f;
                     }
                     PT[v][p] = pt;
                     PU[v][p] = ps;

@@ -240,7 +240,7 @@ void sh_records_kernel(ShArgs a) {
     __shared__ int sid[V];
     __shared__ floatx4 red[KP > 1 ? NC * 64 : 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wc = wave % NC, kh = wave / NC;
+    const int wc = KP > 1 ? wave % NC : wave, kh = KP > 1 ? wave / NC : 0;  // kh a constant 0 without a K split
     const int corner = wc / NWB, nt = TILE ? (int)blockIdx.y : wc % NWB;
     const int col = nt * 16 + (lane & 15);
     const int64_t item = blockIdx.x;
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(G * 16 * SplitK<L>::KP) void sh_records_split_kerne
     __shared__ int sid[V];
     __shared__ floatx4 red[KP > 1 ? NC * 64 : 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wc = wave % NC, kh = wave / NC;
+    const int wc = KP > 1 ? wave % NC : wave, kh = KP > 1 ? wave / NC : 0;  // kh a constant 0 without a K split
     const int corner = wc / NW, nt = wc % NW;
     const int col = nt * 16 + (lane & 15);
     const int64_t item = blockIdx.x;

@@ -406,6 +406,193 @@ __global__ __launch_bounds__(256) void transport_geom_kernel(TransportArgs a) {
     }
 }
 
+// Sun geometry of every node for first light (one thread per ray), 3 float4 per node:
+//   [0] col[0..3]  [1] col[4], col[5], lit, air profile  [2] aerosol profile, sulfate profile, 0, 0
+// col[k] are the bilinear sun-table column integrals of the six components at the node (FP64 lit test near the
+// terminator, numpy_reference.py:431-458), computed exactly as the per-lane first-light path computes them.
+__global__ void plan_sun_kernel(TransportArgs a, const int64_t *node_off, float4 *sun) {
+    const int64_t ray = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (ray >= a.rays) return;
+    const int grp = a.ray_group[ray];
+    const double p[3] = {a.starts[3 * grp], a.starts[3 * grp + 1], a.starts[3 * grp + 2]};
+    const double d[3] = {a.dirs[3 * ray], a.dirs[3 * ray + 1], a.dirs[3 * ray + 2]};
+    const int64_t e0 = a.edge_off[ray];
+    const int nseg = (int)(a.edge_off[ray + 1] - e0) - 1;
+    const unsigned bits = a.lower[ray];
+    int64_t n = node_off[ray];
+    for (int seg = 0; seg < nseg; ++seg) {
+        const double sa = a.edges[e0 + seg], sb = a.edges[e0 + seg + 1];
+        if (!(sb - sa > 1e-8)) continue;
+        const bool lower = (bits >> seg) & 1u;
+        for (int step = 0; step < a.nq; ++step, ++n) {
+            double t;
+            float ds;
+            node_t(sa, sb, lower, a.gl_u[step], a.gl_w[step], t, ds);
+            double r, qz, qx, qy;
+            node_position(p, d, t, r, qz, qx, qy);
+            const float h = node_height(r, a.top);
+            double height = r - SOS_R_EARTH;
+            if (!(height > 0.0)) height = 0.0;
+            const double rad = SOS_R_EARTH + height;
+            const double mg = -sqrt(height * (2.0 * SOS_R_EARTH + height)) / rad;
+            double delta = qz / r - mg;
+            if (fabs(delta) < (double)SOS_FLT_MIN) delta = 0.0;
+            const bool lit = delta >= 0.0;
+            const float x = clampf((float)sqrt((lit ? delta : 0.0) / (1.0 - mg)), 0.0f, 1.0f);
+            const float hh = clampf(h, 0.0f, a.top);
+            const int si = interval(a.sun_h, a.sun_nh, hh);
+            const float al = (hh - a.sun_h[si]) / (a.sun_h[si + 1] - a.sun_h[si]);
+            const float fx = x * (float)(a.sun_nx - 1);
+            int sj = (int)fx;
+            sj = sj < 0 ? 0 : (sj > a.sun_nx - 2 ? a.sun_nx - 2 : sj);
+            const float bx = fx - (float)sj;
+            const float w00 = (1.0f - al) * (1.0f - bx), w10 = al * (1.0f - bx), w01 = (1.0f - al) * bx, w11 = al * bx;
+            const float *c00 = a.sun_col + ((int64_t)si * a.sun_nx + sj) * 6;
+            const float *c10 = c00 + (int64_t)a.sun_nx * 6;
+            float col[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                float v = w00 * c00[k];
+                v = v + w10 * c10[k];
+                v = v + w01 * c00[6 + k];
+                v = v + w11 * c10[6 + k];
+                col[k] = v;
+            }
+            float prof[5], stp;
+            profiles(a.shape, h, prof, stp);
+            float4 *o = sun + n * 3;
+            o[0] = make_float4(col[0], col[1], col[2], col[3]);
+            o[1] = make_float4(col[4], col[5], lit ? 1.0f : 0.0f, prof[0]);
+            o[2] = make_float4(prof[1], stp, 0.0f, 0.0f);
+        }
+    }
+}
+
+// First light on the static node geometry and sun geometry, with the ground term from the ground records (pairs
+// xx yy zz xz). Same operations as transport_kernel's first-light path with ground records.
+template <int G>
+__global__ __launch_bounds__(256) void transport_first_geom_kernel(TransportArgs a) {
+    constexpr int RPW = 64 / G;
+    constexpr int P = 4;
+    const int lane = threadIdx.x & 63;
+    const int w = lane % G;
+    const int sub = lane / G;
+    float lc[SOS_LANE_CONSTANTS];
+#pragma unroll
+    for (int k = 0; k < SOS_LANE_CONSTANTS; ++k) lc[k] = a.lane[k * G + w];
+    const float mix = lc[SOS_LANE_MIX], gasy = lc[SOS_LANE_G], asca = lc[SOS_LANE_ASCA];
+    const float sext = a.strato ? lc[SOS_LANE_SEXT] : 0.0f;
+    const float ext6[6] = {lc[0], lc[1], lc[2], lc[3], lc[4], sext};
+    const int64_t nwg = gridDim.x;
+    const int xcd = blockIdx.x & 7, idx = blockIdx.x >> 3;
+    const int64_t per_xcd_wg = (nwg + 7 - xcd) >> 3;
+    const int64_t waves_xcd = per_xcd_wg * 4;
+    const int64_t chunk = (a.tiles + 7) >> 3;
+    const int64_t t0 = xcd * chunk, t1 = min(a.tiles, t0 + chunk);
+    const int nc = a.nc;
+    for (int64_t tile = t0 + idx * 4 + (threadIdx.x >> 6); tile < t1; tile += waves_xcd) {
+        const int64_t ray = tile * RPW + sub;
+        if (ray >= a.rays) continue;
+        const int grp = a.ray_group[ray];
+        const float df2 = (float)a.dirs[3 * ray + 2];
+        const float dl[3] = {a.dloc[3 * ray], a.dloc[3 * ray + 1], a.dloc[3 * ray + 2]};
+        float pu[P], direct[P];
+        {
+            const float u0 = a.targets[9 * grp + 6], u1 = a.targets[9 * grp + 7], u2 = a.targets[9 * grp + 8];
+            const float dot = (dl[0] * u0 + dl[1] * u1) + dl[2] * u2;
+            const float e[3] = {u0 - dl[0] * dot, u1 - dl[1] * dot, u2 - dl[2] * dot};
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const int i = PairIdx<P>::i[q], k = PairIdx<P>::k[q];
+                pu[q] = (i == k ? 1.0f : 0.0f) - dl[i] * dl[k];
+                direct[q] = (pu[q] - e[i] * e[k]) / 2.0f;
+            }
+        }
+        const float phase = hg(df2, gasy);
+        const float sphase = a.strato ? hg(df2, lc[SOS_LANE_SG]) : 0.0f;
+        float total[P], corr[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) total[q] = corr[q] = 0.0f;
+        float tau = 0.0f, tc = 0.0f, acc = 0.0f;
+        const int64_t n0 = a.node_off[ray], n1 = a.node_off[ray + 1];
+        for (int64_t n = n0; n < n1; ++n) {
+            const float4 *g = a.node_geom + n * 8;
+            const float4 g0 = g[0], g1 = g[1], g3 = g[3];
+            acc = acc + (((((g0.x * ext6[0] + g0.y * ext6[1]) + g0.z * ext6[2]) + g0.w * ext6[3]) + g1.x * ext6[4]) +
+                         g1.y * ext6[5]);
+            const float mid = (tau - tc) + acc;
+            const float att = g3.x * expf(-mid);
+            const float ah = g3.y, ac = g3.z;
+            const unsigned cb = __float_as_uint(g3.w);
+            const int cell0 = (int)(cb & 0x7fffffffu);
+            const int cells[4] = {cell0, cell0 + 1, cell0 + nc, cell0 + nc + 1};
+            const float sw[4] = {(1.0f - ah) * (1.0f - ac), (1.0f - ah) * ac, ah * (1.0f - ac), ah * ac};
+            const float4 *sg = a.node_sun + n * 3;
+            const float4 s0 = sg[0], s1 = sg[1], s2 = sg[2];
+            const float col[6] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y};
+            float optical = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) optical = optical + col[k] * (k < 5 ? lc[k] : sext);
+            const float sun = s1.z != 0.0f ? expf(-optical) : 0.0f;
+            const float prof0 = s1.w, prof1 = s2.x, stp = s2.y;
+            const float *rec = a.ground_records + n * 4 * G + w;
+            const float grec = ((sw[0] * rec[0] + sw[1] * rec[G]) + sw[2] * rec[2 * G]) + sw[3] * rec[3 * G];
+            const float beam = (prof1 * asca) * (sun * phase + 0.0f);
+            const float ray_ = (sun * prof0) * lc[0];
+            const float sbeam = (stp * lc[SOS_LANE_SSCA]) * (sun * sphase + 0.0f);
+            float val[P];
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                val[q] = ray_ * (mix * 0.119366207318921496f * direct[q] +
+                                 (1.0f - mix) * 0.0397887357729738339f * pu[q]) +
+                         beam * pu[q] / 2.0f;
+                if (a.strato) val[q] = val[q] + (sbeam * pu[q]) / 2.0f;
+                val[q] = val[q] + (grec * pu[q]) / 2.0f;
+            }
+            float T[P][4];
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const float4 t4 = g[4 + q];
+                T[q][0] = t4.x; T[q][1] = t4.y; T[q][2] = t4.z; T[q][3] = t4.w;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float *gm = a.gm + (int64_t)cells[c] * 4 * G + w;
+                const float rm[4] = {gm[0], gm[G], gm[2 * G], gm[3 * G]};
+                const float trace = (rm[0] + rm[1]) + rm[2];
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    float rot = 0.0f;
+#pragma unroll
+                    for (int uu = 0; uu < 4; ++uu) rot = rot + T[q][uu] * rm[uu];
+                    const float tensor = prof0 * (mix * rot + ((1.0f - mix) / 3.0f * trace) * pu[q]);
+                    val[q] = val[q] + sw[c] * tensor;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const float y = val[q] * att - corr[q];
+                const float tn = total[q] + y;
+                corr[q] = (tn - total[q]) - y;
+                total[q] = tn;
+            }
+            if (cb & 0x80000000u) {
+                const float4 g2 = g[2];
+                acc = acc + (((((g2.x * ext6[0] + g2.y * ext6[1]) + g2.z * ext6[2]) + g2.w * ext6[3]) + g1.z * ext6[4]) +
+                             g1.w * ext6[5]);
+                const float y = acc - tc;
+                const float tn = tau + y;
+                tc = (tn - tau) - y;
+                tau = tn;
+                acc = 0.0f;
+            }
+        }
+        float *o = a.out + (ray * G + w) * P;
+#pragma unroll
+        for (int q = 0; q < P; ++q) o[q] = total[q];
+    }
+}
+
 enum Aero { DIRECT = 0, RECORDS = 1 };
 
 template <int G, int L, int P, bool FIRST, bool BOUNDARY, int AERO, bool UNI = false>
@@ -814,7 +1001,7 @@ static int launch_one(const TransportArgs &a, hipStream_t s) {
     return SOS_OK;
 }
 
-template <int G, bool BOUNDARY>
+template <int G, bool BOUNDARY, bool FIRST>
 static int launch_geom(const TransportArgs &a, hipStream_t s) {
     int dev = 0, cus = 256;
     hipGetDevice(&dev);
@@ -824,7 +1011,8 @@ static int launch_geom(const TransportArgs &a, hipStream_t s) {
     if (grid > wgs_needed) grid = wgs_needed;
     grid = ((grid + 7) / 8) * 8;
     if (grid < 8) grid = 8;
-    hipLaunchKernelGGL((transport_geom_kernel<G, BOUNDARY>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    if constexpr (FIRST) hipLaunchKernelGGL((transport_first_geom_kernel<G>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((transport_geom_kernel<G, BOUNDARY>), dim3((unsigned)grid), dim3(256), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error("transport launch: %s", hipGetErrorString(e));
     return SOS_OK;
@@ -835,11 +1023,15 @@ template <int G, int P>
 static int launch_flags(const TransportArgs &a, int L, hipStream_t s) {
     const bool first = a.flags & SOS_FIRST, bnd = a.flags & SOS_BOUNDARY;
     const bool records = a.flags & SOS_RECORDS;
+    if (first && (a.flags & SOS_NODE_GEOMETRY)) {
+        if constexpr (P == 4) return launch_geom<G, false, true>(a, s);
+        else return set_error("node geometry: pairs must be 4");
+    }
     if (first)
         return (a.flags & SOS_UNIFORM_PHASE) ? launch_one<G, 8, P, true, false, RECORDS, true>(a, s)
                                              : launch_one<G, 8, P, true, false, RECORDS, false>(a, s);
     if (records && (a.flags & SOS_NODE_GEOMETRY)) {
-        if constexpr (P == 4) return bnd ? launch_geom<G, true>(a, s) : launch_geom<G, false>(a, s);
+        if constexpr (P == 4) return bnd ? launch_geom<G, true, false>(a, s) : launch_geom<G, false, false>(a, s);
         else return set_error("node geometry: pairs must be 4");
     }
     if (records)
@@ -886,6 +1078,13 @@ int plan_nodes_launch(const TransportArgs &a, const int64_t *node_off, int32_t *
                            reinterpret_cast<float4 *>(dir));
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? SOS_OK : set_error("plan_nodes: %s", hipGetErrorString(e));
+}
+
+int plan_sun_launch(const TransportArgs &a, const int64_t *node_off, float *sun, hipStream_t s) {
+    const unsigned b = (unsigned)((a.rays + 255) / 256);
+    if (b) hipLaunchKernelGGL(plan_sun_kernel, dim3(b), dim3(256), 0, s, a, node_off, reinterpret_cast<float4 *>(sun));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SOS_OK : set_error("plan_sun: %s", hipGetErrorString(e));
 }
 
 int plan_geometry_launch(const TransportArgs &a, const int64_t *node_off, float *geom, hipStream_t s) {

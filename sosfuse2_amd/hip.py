@@ -17,7 +17,7 @@ class Bundle(ctypes.Structure):
         ('groups', c_i64), ('rays', c_i64), ('group_off_dev', P), ('starts_dev', P), ('targets_dev', P),
         ('ray_group_dev', P), ('dirs_dev', P), ('dloc_dev', P), ('weights_dev', P), ('edge_off_dev', P),
         ('edges_dev', P), ('lower_dev', P), ('end_j_dev', P), ('end_f_dev', P), ('end_col_dev', P), ('hits_dev', P),
-        ('node_off_dev', P), ('node_geom_dev', P),
+        ('node_off_dev', P), ('node_geom_dev', P), ('node_sun_dev', P),
     ]
 
 
@@ -37,6 +37,7 @@ _SIGNATURES = {
     'sos_plan_count': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, P],
     'sos_plan_nodes': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, P, P, P, P, P],
     'sos_plan_geometry': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, P, P],
+    'sos_plan_sun': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, P, P],
     'sos_ground_records': [ctypes.POINTER(Grid), c_i32, ctypes.POINTER(c_f32), ctypes.POINTER(c_f32), c_i64, P, P, P, P,
                            P, P, P],
     'sos_sh_records': [ctypes.POINTER(Grid), P, P, c_i32, c_i64, P, P, P, P, P, P, P, P],

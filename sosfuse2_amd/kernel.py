@@ -61,7 +61,8 @@ class DeviceBundle:
         self.struct = hip.Bundle(
             self.groups, self.rays, *(hip.ptr(x) for x in (
                 self.group_off, self.starts, self.targets, self.ray_group, self.dirs, self.dloc, self.weights,
-                self.edge_off, self.edges, self.lower, self.end_j, self.end_f, self.end_col, self.hits)), None, None)
+                self.edge_off, self.edges, self.lower, self.end_j, self.end_f, self.end_col, self.hits)), None, None,
+            None)
 
     def nbytes(self):
         return sum(x.numel() * x.element_size() for x in (
@@ -109,6 +110,7 @@ class Plan:
         self.visits = 4 * N
         self.records = torch.empty((max(4 * N, 1), dc.G), dtype=torch.float32, device=dev)
         self.geom = None  # static node geometry of the warm consumer, built on first use (geometry())
+        self.sun = None  # sun geometry of first light, built on first use (sun_geometry())
 
     def geometry(self, kernel, dc, db):
         """[nodes, 32] static node geometry (sos_plan_geometry), built once."""
@@ -118,6 +120,14 @@ class Plan:
                      kernel.stream())
             db.struct.node_geom_dev = hip.ptr(self.geom)
         return self.geom
+
+    def sun_geometry(self, kernel, dc, db):
+        """[nodes, 12] sun geometry of first light (sos_plan_sun), built once."""
+        if self.sun is None:
+            self.sun = torch.empty((max(self.nodes, 1), 12), dtype=torch.float32, device=self.node_off.device)
+            hip.call('sos_plan_sun', dc.struct, db.struct, hip.ptr(self.node_off), hip.ptr(self.sun), kernel.stream())
+            db.struct.node_sun_dev = hip.ptr(self.sun)
+        return self.sun
 
     def set_node_chunk(self, node_chunk):
         """Work items of the SH records: runs of at most node_chunk nodes of one base cell (a tuning knob; the
@@ -306,6 +316,10 @@ class Kernel:
             if e0 is not None:
                 self.timed.append(('ground_records', e0, self._event()))
             flags |= hip.GROUND_RECORDS
+            if pairs == 4 and self.node_geometry:
+                pl.geometry(self, dc, db)
+                pl.sun_geometry(self, dc, db)
+                flags |= hip.NODE_GEOMETRY
         name = 'first' if first else ('warm' if boundary is not None else
                                       ('query' if pairs == 6 or db.groups != ctx.geo.nc else 'ground'))
         if not first:

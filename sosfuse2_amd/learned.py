@@ -9,8 +9,11 @@ the HIP kernels of libsosfuse2_hip.so (sos_mlp_*); PyTorch holds the buffers and
 Samples are (cell, lane) pairs. Inputs: the cell's first-light source rows (tensor rows, SH x Rayleigh rows and the
 SH planes) divided by the lane's scale s = max over cells of the first-light tensor trace, and 16 physical features
 (height, cos of the solar-zenith angle, log local extinction of the six components, the lane's asymmetries and
-albedo, log vertical optical depths of Rayleigh, aerosol, absorbers and sulfate, log s). Target: the converged
-source rows divided by the same s.
+albedo, log vertical optical depths of Rayleigh, aerosol, absorbers and sulfate, log s). Target: the multiple-
+scattering increment (converged - first-light source) / s. Inputs and targets are standardised per row with
+statistics of the training set (kept with the network), so that small rows (the xz tensor row, high SH orders)
+weigh in the loss as much as large ones. The warm start is x0 = first + increment in sunlit cells, with the tensor
+diagonal kept at or above the first-light value (multiple scattering only adds light), and 0 in shadow cells.
 """
 
 import math
@@ -48,6 +51,7 @@ class Network:
         self.m = torch.zeros_like(self.flat)
         self.v = torch.zeros_like(self.flat)
         self.steps = 0
+        self.in_mu = self.in_sd = self.out_sd = None  # per-row standardisation of inputs and targets
         self._work = None
         self._lossbuf = torch.zeros(1, dtype=torch.float32, device=self.device)
         self._lwork = torch.zeros(1024, dtype=torch.float64, device=self.device)
@@ -67,11 +71,20 @@ class Network:
         return self._param(self.flat if buf is None else buf, 2 * i + 1)
 
     def state_dict(self):
-        return dict(sizes=self.sizes, flat=self.flat.cpu().numpy(), steps=self.steps)
+        return dict(sizes=self.sizes, flat=self.flat.cpu().numpy(), steps=self.steps, in_mu=self.in_mu.cpu().numpy(),
+                    in_sd=self.in_sd.cpu().numpy(), out_sd=self.out_sd.cpu().numpy())
 
     def load(self, d):
         assert list(d['sizes']) == self.sizes
         self.flat.copy_(torch.from_numpy(np.asarray(d['flat'], 'f4')))
+        for k in ('in_mu', 'in_sd', 'out_sd'):
+            setattr(self, k, torch.from_numpy(np.asarray(d[k], 'f4')).to(self.device))
+
+    def fit_normalization(self, X, T):
+        """Per-row mean / deviation of the inputs and deviation of the targets over a training set."""
+        self.in_mu = X.mean(0)
+        self.in_sd = X.std(0).clamp_min(1e-6)
+        self.out_sd = T.pow(2).mean(0).sqrt().clamp_min(1e-6)
 
     def _stream(self):
         return torch.cuda.current_stream(self.device).cuda_stream
@@ -143,9 +156,11 @@ class Network:
                  hip.ptr(self.v), self.steps, lr, beta1, beta2, eps, self._stream())
 
     def predict(self, x, batch=65536):
+        """Targets in physical units for raw inputs (the standardisation is applied and undone here)."""
         out = torch.empty((x.shape[0], self.sizes[-1]), dtype=torch.float32, device=self.device)
         for i in range(0, x.shape[0], batch):
-            out[i : i + batch] = self.forward(x[i : i + batch])
+            xb = ((x[i : i + batch] - self.in_mu) / self.in_sd).contiguous()
+            out[i : i + batch] = self.forward(xb) * self.out_sd
         return out
 
 
@@ -209,7 +224,8 @@ def sizes_for(rows, width=256):
 
 
 def samples(s, solver, x, cells_per_lane=None, rng=None):
-    """Training pairs (inputs, targets) of one solved context: x the converged source."""
+    """Training pairs (inputs, targets) of one solved context: x the converged source; the targets are the
+    increments (x - first) / s."""
     first = first_source(solver)
     C = s.geo.ns
     if cells_per_lane is not None and cells_per_lane < C:
@@ -218,18 +234,21 @@ def samples(s, solver, x, cells_per_lane=None, rng=None):
         cells = np.arange(C)
     X, scale = features(s, first, cells=cells)
     cd = torch.as_tensor(cells, device=x.device)
-    T = torch.cat([x[cd, :, w] / scale[w] for w in range(s.W)], 0).contiguous()
+    T = torch.cat([(x[cd, :, w] - first[cd, :, w]) / scale[w] for w in range(s.W)], 0).contiguous()
     return X, T
 
 
 def train(net, X, T, steps, batch=4096, lr=1e-3, seed=0, log=None):
-    """Adam on random minibatches; returns the loss history (one value per 50 steps)."""
+    """Adam on random minibatches of the standardised data; returns the loss history (one value per 50 steps)."""
+    if net.in_mu is None:
+        net.fit_normalization(X, T)
     g = torch.Generator(device='cpu').manual_seed(seed)
     n = X.shape[0]
     hist = []
     for step in range(steps):
         idx = torch.randint(0, n, (min(batch, n),), generator=g).to(X.device)
-        xb, tb = X[idx].contiguous(), T[idx].contiguous()
+        xb = ((X[idx] - net.in_mu) / net.in_sd).contiguous()
+        tb = (T[idx] / net.out_sd).contiguous()
         y = net.forward(xb, keep=True)
         loss, dy = net.loss_and_grad(y, tb)
         net.backward(dy)
@@ -241,9 +260,12 @@ def train(net, X, T, steps, batch=4096, lr=1e-3, seed=0, log=None):
     return hist
 
 
-def warm_start(net, s, solver):
-    """x0 [cells, rows, G] for Solver.solve: the network's converged-source estimate, lanes beyond W repeat the
-    last lane (the padding rule of the lane table)."""
+def warm_start(net, s, solver, lit=1e-3):
+    """x0 [cells, rows, G] for Solver.solve: in sunlit cells (first-light trace >= lit x the lane's largest) the
+    first-light source + the network's multiple-scattering increment, with the tensor diagonal kept at or above its
+    first-light value; elsewhere 0, the cold start's value. Shadow cells are left out because their traces are tiny:
+    the log-space corner blend magnifies a relative error there, and a guess the iteration must undo costs more
+    evaluations than it saves. Lanes beyond W repeat the last lane (the padding rule of the lane table)."""
     first = first_source(solver)
     X, scale = features(s, first)
     Y = net.predict(X)
@@ -251,5 +273,8 @@ def warm_start(net, s, solver):
     x0 = torch.empty_like(first)
     for w in range(first.shape[2]):
         wl = min(w, W - 1)
-        x0[:, :, w] = Y[wl * C : (wl + 1) * C] * scale[wl]
-    return x0
+        x0[:, :, w] = first[:, :, wl] + Y[wl * C : (wl + 1) * C] * scale[wl]
+    x0[:, 0:3] = torch.maximum(x0[:, 0:3], first[:, 0:3])
+    trace = first[:, 0:3].sum(1)
+    sunlit = trace.abs() >= lit * trace.abs().amax(0, keepdim=True)
+    return torch.where(sunlit[:, None, :], x0, torch.zeros_like(x0))

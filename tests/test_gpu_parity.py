@@ -469,9 +469,10 @@ def test_autotuner_keeps_results(gpu):
 
 @pytest.mark.parametrize('W,L', [(3, 16), (20, 32)])
 def test_node_geometry_consumer_matches_per_lane_path(gpu, W, L):
-    """The warm consumer on the plan's static node geometry (Simpson panels as profile weights, rotation, corners and
-    weights computed once per bundle) against the per-lane consumer that recomputes them at every node: the
-    same radiances to FP32 rounding (the Simpson panels sum in another order)."""
+    """The warm consumer and first light on the plan's static node and sun geometry (Simpson panels as profile
+    weights, rotation, corners and weights, sun-table columns computed once per bundle) against the per-lane kernels
+    that recompute them at every node: the same radiances to FP32 rounding (the Simpson panels sum in another
+    order)."""
     s, scope, _ = setup_case(W, L)
     k = s.kernel
     solver = scope.solver(s, log=None)
@@ -481,8 +482,11 @@ def test_node_geometry_consumer_matches_per_lane_path(gpu, W, L):
     bnd = torch.from_numpy(np.abs(np.random.default_rng(2).normal(size=(s.geo.nc, dc.G))).astype('f4') * 1e-3).cuda()
     k.node_geometry = True
     a = k.radiance(db, s, f, False, bnd, 4).cpu().numpy()
+    fa = k.radiance(db, s, None, True, None, 4).cpu().numpy()
     k.node_geometry = False
     b = k.radiance(db, s, f, False, bnd, 4).cpu().numpy()
+    fb = k.radiance(db, s, None, True, None, 4).cpu().numpy()
     k.node_geometry = True
-    assert db.plan.geom is not None
+    assert db.plan.geom is not None and db.plan.sun is not None
     assert rel(a, b) < 2e-6
+    assert rel(fa, fb) < 2e-6
