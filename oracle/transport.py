@@ -20,6 +20,15 @@ Deliberate differences from the reference, each shared with the HIP kernels (so 
   the closed Henyey-Greenstein form (the reference's 33-term Legendre sum of an HG layer differs by < g^33);
 - phase 2 may return six tensor components (xx, yy, zz, xz, xy, yz) so that Stokes Q and U exist; the reference
   forms the first four only (`numpy_reference.py:14`).
+
+What pins each part (tests/test_oracle_pins.py): `Simpson` with `sample`'s path coordinates and weights by fourth-order
+convergence to an FP64 integral and the attenuation closure; `sun_columns` by FP64 path integrals toward the sun;
+`hg`, the log-space blend, the SH basis and cos(m phi) by closed forms; `project` by the projection of isotropic light.
+The first-light assembly of `_transport` (sampling, attenuation, sun columns, the Rayleigh tensor with
+depolarisation) is pinned by the textbook single-scattering integral of a pure Rayleigh atmosphere. The warm
+assembly (the corner blend of the source tensor and SH values) is parity unpinned against reference outputs: the
+reference's own golden tables need its atmosphere and geometry inputs, which only an import of the reference could
+produce; it is pinned through its parts and by the FP64 twin.
 """
 
 import math

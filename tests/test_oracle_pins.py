@@ -390,3 +390,123 @@ def test_adam_first_step_moves_by_the_learning_rate():
     m, v = np.zeros(3), np.zeros(3)
     mlp.adam(p, g, m, v, 1, lr=0.01)
     assert np.allclose(p, [1.0 - 0.01, -2.0 + 0.01, 0.5 - 0.01], rtol=0, atol=1e-7)
+
+
+def _sun_truth(r, mu, c, top):
+    """FP64 column integrals of the six unit profiles from a point at radius r along a direction of cosine mu to the
+    z axis (the sun), up to the top of the atmosphere (scipy quad, split at the closest approach)."""
+    from scipy.integrate import quad
+
+    R = 6371.0
+    s_top = -r * mu + math.sqrt(r * r * mu * mu - r * r + (R + top) ** 2)
+    st = math.sqrt(max(0.0, 1 - mu * mu))
+    split = [s for s in (-r * mu,) if 0 < s < s_top]
+
+    def height(s):
+        return math.hypot(r * st * 0 + s * st, r + s * mu) - R
+
+    def prof(s, k):
+        h = min(max(height(s), 0.0), top)
+        p, sp = ot._profiles(np.array([h]), c, np.float64)
+        return float(p[0, k]) if k < 5 else float(sp[0])
+
+    return np.array([quad(prof, 0.0, s_top, args=(k,), points=split or None, limit=400, epsabs=0, epsrel=1e-10)[0]
+                     for k in range(6)])
+
+
+def test_sun_columns_against_fp64_path_integrals():
+    """The oracle's sun columns (table lookup of sun_columns, numpy_reference.py:431-458, on the host-built sun
+    table) against FP64 path integrals of the same profiles toward the sun: at table nodes the only error is the
+    table's quadrature (< 1e-6 relative); between nodes the bilinear interpolation of the 81 x 401 table (up to 13 %
+    on the small columns of grazing paths near the top, so the check there is 25 %). A swapped component, a wrong
+    solar coordinate x or lit flag fails by O(1)."""
+    ctx, scope = _cpu_context()
+    c = ctx.constants(ctx.geo.path_n, np.float64)
+    top = c['top']
+    R = 6371.0
+    ops = ot._Ops(np.float64)
+    sh, scol = c['sun_h'], c['sun_col']
+    nx = scol.shape[1]
+    rng = np.random.default_rng(4)
+    # table nodes
+    for i in (0, 5, len(sh) // 3, len(sh) - 3):
+        for j in (1, nx // 7, nx // 2, nx - 1):
+            r = R + float(sh[i])
+            mg = -math.sqrt(max(0.0, 1 - (R / r) ** 2))
+            x = j / (nx - 1)
+            mu = mg + (1 - mg) * x * x
+            q = dict(h=np.array([sh[i]]), r=(np.array([r]), np.zeros(1)), qz=(np.array([mu * r]), np.zeros(1)))
+            col, lit = ot.sun_columns(q, c, ops)
+            assert lit[0]
+            truth = _sun_truth(r, mu, c, top)
+            big = np.abs(truth) > 1e-9 * np.abs(truth).max()
+            assert np.allclose(col[0][big], truth[big], rtol=1e-6, atol=0), (i, j)
+    # between nodes
+    for _ in range(12):
+        h = float(rng.uniform(0.5, top - 1))
+        r = R + h
+        mg = -math.sqrt(max(0.0, 1 - (R / r) ** 2))
+        mu = float(mg + (1 - mg) * rng.uniform(0.05, 1.0) ** 2)
+        q = dict(h=np.array([h]), r=(np.array([r]), np.zeros(1)), qz=(np.array([mu * r]), np.zeros(1)))
+        col, lit = ot.sun_columns(q, c, ops)
+        truth = _sun_truth(r, mu, c, top)
+        big = np.abs(truth) > 1e-3 * np.abs(truth).max()
+        assert lit[0]
+        assert np.allclose(col[0][big], truth[big], rtol=0.25, atol=0), (h, mu)
+    # below the horizon: not lit
+    r = R + 10.0
+    mg = -math.sqrt(1 - (R / r) ** 2)
+    q = dict(h=np.array([10.0]), r=(np.array([r]), np.zeros(1)), qz=(np.array([(mg - 0.01) * r]), np.zeros(1)))
+    assert not ot.sun_columns(q, c, ops)[1][0]
+
+
+def test_first_light_rayleigh_single_scattering():
+    """The first-light transport assembled by the oracle (node sampling, Simpson attenuation, sun columns, the
+    Rayleigh tensor with depolarisation, the corner blend) against the textbook single-scattering integral of a
+    pure Rayleigh atmosphere, evaluated independently in FP64:
+        I = int_0^L beta_R(s) exp(-tau_view(0, s)) exp(-tau_sun(s)) [mix 3 (1 + cos^2) / (16 pi) + (1 - mix) / (4 pi)] ds
+    with cos = d_z (the sun on the z axis), for rays of cells in full sun. The trace xx + yy + zz of the oracle's
+    tensor radiance must agree to 5e-3 (its node quadrature and the sun table's interpolation); a lost factor,
+    a wrong depolarisation weight or a missing attenuation fails by O(1)."""
+    from scipy.integrate import quad
+
+    from sosfuse2_amd import atmosphere
+
+    st = atmosphere.State(aod550=0.0, ozone_du=0.0, no2_column=0.0, o4=False, sulfate_aod550=0.0, albedo=0.0)
+    ctx, scope = _cpu_context(state=st, path_n=8, nmu=2, nphi=4, wavelengths=(550.0,))
+    c = ctx.constants(8, np.float64)
+    extR = float(c['ext'][0][0])
+    mix = float(c['mix'][0])
+    top, R, H = c['top'], 6371.0, st.air_height_km
+    air = lambda x: math.exp(-min(max(np.linalg.norm(x) - R, 0.0), top) / H)
+    nc = ctx.geo.nc
+    checked = 0
+    for ih in (1, 3):
+        for ic in (0, 1):  # chi = 0 and 60 degrees: every node sunlit
+            g = scope.packs[ih * nc + ic]
+            pick = np.linspace(0, len(g['d']) - 1, 5).astype(int)
+            sub = {k: (v[pick] if isinstance(v, np.ndarray) and v.shape[:1] == (len(g['d']),) else v)
+                   for k, v in g.items()}
+            tens = ot.transport(sub, c, np.zeros((ctx.geo.ns, ctx.rows, 1)), True)
+            trace = tens[:, 0, 0] + tens[:, 0, 1] + tens[:, 0, 2]
+            for r in range(len(pick)):
+                p, d = sub['p'][r], sub['d'][r]
+                L = float(sub['edges'][r][-1])
+                if L <= 0:
+                    continue
+
+                def sun_col(x):
+                    rr = np.linalg.norm(x)
+                    mu = x[2] / rr
+                    s_top = -rr * mu + math.sqrt(rr * rr * mu * mu - rr * rr + (R + top) ** 2)
+                    return quad(lambda t: air(x + t * np.array([0, 0, 1.0])), 0, s_top, limit=200,
+                                epsrel=1e-10)[0]
+
+                phase = mix * 3 * (1 + d[2] ** 2) / (16 * math.pi) + (1 - mix) / (4 * math.pi)
+                f = lambda s: extR * air(p + s * d) * math.exp(
+                    -extR * quad(lambda u: air(p + u * d), 0, s, limit=200, epsrel=1e-10)[0]
+                    - extR * sun_col(p + s * d)) * phase
+                truth = quad(f, 0, L, limit=200, epsrel=1e-8)[0]
+                assert abs(trace[r] - truth) <= 5e-3 * truth, (ih, ic, r, trace[r], truth)
+                checked += 1
+    assert checked >= 12
