@@ -176,6 +176,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--config', default='config1', choices=['config0', 'config1'])
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-tune', action='store_true', help='keep the default launch shapes (no autotuning)')
     ap.add_argument('--mixed', action='store_true', help='headline in the mixed-precision form (split-FP16 SH)')
     ap.add_argument('--variants', default='mixed,batched',
                     help='forms measured beside the headline: mixed, batched (comma list, or "none")')
@@ -205,6 +206,14 @@ def main():
     torch.cuda.synchronize()
     prep = time.perf_counter() - t0
     calibration_copy()
+    tuning = None
+    if not args.no_tune:
+        from sosfuse2_amd import tune
+
+        t = time.perf_counter()
+        best, timings = tune.autotune(solver, node_chunks=(512, 1024, 2048), blocks_per_cu=(8, 16), log=log)
+        tuning = dict(best, seconds=round(time.perf_counter() - t, 1), candidates=len(timings))
+        log(f'autotuned in {tuning["seconds"]} s: {best}')
     log(f'prepared in {prep:.1f} s: {s.geo.ns} cells, {solver.geometry.bundle.rays} rays, '
         f'{solver.geometry.bundle.samples} nodes per evaluation, W={s.W}')
     tol = float(cfg['anderson_tol'])
@@ -268,7 +277,8 @@ def main():
         config=dict(workload=f'{args.config}: {"mixed FP16 SH" if args.mixed else "plain FP32"} HIP, {s.geo.nh} levels x {s.geo.nc} chi, '
                              f'{s.W} wavelengths, L={k.L}, nmu={s.geo.nmu} nphi={s.geo.nphi} path_n={s.geo.path_n}, '
                              f'{len(hs)} solar elevations x {len(el)} x {len(az)} views, I/Q/U, tol {tol:g}',
-                    evaluations=evals, rays=solver.geometry.bundle.rays, prepare_seconds=round(prep, 1)),
+                    evaluations=evals, rays=solver.geometry.bundle.rays, prepare_seconds=round(prep, 1),
+                    tuning=tuning),
         roofline=roofline,
         launch_seconds_per_step=launch_s,
     )

@@ -320,14 +320,15 @@ void sh_records_kernel(ShArgs a) {
     }
 }
 
-// The split kernel divides the K range over KP wavefronts per (corner, N-tile) (KP = 2 at L = 32), so that each
-// keeps only its part of the hi / lo B operands in registers (4 x G/16 x KP wavefronts, at most 128 VGPRs each);
-// the parts are summed through LDS after each round. The basis rows are generated once per node.
+// The split kernel runs 4 x G/16 wavefronts, wavefront (corner, N-tile) holding the whole K range of its hi / lo B
+// operands (144 VGPRs at L = 32 in the quad layout, about 240 in all: two wavefronts per SIMD); the basis rows are
+// generated once per node. (KP > 1 would split K over wavefronts and sum the parts through LDS.)
 template <int L>
-struct SplitK { static constexpr int KP = L >= 32 ? 2 : 1; };
+struct SplitK { static constexpr int KP = 1; };
 
 template <int G, int L>
-__global__ __launch_bounds__(G * 16 * SplitK<L>::KP) void sh_records_split_kernel(ShArgs a) {
+__global__ __launch_bounds__(G * 16 * SplitK<L>::KP) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void sh_records_split_kernel(ShArgs a) {
     constexpr int K = (L + 1) * (L + 2) / 2;
     constexpr int KZ = (K + 15) / 16 * 16;
     constexpr int KS = KZ / 16;               // MFMA k-steps of 16
@@ -422,7 +423,6 @@ __global__ __launch_bounds__(G * 16 * SplitK<L>::KP) void sh_records_split_kerne
                 am = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh[i], am, 0, 0, 0);
                 ac = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl[i], ac, 0, 0, 0);
                 ac = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh[i], ac, 0, 0, 0);
-                __builtin_amdgcn_sched_barrier(0);  // one step at a time: keeps the LDS operands out of registers
             }
         }
         floatx4 d = am + ac * (1.0f / 2048.0f);

@@ -41,7 +41,23 @@ class Geometry:
 
 
 def observer_bundle(s, hs, el, az, kernel=None):
-    """One device bundle of the observer rays of several solar elevations (packs of 64 rays, as engine_core.query)."""
+    """One device bundle of the observer rays of several solar elevations (packs of 64 rays, as engine_core.query).
+    The bundle depends on the grid and the view axes only, so it is kept per kernel and reused by every state."""
+    k = kernel or s.kernel
+    el, az = np.asarray(el, 'f8'), np.asarray(az, 'f8')
+    g = s.geo
+    key = (np.asarray(g.h).tobytes(), np.asarray(g.chi).tobytes(), g.query_n, float(g.domain.top_km),
+           float(s.observer_km), tuple(float(h) for h in hs), el.shape, el.tobytes(), az.tobytes())
+    cache = k.__dict__.setdefault('_observer_bundles', {})
+    if key in cache:
+        return cache[key]
+    if len(cache) >= 16:
+        cache.pop(next(iter(cache)))
+    cache[key] = _observer_bundle(s, hs, el, az, k)
+    return cache[key]
+
+
+def _observer_bundle(s, hs, el, az, k):
     packs = []
     sizes = []
     for h in hs:
@@ -51,7 +67,7 @@ def observer_bundle(s, hs, el, az, kernel=None):
     host = geometry.Bundle(packs, end_columns=False)
     host.sizes = sizes
     host.view = np.concatenate([x['view'] for x in packs])
-    return host, (kernel or s.kernel).bundle(host)
+    return host, k.bundle(host)
 
 
 class Solver:
