@@ -98,8 +98,9 @@ typedef struct sos_grid {
     const float *gm_dev;       /* [cells][4][lanes] Rayleigh tensor of the ground-reflected sun */
     int32_t blocks_per_cu;     /* transport launch: persistent workgroups per CU (0: 8); a tuning knob, results do
                                   not depend on it */
-    int32_t sh_tile;           /* sos_sh_records FP32 workgroup shape: 0 all lanes of a node run per workgroup,
-                                  1 one 16-lane tile per workgroup; a tuning knob, results do not depend on it */
+    int32_t sh_tile;           /* sos_sh_records FP32 workgroup form: 0 flow (double-buffered basis tiles, one
+                                  barrier per round of 16 nodes), 1 stepwise (seeds, basis and products separated by
+                                  barriers); a tuning knob, the records are bitwise the same */
 } sos_grid;
 
 /* Radiance tensor of every ray: out_dev [rays][lanes][pairs] (pairs 4: xx yy zz xz; 6: + xy yz), FP32.

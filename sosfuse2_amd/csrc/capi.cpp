@@ -172,7 +172,7 @@ int sos_sh_records(const sos_grid *g, const void *plane_dev, const float *scale_
     a.item_cell = item_cell_dev; a.item_v0 = item_v0_dev; a.item_v1 = item_v1_dev; a.node = node_dev;
     a.node_mu = node_mu_dev; a.node_cp = node_cp_dev; a.records = records_dev; a.scale = scale_dev;
     a.plane_lo = (int64_t)g->nh * g->nc * ((a.K + 15) / 16 * 16) * a.G;
-    a.tile = g->sh_tile != 0;
+    a.tile = g->sh_tile;
     return sh_records_dispatch(a, half != 0, (hipStream_t)stream);
 }
 

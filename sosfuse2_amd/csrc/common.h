@@ -127,7 +127,7 @@ struct ShArgs {
     float *records;            // [nodes][4][G]: corner c of node n at (4 n + c) G
     const float *scale;        // [cells][G] scale of a split FP16 plane (mixed precision)
     int64_t plane_lo;          // offset (elements) of the lo plane after the hi plane: cells * KZ * G
-    int tile;                  // FP32 workgroup shape: one 16-lane tile per workgroup (1) or all lanes (0)
+    int tile;                  // FP32 workgroup form: stepwise (1) or flow (0)
 };
 
 struct GroundArgs {

@@ -1,9 +1,9 @@
 """Autotuner of the launch shapes: the SH-records work-item size (nodes of one base cell per workgroup), the FP32
-SH-records workgroup shape (all lanes, or one 16-lane tile with the K range split over two wavefronts) and the
-transport's persistent workgroups per CU. None changes a result (every record and every ray is computed the same
-way whatever the shape), so the tuner times one source evaluation per candidate on the solver's own grid and lane
-count and keeps the fastest. Results are cached per (grid, L, lanes, precision) in memory and, when a path is
-given, in a JSON file.
+SH-records workgroup form (flow: double-buffered basis tiles with one barrier per round; stepwise: seeds, basis and
+products separated by barriers) and the transport's persistent workgroups per CU. None changes a result (every
+record and every ray is computed the same way whatever the shape), so the tuner times one source evaluation per
+candidate on the solver's own grid and lane count and keeps the fastest. Results are cached per (grid, L, lanes,
+precision) in memory and, when a path is given, in a JSON file.
 """
 
 import json
