@@ -146,18 +146,25 @@ int sos_ground_records(const sos_grid *grid, int32_t groups, const float *group_
                        const int32_t *item_cell_dev, const int64_t *item_v0_dev, const int64_t *item_v1_dev,
                        const int32_t *vis_dev, const float *node_dir_dev, float *records_dev, void *stream);
 
+/* Basis table of a bundle (fixed by its geometry): basis_dev [rays][KZ] (KZ = K rounded up to a multiple of 16)
+ * holds fp32(x_m P_lm(mu)) in the m-major K order k' = m (L + 1) - m (m - 1) / 2 + l - m, zero for K <= k' < KZ,
+ * with mu = fp32 of the ray direction's component along the sun axis and x_m = sqrt 2 for m > 0: the normalised
+ * associated-Legendre part of the SH basis (the FP64 recurrence of arithmetic.py:86-109), which is the same at every
+ * node of a straight ray. sos_sh_records multiplies it by each node's cos(m phi). */
+int sos_plan_basis(const sos_grid *grid, const sos_bundle *bundle, float *basis_dev, void *stream);
+
 /* SH values of every (node, corner) as an MFMA contraction: records[4 n + c][w] = sum_k Z[n][k] plane[cell_c][k][w]
- * with Z the basis row of node n and cell_c its corners base, base + 1, base + nc, base + nc + 1 (the replaced
- * per-corner sum of numpy_reference.py:579-600). Work item i covers the nodes node_dev[v0[i] .. v1[i]) whose base
- * cell is item_cell[i]; node_dev holds node ids sorted by base cell and node_mu_dev / node_cp_dev the ray's mu and
- * the node's cos(phi) in the same order. half = 1: the plane is the split FP16 plane of sos_make_plane with its
- * per-(cell, lane) scale_dev; Z rows are split the same way and the products run as three v_mfma_f32_16x16x16f16
- * (hi hi, hi lo, lo hi) with FP32 accumulation (mixed precision, FP32-level accuracy; one lane recombines the plane
- * in FP32). */
+ * with Z[n][k] = basis[ray(n)][k] cos(m phi_n) the basis row of node n and cell_c its corners base, base + 1,
+ * base + nc, base + nc + 1 (the replaced per-corner sum of numpy_reference.py:579-600). Work item i covers the nodes
+ * node_dev[v0[i] .. v1[i]) whose base cell is item_cell[i]; node_dev holds node ids sorted by base cell and
+ * node_ray_dev / node_cp_dev the ray and the node's cos(phi) in the same order; basis_dev is the bundle's table from
+ * sos_plan_basis. half = 1: the plane is the split FP16 plane of sos_make_plane with its per-(cell, lane) scale_dev;
+ * Z rows are split the same way and the products run as three v_mfma_f32_16x16x16f16 (hi hi, hi lo, lo hi) with
+ * FP32 accumulation (mixed precision, FP32-level accuracy; one lane recombines the plane in FP32). */
 int sos_sh_records(const sos_grid *grid, const void *plane_dev, const float *scale_dev, int32_t half, int64_t items,
                    const int32_t *item_cell_dev, const int64_t *item_v0_dev, const int64_t *item_v1_dev,
-                   const int32_t *node_dev, const float *node_mu_dev, const float *node_cp_dev, float *records_dev,
-                   void *stream);
+                   const int32_t *node_dev, const int32_t *node_ray_dev, const float *node_cp_dev,
+                   const float *basis_dev, float *records_dev, void *stream);
 
 /* Moments [groups][4 + K][lanes] and ground flux [groups][lanes] of the per-ray radiances rad_dev
  * [rays][lanes][4] (the bundle's groups are the destination cells). ground_only writes the flux only. */

@@ -37,6 +37,7 @@ int plan_nodes_launch(const TransportArgs &a, const int64_t *node_off, int32_t *
 int ground_records_dispatch(const GroundArgs &a, hipStream_t s);
 
 int sh_records_dispatch(const ShArgs &a, bool half, hipStream_t s);
+int plan_basis_launch(int L, int64_t rays, const double *dirs, float *basis, hipStream_t s);
 int compose_launch(int64_t cells, int rows, int K, int G, const float *mom, const float *first, const float *factor,
                    float *out, hipStream_t s);
 int plane_launch(int nh, int nc, int L, int rows, int G, int strato, const float *src, const float *ha,
@@ -157,12 +158,20 @@ int sos_ground_records(const sos_grid *g, int32_t groups, const float *group_g, 
     return ground_records_dispatch(a, (hipStream_t)stream);
 }
 
+int sos_plan_basis(const sos_grid *g, const sos_bundle *b, float *basis_dev, void *stream) {
+    if (!g || !b || !basis_dev) return arg_error("sos_plan_basis: null argument");
+    if (!order_ok(g->L)) return arg_error("sos_plan_basis: L %d", g->L);
+    if (b->rays <= 0) return SOS_OK;
+    if (!b->dirs_dev) return arg_error("sos_plan_basis: bundle without directions");
+    return plan_basis_launch(g->L, b->rays, b->dirs_dev, basis_dev, (hipStream_t)stream);
+}
+
 int sos_sh_records(const sos_grid *g, const void *plane_dev, const float *scale_dev, int32_t half, int64_t items,
                    const int32_t *item_cell_dev, const int64_t *item_v0_dev, const int64_t *item_v1_dev,
-                   const int32_t *node_dev, const float *node_mu_dev, const float *node_cp_dev, float *records_dev,
-                   void *stream) {
-    if (!g || !plane_dev || !item_cell_dev || !item_v0_dev || !item_v1_dev || !node_dev || !node_mu_dev ||
-        !node_cp_dev || !records_dev)
+                   const int32_t *node_dev, const int32_t *node_ray_dev, const float *node_cp_dev,
+                   const float *basis_dev, float *records_dev, void *stream) {
+    if (!g || !plane_dev || !item_cell_dev || !item_v0_dev || !item_v1_dev || !node_dev || !node_ray_dev ||
+        !node_cp_dev || !basis_dev || !records_dev)
         return arg_error("sos_sh_records: null argument");
     if (half && !scale_dev) return arg_error("sos_sh_records: FP16 plane without its scale");
     if (!lanes_ok(g->lanes) || !order_ok(g->L)) return arg_error("sos_sh_records: lanes %d / L %d", g->lanes, g->L);
@@ -170,7 +179,8 @@ int sos_sh_records(const sos_grid *g, const void *plane_dev, const float *scale_
     ShArgs a;
     a.L = g->L; a.G = g->lanes; a.K = (g->L + 1) * (g->L + 2) / 2; a.nc = g->nc; a.plane = plane_dev; a.items = items;
     a.item_cell = item_cell_dev; a.item_v0 = item_v0_dev; a.item_v1 = item_v1_dev; a.node = node_dev;
-    a.node_mu = node_mu_dev; a.node_cp = node_cp_dev; a.records = records_dev; a.scale = scale_dev;
+    a.node_ray = node_ray_dev; a.node_cp = node_cp_dev; a.basis = basis_dev; a.records = records_dev;
+    a.scale = scale_dev;
     a.plane_lo = (int64_t)g->nh * g->nc * ((a.K + 15) / 16 * 16) * a.G;
     a.tile = g->sh_tile;
     return sh_records_dispatch(a, half != 0, (hipStream_t)stream);

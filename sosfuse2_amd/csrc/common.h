@@ -123,7 +123,9 @@ struct ShArgs {
     const int32_t *item_cell;  // [items] base cell (ir * nc + j) of the item's nodes
     const int64_t *item_v0, *item_v1;  // node range of the item in the sorted order
     const int32_t *node;       // [nodes] node ids sorted by base cell
-    const float *node_mu, *node_cp;  // [nodes] the ray's mu and the node's cos(phi), in the same order
+    const int32_t *node_ray;   // [nodes] the ray of each node, in the same order
+    const float *node_cp;      // [nodes] the node's cos(phi), in the same order
+    const float *basis;        // [rays][KZ] fp32(x_m P_lm(mu_ray)), m-major K order, zero padded (sos_plan_basis)
     float *records;            // [nodes][4][G]: corner c of node n at (4 n + c) G
     const float *scale;        // [cells][G] scale of a split FP16 plane (mixed precision)
     int64_t plane_lo;          // offset (elements) of the lo plane after the hi plane: cells * KZ * G

@@ -63,30 +63,6 @@ __device__ inline void visit_seeds(int v, double mu, float cp, const double *dd,
     }
 }
 
-// The same in FP32 for the split path: threads t < V run the seed chain of node t, threads V <= t < 2V the cosine
-// chain of node t - V.
-template <int L, int V>
-__device__ inline void visit_seeds32(int t, float mu, float cp, const float *dd, float (*seed)[V], float (*cmv)[V]) {
-    if (t < V) {
-        const float st = sqrtf(fmaxf(0.0f, 1.0f - mu * mu));
-        float diag = 0.28209479177387814f;
-        for (int m = 0; m <= L; ++m) {
-            if (m) diag = (diag * dd[m]) * st;
-            seed[m][t] = diag;
-        }
-    } else {
-        const int v = t - V;
-        float c0 = 1.0f, c1 = cp;
-        for (int m = 0; m <= L; ++m) {
-            float cm;
-            if (m == 0) cm = 1.0f;
-            else if (m == 1) cm = cp;
-            else { cm = (2.0f * cp) * c1 - c0; c0 = c1; c1 = cm; }
-            cmv[m][v] = cm;
-        }
-    }
-}
-
 // Basis rows of V nodes into LDS, tasks first, first + stride, ... below last. Task (node, p) runs the Legendre
 // recurrence over l for the two orders m = p and m = L - p side by side (their lengths add up to L + 2, so tasks
 // are balanced and each thread has two independent dependency chains): a loop over both chains while the short one
@@ -154,6 +130,129 @@ __device__ inline void build_rows(const T *mus, const T (*seed)[V], const float 
 // Corner c of a node with base cell b (ir * nc + j): b, b + 1, b + nc, b + nc + 1, the transport's corner order.
 __device__ inline int corner_cell(int b, int c, int nc) { return b + (c & 1) + (c >> 1) * nc; }
 
+// Basis table of a bundle: one workgroup per 16 rays builds their rows fp32(x_m P_lm(mu)) with the FP64
+// recurrence (build_rows with cos(m phi) = 1, exactly: the Chebyshev chain of cp = 1 is 1 at every m) in LDS and
+// writes them out row by row, coalesced.
+template <int L>
+__global__ __launch_bounds__(256) void plan_basis_kernel(int64_t rays, const double *dirs, float *basis) {
+    constexpr int K = (L + 1) * (L + 2) / 2;
+    constexpr int KZ = (K + 15) / 16 * 16;
+    constexpr int V = 16;
+    constexpr int NT = 256;
+    __shared__ __attribute__((aligned(16))) float Z[V][KZ];
+    __shared__ double aa[K], bb[K];
+    __shared__ double seed[L + 1][V];
+    __shared__ float cmv[L + 1][V];
+    __shared__ double smu[V];
+    __shared__ double dd[L + 1];
+    const int tid = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * V;
+    const int nv = (int)min((int64_t)V, rays - r0);
+    stage_constants<L>(aa, bb, NT);
+    stage_dd<L>(dd, NT);
+    for (int e = tid; e < V * (KZ - K); e += NT) Z[e / (KZ - K)][K + e % (KZ - K)] = 0.0f;
+    __syncthreads();
+    if (tid < V) {
+        const double mu = tid < nv ? (double)(float)dirs[3 * (r0 + tid) + 2] : 0.0;
+        smu[tid] = mu;
+        visit_seeds<L, V>(tid, mu, 1.0f, dd, seed, cmv);
+    }
+    __syncthreads();
+    build_rows<L, V, 2, 4, double>(smu, seed, cmv, aa, bb, nv, tid, NT, V * (L / 2 + 1),
+                                   [&](int v, int k, float z) { Z[v][k] = z; });
+    __syncthreads();
+    for (int e = tid; e < nv * (KZ / 4); e += NT) {
+        const int v = e / (KZ / 4), q = e % (KZ / 4);
+        reinterpret_cast<float4 *>(basis + (r0 + v) * KZ)[q] = reinterpret_cast<const float4 *>(&Z[v][0])[q];
+    }
+}
+
+int plan_basis_launch(int L, int64_t rays, const double *dirs, float *basis, hipStream_t s) {
+    const dim3 grid((unsigned)((rays + 15) / 16));
+    switch (L) {
+        case 8: hipLaunchKernelGGL(plan_basis_kernel<8>, grid, dim3(256), 0, s, rays, dirs, basis); break;
+        case 16: hipLaunchKernelGGL(plan_basis_kernel<16>, grid, dim3(256), 0, s, rays, dirs, basis); break;
+        case 32: hipLaunchKernelGGL(plan_basis_kernel<32>, grid, dim3(256), 0, s, rays, dirs, basis); break;
+        default: return set_error("plan_basis: L %d", L);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error("plan_basis launch: %s", hipGetErrorString(e));
+    return SOS_OK;
+}
+
+// Order m of every position k' of the m-major K order (0 in the padding K <= k' < KZ).
+template <int L, int KZ>
+__device__ inline void stage_km(unsigned char *km, int nt) {
+    constexpr int K = (L + 1) * (L + 2) / 2;
+    for (int k = threadIdx.x; k < KZ; k += nt) {
+        int m = 0;
+        if (k < K)
+            while (moff<L>(m + 1) <= k) ++m;
+        km[k] = (unsigned char)m;
+    }
+}
+
+// cos(m phi), m = 0 .. L, by the FP32 Chebyshev recurrence (arithmetic.py:112-119).
+template <int L>
+__device__ inline void cos_chain(float cp, float *c) {
+    float c0 = 1.0f, c1 = cp;
+    c[0] = 1.0f;
+    c[1] = cp;
+    for (int m = 2; m <= L; ++m) {
+        const float cm = (2.0f * cp) * c1 - c0;
+        c0 = c1;
+        c1 = cm;
+        c[m] = cm;
+    }
+}
+
+// Per-round node data, prepared two rounds ahead by 16 threads: the node ids (record rows), the rays (basis rows)
+// and cos(m phi) of the round's nodes, in slot r & 3.
+template <int L, int V>
+struct RoundMeta {
+    int id[4][V];
+    int ray[4][V];
+    float cm[4][V][L + 1];
+};
+
+// Thread v < V: node v of the round starting at sorted position r0 into slot q (id -1 and ray -1 past v1).
+template <int L, int V>
+__device__ inline void meta_fill(RoundMeta<L, V> &mt, int q, int v, bool in, int id, int ray, float cp) {
+    mt.id[q][v] = in ? id : -1;
+    mt.ray[q][v] = in ? ray : -1;
+    cos_chain<L>(in ? cp : 1.0f, mt.cm[q][v]);
+}
+
+// Basis rows of the round in slot q, PT float4 chunks per thread (chunk c = tid + j NT: row c / (KZ/4)), read into
+// registers; rows of absent nodes read as 0.
+template <int V, int KZ, int NT, int PT, typename Meta>
+__device__ inline void fetch_rows(float4 *pr, const float *basis, const Meta &mt, int q, int tid) {
+#pragma unroll
+    for (int j = 0; j < PT; ++j) {
+        const int c = tid + j * NT;
+        pr[j] = float4{0.0f, 0.0f, 0.0f, 0.0f};
+        if (c < V * (KZ / 4)) {
+            const int ray = mt.ray[q][c / (KZ / 4)];
+            if (ray >= 0) pr[j] = reinterpret_cast<const float4 *>(basis + (int64_t)ray * KZ)[c % (KZ / 4)];
+        }
+    }
+}
+
+// Z[v][k] = basis row [k] cos(m(k) phi_v), four positions per chunk, through sink(v, q, z4) (q = k / 4).
+template <int V, int KZ, int NT, int PT, typename Meta, typename Sink>
+__device__ inline void rows_to_z(const float4 *pr, const Meta &mt, int q, const unsigned char *km, int tid, Sink sink) {
+#pragma unroll
+    for (int j = 0; j < PT; ++j) {
+        const int c = tid + j * NT;
+        if (c < V * (KZ / 4)) {
+            const int v = c / (KZ / 4), k4 = c % (KZ / 4);
+            const uchar4 m = reinterpret_cast<const uchar4 *>(km)[k4];
+            const float *cm = mt.cm[q][v];
+            sink(v, k4, float4{pr[j].x * cm[m.x], pr[j].y * cm[m.y], pr[j].z * cm[m.z], pr[j].w * cm[m.w]});
+        }
+    }
+}
+
 // FP32 MFMA products of one round: d = Z rows (16 nodes) x this wavefront's B operand (b[4 i + u] = plane column at
 // k = 16 i + 4 (lane / 16) + u), four independent accumulators. The A operands (one ds_read_b128 per group of four
 // products) are read D groups ahead, and the schedule is pinned to (four products, one read) per group so that the
@@ -197,14 +296,32 @@ __device__ inline void load_b(float *b, const void *plane, int cell, int col, in
         }
 }
 
+// Node data of the round starting at sorted position r0, node v (for meta_fill).
+struct NodeRef {
+    int id, ray;
+    float cp;
+};
+
+__device__ inline NodeRef node_ref(const ShArgs &a, int64_t r, int64_t v1) {
+    NodeRef n{-1, -1, 1.0f};
+    if (r < v1) {
+        n.id = a.node[r];
+        n.ray = a.node_ray[r];
+        n.cp = a.node_cp[r];
+    }
+    return n;
+}
+
 // FP32 SH records, one workgroup per work item (a run of nodes of one base cell), 4 x G/16 wavefronts: wavefront
 // (c, nt) holds corner c's coefficients of lanes 16 nt .. 16 nt + 15 in registers (whole K, about 250 VGPRs, two
-// wavefronts per SIMD, one workgroup per CU). Two forms, a tuning choice (the records are bitwise the same):
-//   sh_records_kernel (flow): per round of 16 nodes, the threads t < 16 (L / 2 + 1) build the basis tile while the
-//     last wavefront computes the seeds of the next round, one barrier, then the products of all wavefronts. Tiles,
-//     seeds and node ids are multi-buffered, so a wavefront that finishes its products starts building the next tile
-//     at once and the others' products overlap it;
-//   sh_records_step_kernel: seeds, barrier, build, barrier, products, barrier.
+// wavefronts per SIMD, one workgroup per CU). Per round of 16 nodes the basis tile is the nodes' basis rows (read
+// from the bundle's table a round ahead) times their cos(m phi). Two forms, a tuning choice (the records are bitwise
+// the same):
+//   sh_records_kernel (flow): tile r is formed from the rows fetched during round r - 1, the rows of round r + 1 are
+//     fetched, 16 threads of the last wavefront prepare the node data of round r + 2, one barrier, then the
+//     products of all wavefronts; tiles are double-buffered and node data four-buffered, so the fetches run under
+//     the products and a wavefront that finishes its products goes on to the next tile at once;
+//   sh_records_step_kernel: node data, barrier, fetch and tile, barrier, products, barrier.
 template <int G, int L>
 __global__ __launch_bounds__(G * 16) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void sh_records_kernel(ShArgs a) {
@@ -215,15 +332,11 @@ void sh_records_kernel(ShArgs a) {
     constexpr int NT = 64 * 4 * NWB;
     constexpr int V = 16;                    // nodes per round (one M-tile)
     constexpr int ZS = KZ + 4;               // row stride = 4 (mod 32 banks): 16 rows spread over the LDS banks
-    constexpr int SW = NT - 64;              // first thread of the seed wavefront
-    constexpr int TASKS = V * (L / 2 + 1);
-    __shared__ float Z[2][V][ZS];
-    __shared__ double aa[K], bb[K];
-    __shared__ double seed[2][L + 1][V];
-    __shared__ float cmv[2][L + 1][V];
-    __shared__ double smu[2][V];
-    __shared__ double dd[L + 1];
-    __shared__ int sid[3][V];  // node ids of round r in sid[r % 3]
+    constexpr int SW = NT - 64;              // first thread of the node-data wavefront
+    constexpr int PT = (V * KZ / 4 + NT - 1) / NT;
+    __shared__ __attribute__((aligned(16))) float Z[2][V][ZS];
+    __shared__ RoundMeta<L, V> mt;
+    __shared__ __attribute__((aligned(4))) unsigned char km[KZ];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int corner = wave / NWB, nt = wave % NWB;
     const int col = nt * 16 + (lane & 15);
@@ -231,59 +344,39 @@ void sh_records_kernel(ShArgs a) {
     const int cell = corner_cell(a.item_cell[item], corner, a.nc);
     const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
     const int rounds = (int)((v1 - v0 + V - 1) / V);
-    const bool seeder = tid >= SW && tid < SW + V;
-    stage_constants<L>(aa, bb, NT);
-    stage_dd<L>(dd, NT);
-    for (int e = tid; e < 2 * V * (KZ - K); e += NT) {
-        const int q = e / (KZ - K);
-        Z[q / V][q % V][K + e % (KZ - K)] = 0.0f;
-    }
+    const bool meta = tid >= SW && tid < SW + V;
+    stage_km<L, KZ>(km, NT);
     float b[4 * KQ];
     load_b<G, L, KQ>(b, a.plane, cell, col, lane);
-    __syncthreads();  // dd
-    // seeds of round 0; node (id, mu, cos phi) of round 1 held in registers by the seed threads
-    int nid = -1;
-    float nmu = 0.0f, ncp = 1.0f;
-    if (seeder) {
-        const int v = tid - SW;
-        const bool in = v0 + v < v1;
-        const double mu = in ? (double)a.node_mu[v0 + v] : 0.0;
-        smu[0][v] = mu;
-        sid[0][v] = in ? a.node[v0 + v] : -1;
-        visit_seeds<L, V>(v, mu, in ? a.node_cp[v0 + v] : 1.0f, dd, seed[0], cmv[0]);
-        if (v0 + V + v < v1) {
-            nid = a.node[v0 + V + v];
-            nmu = a.node_mu[v0 + V + v];
-            ncp = a.node_cp[v0 + V + v];
-        }
+    // node data of rounds 0 and 1 (threads t < 2V); that of round 2 held in registers by the node-data threads
+    if (tid < 2 * V) {
+        const NodeRef n = node_ref(a, v0 + tid, v1);
+        meta_fill<L, V>(mt, tid / V, tid % V, n.id >= 0, n.id, n.ray, n.cp);
     }
+    NodeRef nn{-1, -1, 1.0f};
+    if (meta) nn = node_ref(a, v0 + 2 * V + (tid - SW), v1);
     __syncthreads();
+    float4 pr[PT];
+    fetch_rows<V, KZ, NT, PT>(pr, a.basis, mt, 0, tid);
     for (int r = 0; r < rounds; ++r) {
-        const int cur = r & 1, nxt = cur ^ 1;
+        const int cur = r & 1;
         const int64_t r0 = v0 + (int64_t)r * V;
         const int nv = (int)min((int64_t)V, v1 - r0);
-        // Z[cur] was last read by the products of round r - 2, seed[cur] by the build of round r - 2: both before
-        // the barrier of round r - 1
+        // Z[cur] was last read by the products of round r - 2 (before the barrier of round r - 1)
         float (*Zc)[ZS] = Z[cur];
-        build_rows<L, V, 2, 4, double>(smu[cur], seed[cur], cmv[cur], aa, bb, nv, tid, NT, TASKS,
-                                       [&](int v, int k, float z) { Zc[v][k] = z; });
-        if (seeder && r + 1 < rounds) {
+        rows_to_z<V, KZ, NT, PT>(pr, mt, r & 3, km, tid, [&](int v, int k4, float4 z) {
+            reinterpret_cast<float4 *>(&Zc[v][0])[k4] = z;
+        });
+        if (r + 1 < rounds) fetch_rows<V, KZ, NT, PT>(pr, a.basis, mt, (r + 1) & 3, tid);
+        if (meta && r + 2 < rounds) {
+            // slot (r + 2) & 3 last held round r - 2, read by the products of round r - 2 (before this round)
             const int v = tid - SW;
-            const bool in = r0 + V + v < v1;
-            const double mu = in ? (double)nmu : 0.0;
-            smu[nxt][v] = mu;
-            sid[(r + 1) % 3][v] = in ? nid : -1;
-            visit_seeds<L, V>(v, mu, in ? ncp : 1.0f, dd, seed[nxt], cmv[nxt]);
-            const int64_t nx = r0 + 2 * V + v;
-            if (nx < v1) {
-                nid = a.node[nx];
-                nmu = a.node_mu[nx];
-                ncp = a.node_cp[nx];
-            }
+            meta_fill<L, V>(mt, (r + 2) & 3, v, nn.id >= 0, nn.id, nn.ray, nn.cp);
+            nn = node_ref(a, r0 + 3 * V + v, v1);
         }
         __syncthreads();
         const floatx4 d = sh_products<KQ, ZS, 3>(&Z[cur][0][0], b, lane);
-        const int *ids = sid[r % 3];
+        const int *ids = mt.id[r & 3];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int row = 4 * (lane >> 4) + v;
@@ -302,60 +395,37 @@ void sh_records_step_kernel(ShArgs a) {
     constexpr int NT = 64 * 4 * NWB;
     constexpr int V = 16;
     constexpr int ZS = KZ + 4;
-    constexpr int TASKS = V * (L / 2 + 1);
-    __shared__ float Z[V][ZS];
-    __shared__ double aa[K], bb[K];
-    __shared__ double seed[L + 1][V];
-    __shared__ float cmv[L + 1][V];
-    __shared__ double smu[V];
-    __shared__ double dd[L + 1];
-    __shared__ int sid[V];
+    constexpr int PT = (V * KZ / 4 + NT - 1) / NT;
+    __shared__ __attribute__((aligned(16))) float Z[V][ZS];
+    __shared__ RoundMeta<L, V> mt;
+    __shared__ __attribute__((aligned(4))) unsigned char km[KZ];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int corner = wave / NWB, nt = wave % NWB;
     const int col = nt * 16 + (lane & 15);
     const int64_t item = blockIdx.x;
     const int cell = corner_cell(a.item_cell[item], corner, a.nc);
     const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
-    stage_constants<L>(aa, bb, NT);
-    stage_dd<L>(dd, NT);
-    for (int e = tid; e < V * (KZ - K); e += NT) Z[e / (KZ - K)][K + e % (KZ - K)] = 0.0f;
+    stage_km<L, KZ>(km, NT);
     float b[4 * KQ];
     load_b<G, L, KQ>(b, a.plane, cell, col, lane);
-    // node data of the next round is loaded one round ahead (contiguous in the sorted order)
-    int nid = -1;
-    float nmu = 0.0f, ncp = 1.0f;
-    if (tid < V && v0 + tid < v1) {
-        nid = a.node[v0 + tid];
-        nmu = a.node_mu[v0 + tid];
-        ncp = a.node_cp[v0 + tid];
-    }
-    __syncthreads();  // dd
     for (int64_t r0 = v0; r0 < v1; r0 += V) {
         const int nv = (int)min((int64_t)V, v1 - r0);
         if (tid < V) {
-            const int id = nid;
-            const double mu = (double)nmu;
-            const float cp = ncp;
-            const int64_t nx = r0 + V + tid;
-            nid = -1;
-            if (nx < v1) {
-                nid = a.node[nx];
-                nmu = a.node_mu[nx];
-                ncp = a.node_cp[nx];
-            }
-            sid[tid] = tid < nv ? id : -1;
-            smu[tid] = tid < nv ? mu : 0.0;
-            visit_seeds<L, V>(tid, tid < nv ? mu : 0.0, tid < nv ? cp : 1.0f, dd, seed, cmv);
+            const NodeRef n = node_ref(a, r0 + tid, v1);
+            meta_fill<L, V>(mt, 0, tid, n.id >= 0, n.id, n.ray, n.cp);
         }
         __syncthreads();
-        build_rows<L, V, 2, 4, double>(smu, seed, cmv, aa, bb, nv, tid, NT, TASKS,
-                                       [&](int v, int k, float z) { Z[v][k] = z; });
+        float4 pr[PT];
+        fetch_rows<V, KZ, NT, PT>(pr, a.basis, mt, 0, tid);
+        rows_to_z<V, KZ, NT, PT>(pr, mt, 0, km, tid, [&](int v, int k4, float4 z) {
+            reinterpret_cast<float4 *>(&Z[v][0])[k4] = z;
+        });
         __syncthreads();
         const floatx4 d = sh_products<KQ, ZS, 3>(&Z[0][0], b, lane);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int row = 4 * (lane >> 4) + v;
-            if (row < nv) a.records[((int64_t)sid[row] * 4 + corner) * G + col] = d[v];
+            if (row < nv) a.records[((int64_t)mt.id[0][row] * 4 + corner) * G + col] = d[v];
         }
         __syncthreads();
     }
@@ -365,15 +435,18 @@ void sh_records_step_kernel(ShArgs a) {
 // of Ootomo and Yokota (2022, "Recovering single precision accuracy from Tensor Cores while surpassing the FP32
 // theoretical peak performance"): x = hi + lo 2^-11 with hi = fp16(x) and lo = fp16((x - hi) 2^11), and
 // a b = ah bh + (ah bl + al bh) 2^-11 (the lo lo term, 2^-22 relative, is dropped). The plane is stored scaled per
-// (cell, lane) to [-1, 1] (plane = c / scale, scale a power of two) so hi and lo stay in FP16's normal range; Z rows
-// are generated by the FP32 recurrence and split on the way into LDS. Each k-step of 16 costs three 8-pass MFMAs
-// instead of four 32-pass FP32 MFMAs; the error per record is about 2^-21 of sum_k |Z_k c_k|, the FP32 level.
+// (cell, lane) to [-1, 1] (plane = c / scale, scale a power of two) so hi and lo stay in FP16's normal range; the
+// FP32 basis tile (table row times cos(m phi)) is split on the way into LDS. Each k-step of 16 costs three 8-pass
+// MFMAs instead of four 32-pass FP32 MFMAs; the error per record is about 2^-21 of sum_k |Z_k c_k|, the FP32 level.
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 
-__device__ inline void split_store(_Float16 *hi, _Float16 *lo, float z) {
-    const _Float16 h = (_Float16)z;
-    *hi = h;
-    *lo = (_Float16)((z - (float)h) * 2048.0f);
+__device__ inline half4_t split_hi(float4 z) {
+    return half4_t{(_Float16)z.x, (_Float16)z.y, (_Float16)z.z, (_Float16)z.w};
+}
+
+__device__ inline half4_t split_lo(float4 z, half4_t h) {
+    return half4_t{(_Float16)((z.x - (float)h.x) * 2048.0f), (_Float16)((z.y - (float)h.y) * 2048.0f),
+                   (_Float16)((z.z - (float)h.z) * 2048.0f), (_Float16)((z.w - (float)h.w) * 2048.0f)};
 }
 
 // Split-FP16 products of one round: three independent accumulators (hi hi, hi lo, lo hi) so that no product waits
@@ -408,8 +481,7 @@ __device__ inline floatx4 split_products(const _Float16 *hrow, const _Float16 *l
 
 // Split-FP16 SH records in the flow form of sh_records_kernel: 4 x G/16 wavefronts, wavefront (corner, N-tile)
 // holding the whole K range of its hi / lo B operands (144 VGPRs at L = 32 in the quad layout [cell][KZ/4][G][4],
-// about 240 in all: two wavefronts per SIMD); the seed wavefront's first 2 x 16 threads run the seed and cosine
-// chains of the next round.
+// about 240 in all: two wavefronts per SIMD).
 template <int G, int L>
 __global__ __launch_bounds__(G * 16) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void sh_records_split_kernel(ShArgs a) {
@@ -421,14 +493,11 @@ void sh_records_split_kernel(ShArgs a) {
     constexpr int V = 16;                     // nodes per round (one M-tile)
     constexpr int ZS = KZ + 8;                // halves: row stride = 4 (mod 32 banks) in 32-bit words
     constexpr int SW = NT - 64;
-    constexpr int TASKS = V * (L / 2 + 1);
-    __shared__ _Float16 Zh[2][V][ZS], Zl[2][V][ZS];
-    __shared__ float aa[K], bb[K];
-    __shared__ float seed[2][L + 1][V];
-    __shared__ float cmv[2][L + 1][V];
-    __shared__ float smu[2][V];
-    __shared__ float dd[L + 1];
-    __shared__ int sid[3][V];
+    constexpr int PT = (V * KZ / 4 + NT - 1) / NT;
+    __shared__ __attribute__((aligned(16))) _Float16 Zh[2][V][ZS];
+    __shared__ __attribute__((aligned(16))) _Float16 Zl[2][V][ZS];
+    __shared__ RoundMeta<L, V> mt;
+    __shared__ __attribute__((aligned(4))) unsigned char km[KZ];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int corner = wave / NW, nt = wave % NW;
     const int col = nt * 16 + (lane & 15);
@@ -436,14 +505,8 @@ void sh_records_split_kernel(ShArgs a) {
     const int cell = corner_cell(a.item_cell[item], corner, a.nc);
     const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
     const int rounds = (int)((v1 - v0 + V - 1) / V);
-    const bool seeder = tid >= SW && tid < SW + 2 * V;
-    stage_constants<L>(aa, bb, NT);
-    stage_dd<L>(dd, NT);
-    for (int e = tid; e < 2 * V * (KZ - K); e += NT) {
-        const int q = e / (KZ - K);
-        Zh[q / V][q % V][K + e % (KZ - K)] = (_Float16)0.0f;
-        Zl[q / V][q % V][K + e % (KZ - K)] = (_Float16)0.0f;
-    }
+    const bool meta = tid >= SW && tid < SW + V;
+    stage_km<L, KZ>(km, NT);
     const float scale = a.scale[(int64_t)cell * G + col];
     // B operands from the quad layout: bh[i], bl[i] hold K positions 16 i + 4 (lane / 16) + 0..3 of column col
     half4_t bh[KS], bl[KS];
@@ -457,53 +520,36 @@ void sh_records_split_kernel(ShArgs a) {
             bl[i] = pl[(int64_t)q * G];
         }
     }
-    __syncthreads();  // dd
-    int nid = -1;
-    float nmu = 0.0f, ncp = 1.0f;
-    if (seeder) {
-        const int t = tid - SW, v = t % V;
-        const bool in = v0 + v < v1;
-        const float mu = in ? a.node_mu[v0 + v] : 0.0f;
-        if (t < V) {
-            smu[0][v] = mu;
-            sid[0][v] = in ? a.node[v0 + v] : -1;
-        }
-        visit_seeds32<L, V>(t, mu, in ? a.node_cp[v0 + v] : 1.0f, dd, seed[0], cmv[0]);
-        if (v0 + V + v < v1) {
-            nid = a.node[v0 + V + v];
-            nmu = a.node_mu[v0 + V + v];
-            ncp = a.node_cp[v0 + V + v];
-        }
+    if (tid < 2 * V) {
+        const NodeRef n = node_ref(a, v0 + tid, v1);
+        meta_fill<L, V>(mt, tid / V, tid % V, n.id >= 0, n.id, n.ray, n.cp);
     }
+    NodeRef nn{-1, -1, 1.0f};
+    if (meta) nn = node_ref(a, v0 + 2 * V + (tid - SW), v1);
     __syncthreads();
+    float4 pr[PT];
+    fetch_rows<V, KZ, NT, PT>(pr, a.basis, mt, 0, tid);
     for (int r = 0; r < rounds; ++r) {
-        const int cur = r & 1, nxt = cur ^ 1;
+        const int cur = r & 1;
         const int64_t r0 = v0 + (int64_t)r * V;
         const int nv = (int)min((int64_t)V, v1 - r0);
         _Float16 (*Hc)[ZS] = Zh[cur];
         _Float16 (*Lc)[ZS] = Zl[cur];
-        build_rows<L, V, 1, 1, float>(smu[cur], seed[cur], cmv[cur], aa, bb, nv, tid, NT, TASKS,
-                                      [&](int v, int k, float z) { split_store(&Hc[v][k], &Lc[v][k], z); });
-        if (seeder && r + 1 < rounds) {
-            const int t = tid - SW, v = t % V;
-            const bool in = r0 + V + v < v1;
-            const float mu = in ? nmu : 0.0f;
-            if (t < V) {
-                smu[nxt][v] = mu;
-                sid[(r + 1) % 3][v] = in ? nid : -1;
-            }
-            visit_seeds32<L, V>(t, mu, in ? ncp : 1.0f, dd, seed[nxt], cmv[nxt]);
-            const int64_t nx = r0 + 2 * V + v;
-            if (nx < v1) {
-                nid = a.node[nx];
-                nmu = a.node_mu[nx];
-                ncp = a.node_cp[nx];
-            }
+        rows_to_z<V, KZ, NT, PT>(pr, mt, r & 3, km, tid, [&](int v, int k4, float4 z) {
+            const half4_t h = split_hi(z);
+            reinterpret_cast<half4_t *>(&Hc[v][0])[k4] = h;
+            reinterpret_cast<half4_t *>(&Lc[v][0])[k4] = split_lo(z, h);
+        });
+        if (r + 1 < rounds) fetch_rows<V, KZ, NT, PT>(pr, a.basis, mt, (r + 1) & 3, tid);
+        if (meta && r + 2 < rounds) {
+            const int v = tid - SW;
+            meta_fill<L, V>(mt, (r + 2) & 3, v, nn.id >= 0, nn.id, nn.ray, nn.cp);
+            nn = node_ref(a, r0 + 3 * V + v, v1);
         }
         __syncthreads();
         const floatx4 d = split_products<KS, 3>(&Zh[cur][lane & 15][4 * (lane >> 4)],
                                                 &Zl[cur][lane & 15][4 * (lane >> 4)], bh, bl);
-        const int *ids = sid[r % 3];
+        const int *ids = mt.id[r & 3];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int row = 4 * (lane >> 4) + v;
@@ -512,63 +558,44 @@ void sh_records_split_kernel(ShArgs a) {
     }
 }
 
-// One lane (W = 1): one thread per node, FP64 recurrence, dot products against the four corner cells' coefficients
-// staged in LDS (a split FP16 plane is recombined and scaled back to FP32 there).
+// One lane (W = 1): one thread per node, dot products of the node's basis row (table row times cos(m phi)) against
+// the four corner cells' coefficients staged in LDS (a split FP16 plane is recombined and scaled back to FP32 there).
 template <int L, bool HALF>
 __global__ __launch_bounds__(256) void sh_records_scalar(ShArgs a) {
     constexpr int K = (L + 1) * (L + 2) / 2;
+    constexpr int KZ = (K + 15) / 16 * 16;
     __shared__ float c[4][K];
-    __shared__ double aa[K], bb[K];
     const int64_t item = blockIdx.x;
     const int base = a.item_cell[item];
     for (int e = threadIdx.x; e < 4 * K; e += 256) {
         const int cell = corner_cell(base, e / K, a.nc), k = e % K;
         const int64_t i = (int64_t)cell * K + k;
         if constexpr (HALF) {
-            const int64_t j = (int64_t)cell * ((K + 15) / 16 * 16) + k;  // quad layout with one lane
+            const int64_t j = (int64_t)cell * KZ + k;  // quad layout with one lane
             c[e / K][k] = (plane_at<L, true>(a.plane, j) + plane_at<L, true>(a.plane, j + a.plane_lo) * (1.0f / 2048.0f)) *
                           a.scale[cell];
         }
         else
             c[e / K][k] = plane_at<L, false>(a.plane, i);
     }
-    stage_constants<L>(aa, bb, 256);
     __syncthreads();
     for (int64_t r = a.item_v0[item] + threadIdx.x; r < a.item_v1[item]; r += 256) {
         const int id = a.node[r];
-        const double mu = a.node_mu[r];
+        const float *row = a.basis + (int64_t)a.node_ray[r] * KZ;
         const float cp = a.node_cp[r];
-        const double st = sqrt(fmax(0.0, 1.0 - mu * mu));
         float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         float cm0 = 1.0f, cm1 = cp;
-        double diag = 0.28209479177387814;
         for (int m = 0; m <= L; ++m) {
             float cm;
             if (m == 0) cm = 1.0f;
             else if (m == 1) cm = cp;
             else { cm = (2.0f * cp) * cm1 - cm0; cm0 = cm1; cm1 = cm; }
-            if (m) diag = (diag * -sqrt((2.0 * m + 1) / (2.0 * m))) * st;
             const int o = moff<L>(m);
-            const double mult = m ? 1.41421356237309515 : 1.0;
-            double z0 = diag;
-            float acc[4];
-            float z = (float)(mult * z0) * cm;
+            float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int l = m; l <= L; ++l) {
+                const float z = row[o + l - m] * cm;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] = z * c[q][o];
-            if (m < L) {
-                double z1 = (sqrt(2.0 * m + 3.0) * mu) * z0;
-                z = (float)(mult * z1) * cm;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc[q] = acc[q] + z * c[q][o + 1];
-                for (int l = m + 2; l <= L; ++l) {
-                    const int k = kidx(l, m);
-                    const double z2 = (aa[k] * mu) * z1 - bb[k] * z0;
-                    z = (float)(mult * z2) * cm;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) acc[q] = acc[q] + z * c[q][o + l - m];
-                    z0 = z1;
-                    z1 = z2;
-                }
+                for (int q = 0; q < 4; ++q) acc[q] = acc[q] + z * c[q][o + l - m];
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) sum[q] = sum[q] + acc[q];
@@ -577,7 +604,6 @@ __global__ __launch_bounds__(256) void sh_records_scalar(ShArgs a) {
         for (int q = 0; q < 4; ++q) a.records[(int64_t)id * 4 + q] = sum[q];
     }
 }
-
 
 template <int G, int L>
 static int sh_launch(const ShArgs &a, bool half, hipStream_t s) {

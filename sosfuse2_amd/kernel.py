@@ -94,10 +94,13 @@ class Plan:
                  hip.ptr(self.node_mu), hip.ptr(self.node_cp), hip.ptr(self.node_dir), st)
         nc = dc.struct.nc
         cells = dc.struct.nh * nc
-        # nodes by base cell (SH records)
+        # nodes by base cell (SH records), with each node's ray and cos(phi) in the same order
         self.node = torch.argsort(self.node_cell[:N], stable=True).to(torch.int32)
-        self.sorted_mu = self.node_mu[self.node.long()]
+        ray_of = torch.repeat_interleave(torch.arange(db.rays, dtype=torch.int32, device=dev), counts[: db.rays])
+        self.sorted_ray = ray_of[self.node.long()] if N else torch.zeros(1, dtype=torch.int32, device=dev)
+        del ray_of
         self.sorted_cp = self.node_cp[self.node.long()]
+        self.basis = None  # basis table [rays][KZ] (sos_plan_basis), built on first use (basis_table())
         self._cells = cells
         self.set_node_chunk(node_chunk)
         # visits by visited cell (ground records)
@@ -111,6 +114,14 @@ class Plan:
         self.records = torch.empty((max(4 * N, 1), dc.G), dtype=torch.float32, device=dev)
         self.geom = None  # static node geometry of the warm consumer, built on first use (geometry())
         self.sun = None  # sun geometry of first light, built on first use (sun_geometry())
+
+    def basis_table(self, kernel, dc, db):
+        """[rays, KZ] Legendre part of every ray's SH basis row (sos_plan_basis), built once."""
+        if self.basis is None:
+            kz = (kernel.K + 15) // 16 * 16
+            self.basis = torch.empty((max(db.rays, 1), kz), dtype=torch.float32, device=self.node_off.device)
+            hip.call('sos_plan_basis', dc.struct, db.struct, hip.ptr(self.basis), kernel.stream())
+        return self.basis
 
     def geometry(self, kernel, dc, db):
         """[nodes, 32] static node geometry (sos_plan_geometry), built once."""
@@ -329,9 +340,11 @@ class Kernel:
                 pl = self.plan(db, dc)
                 e0 = self._event()
                 records = pl.records_for(dc.G)
+                basis = pl.basis_table(self, dc, db)
                 hip.call('sos_sh_records', dc.struct, hip.ptr(plane[0]), hip.ptr(plane[1]), int(self.mixed),
                          pl.sh_items, hip.ptr(pl.sh_cell), hip.ptr(pl.sh_v0), hip.ptr(pl.sh_v1), hip.ptr(pl.node),
-                         hip.ptr(pl.sorted_mu), hip.ptr(pl.sorted_cp), hip.ptr(records), self.stream())
+                         hip.ptr(pl.sorted_ray), hip.ptr(pl.sorted_cp), hip.ptr(basis), hip.ptr(records),
+                         self.stream())
                 if e0 is not None:
                     e1 = self._event()
                     self.timed.append(('sh_' + name, e0, e1))

@@ -152,6 +152,32 @@ def test_fp32_table_within_1e4_of_fp64(gpu):
     assert err < 1e-4
 
 
+@pytest.mark.parametrize('L', [8, 32])
+def test_basis_table_matches_oracle_basis(gpu, L):
+    """sos_plan_basis against the oracle's FP64 basis (oracle/arith.py basis, the recurrence of arithmetic.py:86-109)
+    at mu = fp32 of each ray's sun-axis direction cosine, reordered to the m-major K order: every entry within one
+    FP32 rounding (the FP64 recurrences may contract differently), the padding zero. The oracle's basis is itself
+    pinned against closed-form spherical harmonics."""
+    from oracle import arith
+
+    s, scope, _ = setup_case(3, L)
+    k = s.kernel
+    solver = scope.solver(s, log=None)
+    db = solver.geometry.bundle
+    dc = k.constants(s, db.nq)
+    pl = k.plan(db, dc)
+    tab = pl.basis_table(k, dc, db).cpu().numpy()
+    K = s.K
+    assert tab.shape == (db.rays, (K + 15) // 16 * 16)
+    assert not tab[:, K:].any()
+    mu = db.dirs[:, 2].cpu().numpy().astype(np.float32).astype('f8')
+    ref = arith.basis(mu, L, np.float64)
+    order = [l * (l + 1) // 2 + m for m in range(L + 1) for l in range(m, L + 1)]  # m-major position -> l-major k
+    ref = ref[:, order]
+    err = np.abs(tab[:, :K] - ref)
+    assert np.all(err <= 2.5e-7 * np.abs(ref) + 1e-30), err.max()
+
+
 @pytest.mark.parametrize('W,L', [(3, 16), (20, 32), (1, 16)])
 def test_mixed_records_match_fp32(gpu, W, L):
     """Split-FP16 SH records (three FP16 MFMA products, FP32 accumulation) against the FP32 records of the same
@@ -173,8 +199,8 @@ def test_mixed_records_match_fp32(gpu, W, L):
     plane16 = k16.plane(s16, fd, dc16)
     assert plane16[0].shape == (2, s.geo.ns, (s.K + 15) // 16 * 4, dc.G, 4)
     hip.call('sos_sh_records', dc16.struct, hip.ptr(plane16[0]), hip.ptr(plane16[1]), 1, pl.sh_items,
-             hip.ptr(pl.sh_cell), hip.ptr(pl.sh_v0), hip.ptr(pl.sh_v1), hip.ptr(pl.node), hip.ptr(pl.sorted_mu),
-             hip.ptr(pl.sorted_cp), hip.ptr(pl.records), k16.stream())
+             hip.ptr(pl.sh_cell), hip.ptr(pl.sh_v0), hip.ptr(pl.sh_v1), hip.ptr(pl.node), hip.ptr(pl.sorted_ray),
+             hip.ptr(pl.sorted_cp), hip.ptr(pl.basis_table(k, dc, db)), hip.ptr(pl.records), k16.stream())
     torch.cuda.synchronize()
     r16 = pl.records[: 4 * pl.nodes].cpu().numpy().astype('f8')
     r32 = rec32[: 4 * pl.nodes].cpu().numpy().astype('f8')
