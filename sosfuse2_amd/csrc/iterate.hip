@@ -71,10 +71,12 @@ __global__ void plane_kernel(int nh, int nc, int L, int rows, int G, int strato,
     }
 }
 
-// Split FP16 plane for the mixed-precision contraction: one workgroup per cell. scale[cell][w] is the power of two
-// just above max_j |c[cell][j][w]| (1 for an all-zero column); x = c / scale lies in [-1, 1] and is stored as
+// Split FP16 plane for the mixed-precision contraction: one workgroup per cell. scale[cell][w] is a power of two
+// with max_j |c[cell][j][w]| / scale in [2^14, 2^15) (1 for an all-zero column); x = c / scale is stored as
 // hi = fp16(x) in the first plane and lo = fp16((x - hi) 2^11) in the second (cells x KZ x G further on), so
-// hi + lo 2^-11 carries x to 2^-23 and neither part leaves FP16's range. Layout of each plane: quads of four
+// hi + lo 2^-11 carries x to 2^-23 relative, neither part leaves FP16's range, and every entry down to 2^-28 of the
+// column's largest is an FP16 normal number (the coefficients of a column span many decades; MFMA products of
+// subnormal FP16 operands are not exact). Layout of each plane: quads of four
 // consecutive K positions per lane, [cell][KZ / 4][G][4] with KZ = K rounded up to 16 and zeros beyond K, so that
 // an MFMA B operand (four K positions of one lane) is one 8-byte load.
 __global__ __launch_bounds__(256) void plane_half_kernel(int nc, int L, int rows, int G, int strato,
@@ -108,7 +110,9 @@ __global__ __launch_bounds__(256) void plane_half_kernel(int nc, int L, int rows
         for (int r = 1; r < nq; ++r) mx = fmaxf(mx, colmax[r * G + tid]);
         int ex = 0;
         frexpf(mx, &ex);  // mx = f 2^ex with f in [0.5, 1)
-        const float sc = mx > 0.0f ? ldexpf(1.0f, ex) : 1.0f;
+        // largest entry in [2^14, 2^15): FP16 keeps entries down to 2^-28 of it normal (MFMA FP16 products of
+        // subnormal operands are not exact); the scale itself stays an FP32 normal power of two
+        const float sc = mx > 0.0f ? ldexpf(1.0f, max(ex - 15, -126)) : 1.0f;
         scale[cell * G + tid] = sc;
         inv[tid] = 1.0f / sc;  // exact: a power of two
     }

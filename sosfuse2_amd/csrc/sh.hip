@@ -434,9 +434,10 @@ void sh_records_step_kernel(ShArgs a) {
 // Mixed precision: split-FP16 operands on v_mfma_f32_16x16x16f16 with FP32 accumulation, the three-product scheme
 // of Ootomo and Yokota (2022, "Recovering single precision accuracy from Tensor Cores while surpassing the FP32
 // theoretical peak performance"): x = hi + lo 2^-11 with hi = fp16(x) and lo = fp16((x - hi) 2^11), and
-// a b = ah bh + (ah bl + al bh) 2^-11 (the lo lo term, 2^-22 relative, is dropped). The plane is stored scaled per
-// (cell, lane) to [-1, 1] (plane = c / scale, scale a power of two) so hi and lo stay in FP16's normal range; the
-// FP32 basis tile (table row times cos(m phi)) is split on the way into LDS. Each k-step of 16 costs three 8-pass
+// a b = ah bh + (ah bl + al bh) 2^-11 (the lo lo term, 2^-22 relative, is dropped). Both operands are scaled by
+// powers of two into the top of FP16's range so that their small entries stay normal numbers: the plane per
+// (cell, lane) to a largest magnitude in [2^14, 2^15) (plane = c / scale), the FP32 basis tile (table row times
+// cos(m phi)) by 2^13 on its way into LDS; the records are scaled back. Each k-step of 16 costs three 8-pass
 // MFMAs instead of four 32-pass FP32 MFMAs; the error per record is about 2^-21 of sum_k |Z_k c_k|, the FP32 level.
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 
@@ -508,6 +509,9 @@ void sh_records_split_kernel(ShArgs a) {
     const bool meta = tid >= SW && tid < SW + V;
     stage_km<L, KZ>(km, NT);
     const float scale = a.scale[(int64_t)cell * G + col];
+    // the basis tile is stored times ZSCALE = 2^13 (|Z| <= sqrt(2 (2L + 1) / 4 pi) < 3.3, so |Z| 2^13 < 2^15): its
+    // entries down to 2^-27 are FP16 normal numbers
+    constexpr float ZSCALE = 8192.0f;
     // B operands from the quad layout: bh[i], bl[i] hold K positions 16 i + 4 (lane / 16) + 0..3 of column col
     half4_t bh[KS], bl[KS];
     {
@@ -536,6 +540,7 @@ void sh_records_split_kernel(ShArgs a) {
         _Float16 (*Hc)[ZS] = Zh[cur];
         _Float16 (*Lc)[ZS] = Zl[cur];
         rows_to_z<V, KZ, NT, PT>(pr, mt, r & 3, km, tid, [&](int v, int k4, float4 z) {
+            z = float4{z.x * ZSCALE, z.y * ZSCALE, z.z * ZSCALE, z.w * ZSCALE};
             const half4_t h = split_hi(z);
             reinterpret_cast<half4_t *>(&Hc[v][0])[k4] = h;
             reinterpret_cast<half4_t *>(&Lc[v][0])[k4] = split_lo(z, h);
@@ -553,7 +558,7 @@ void sh_records_split_kernel(ShArgs a) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int row = 4 * (lane >> 4) + v;
-            if (row < nv) a.records[((int64_t)ids[row] * 4 + corner) * G + col] = d[v] * scale;
+            if (row < nv) a.records[((int64_t)ids[row] * 4 + corner) * G + col] = (d[v] * scale) * (1.0f / ZSCALE);
         }
     }
 }

@@ -181,9 +181,10 @@ def test_basis_table_matches_oracle_basis(gpu, L):
 @pytest.mark.parametrize('W,L', [(3, 16), (20, 32), (1, 16)])
 def test_mixed_records_match_fp32(gpu, W, L):
     """Split-FP16 SH records (three FP16 MFMA products, FP32 accumulation) against the FP32 records of the same
-    field. Operands carry 22 bits (hi + lo 2^-11), the plane's absolute error is <= 2^-23 of its column scale and the
-    dropped lo lo product is 2^-22 relative, so the records agree with the FP32 contraction to a few 1e-7 of the
-    largest record; an error in the lo scaling (2^-11) or a dropped cross product shows up at 1e-4."""
+    field. Operands carry 22 bits (hi + lo 2^-11) and both are scaled into the top of FP16's range (small entries stay
+    normal numbers), and the dropped lo lo product is 2^-22 relative, so the records agree with the FP32 contraction
+    to a few 1e-7 of the largest record; an error in the lo scaling (2^-11), a dropped cross product or operands left
+    subnormal show up at 1e-5 or more."""
     s, scope, orc = setup_case(W, L)
     s16, _, _ = setup_case(W, L, mixed=True)
     k, k16 = s.kernel, s16.kernel
@@ -208,7 +209,7 @@ def test_mixed_records_match_fp32(gpu, W, L):
     err = np.abs(r16 - r32)
     print(f'W={W} L={L}: split-FP16 records vs FP32: max {err.max() / big:.3e}, RMS '
           f'{np.sqrt(np.mean(err ** 2)) / big:.3e} of the largest record')
-    assert err.max() / big < 1e-5
+    assert err.max() / big < 2e-6
 
 
 def test_mixed_table_against_fp64(gpu):
