@@ -146,6 +146,15 @@ int sos_ground_records(const sos_grid *grid, int32_t groups, const float *group_
                        const int32_t *item_cell_dev, const int64_t *item_v0_dev, const int64_t *item_v1_dev,
                        const int32_t *vis_dev, const float *node_dir_dev, float *records_dev, void *stream);
 
+/* Ground pools of the grid's lanes (the state-dependent part of the reference's pool preparation,
+ * engine_core.py:103-147): for the pools of every cell (CSR off_dev [cells + 1] over the pools, cell = ir * nc + j),
+ * gval_dev [pools][lanes] = albedo factor exp(-(column[0..4] . ext[0..4] + column[5] sulfate_ext)) from the lane
+ * table, and gm_dev [cells][4][lanes] = 3/(8 pi) ext_air sum over the cell's pools of ((1 - x^2)/2, (1 - y^2)/2,
+ * (1 - z^2)/2, -x z/2) gval with (x, y, z) = loc_dev [pools][3]; column_dev [pools][6], factor_dev [pools], loc_dev
+ * in FP64, sums in FP64. The two outputs are the grid's gval_dev and gm_dev inputs of first light. */
+int sos_ground_pools(const sos_grid *grid, const int64_t *off_dev, const double *column_dev, const double *factor_dev,
+                     const double *loc_dev, float *gval_dev, float *gm_dev, void *stream);
+
 /* Basis table of a bundle (fixed by its geometry): basis_dev [rays][KZ] (KZ = K rounded up to a multiple of 16)
  * holds fp32(x_m P_lm(mu)) in the m-major K order k' = m (L + 1) - m (m - 1) / 2 + l - m, zero for K <= k' < KZ,
  * with mu = fp32 of the ray direction's component along the sun axis and x_m = sqrt 2 for m > 0: the normalised

@@ -35,6 +35,8 @@ int plan_sun_launch(const TransportArgs &a, const int64_t *node_off, float *sun,
 int plan_nodes_launch(const TransportArgs &a, const int64_t *node_off, int32_t *cell, float *mu, float *cp,
                       float *dir, hipStream_t s);
 int ground_records_dispatch(const GroundArgs &a, hipStream_t s);
+int pool_values_launch(int cells, int G, const int64_t *off, const double *column, const double *factor,
+                       const double *loc, const float *lane, float *gval, float *gm, hipStream_t s);
 
 int sh_records_dispatch(const ShArgs &a, bool half, hipStream_t s);
 int plan_basis_launch(int L, int64_t rays, const double *dirs, float *basis, hipStream_t s);
@@ -156,6 +158,16 @@ int sos_ground_records(const sos_grid *g, int32_t groups, const float *group_g, 
         a.sgg[i] = i < groups ? group_sg[i] : 0.0f;
     }
     return ground_records_dispatch(a, (hipStream_t)stream);
+}
+
+int sos_ground_pools(const sos_grid *g, const int64_t *off_dev, const double *column_dev, const double *factor_dev,
+                     const double *loc_dev, float *gval_dev, float *gm_dev, void *stream) {
+    if (!g || !off_dev || !column_dev || !factor_dev || !loc_dev || !gval_dev || !gm_dev || !g->lane_dev)
+        return arg_error("sos_ground_pools: null argument");
+    if (!lanes_ok(g->lanes)) return arg_error("sos_ground_pools: lanes %d not in {1, 16, 32}", g->lanes);
+    if (g->nh < 2 || g->nc < 2) return arg_error("sos_ground_pools: grid too small");
+    return pool_values_launch(g->nh * g->nc, g->lanes, off_dev, column_dev, factor_dev, loc_dev, g->lane_dev, gval_dev,
+                              gm_dev, (hipStream_t)stream);
 }
 
 int sos_plan_basis(const sos_grid *g, const sos_bundle *b, float *basis_dev, void *stream) {

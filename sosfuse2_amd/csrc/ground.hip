@@ -122,6 +122,47 @@ __global__ __launch_bounds__(G * 4) void ground_records_kernel(GroundArgs a) {
     }
 }
 
+// Ground pools of one set of lanes (engine_core.py:103-147), one workgroup per cell and one thread per lane:
+// gval[p][w] = albedo_w factor_p exp(-tau), tau = column_p[0..4] . ext[0..4][w] + column_p[5] sulfate_ext[w], and
+// the cell's Rayleigh tensor moments of the ground-reflected light gm[cell][j][w] = 3/(8 pi) ext_air[w]
+// sum_p f_j(loc_p) gval[p][w], f = ((1 - x^2)/2, (1 - y^2)/2, (1 - z^2)/2, -x z/2); FP64 throughout, the outputs
+// rounded to FP32 once.
+__global__ void pool_values_kernel(int G, const int64_t *off, const double *column, const double *factor,
+                                   const double *loc, const float *lane, float *gval, float *gm) {
+    const int64_t cell = blockIdx.x;
+    const int w = threadIdx.x;
+    if (w >= G) return;
+    double e[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) e[i] = lane[(SOS_LANE_EXT0 + i) * G + w];
+    const double es = lane[SOS_LANE_SEXT * G + w], alb = lane[SOS_LANE_ALBEDO * G + w];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t p = off[cell]; p < off[cell + 1]; ++p) {
+        const double *c = column + 6 * p;
+        const double tau = c[0] * e[0] + c[1] * e[1] + c[2] * e[2] + c[3] * e[3] + c[4] * e[4] + c[5] * es;
+        const double v = alb * factor[p] * exp(-tau);
+        gval[p * G + w] = (float)v;
+        const double x = loc[3 * p], y = loc[3 * p + 1], z = loc[3 * p + 2];
+        acc[0] += (1.0 - x * x) * 0.5 * v;
+        acc[1] += (1.0 - y * y) * 0.5 * v;
+        acc[2] += (1.0 - z * z) * 0.5 * v;
+        acc[3] += -x * z * 0.5 * v;
+    }
+    const double k = 3.0 / (8.0 * 3.14159265358979324) * e[0];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) gm[(cell * 4 + j) * G + w] = (float)(acc[j] * k);
+}
+
+int pool_values_launch(int cells, int G, const int64_t *off, const double *column, const double *factor,
+                       const double *loc, const float *lane, float *gval, float *gm, hipStream_t s) {
+    if (cells <= 0) return SOS_OK;
+    hipLaunchKernelGGL(pool_values_kernel, dim3((unsigned)cells), dim3(64), 0, s, G, off, column, factor, loc, lane,
+                       gval, gm);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error("pool_values launch: %s", hipGetErrorString(e));
+    return SOS_OK;
+}
+
 template <int G>
 static int ground_launch(const GroundArgs &a, hipStream_t s) {
     if (a.items <= 0) return SOS_OK;

@@ -36,6 +36,10 @@ class Context:
         # Observer boxes of the convergence monitor (`solve/engine.py:435-443`): 7 directions x 4 x 4 GL boxes.
         self.directions = directions or [(2, 0), (10, 0), (30, 0), (10, 180), (30, 180), (10, 90), (89, 0)]
         self._constants = {}
+        self._base = {}
+        self._gv = None
+        self._padded = None
+        self._gm = None
 
     @property
     def L(self):
@@ -55,25 +59,46 @@ class Context:
                        self.monitors, self.directions)
 
     # -------------------------------------------------------------------------------------------- constants
-    def constants(self, nq, dtype=np.float32):
-        """Transport constants (the reference's Kernel.setup, `src/sosfuse2/solve/source_solver.py:1243-1339`)."""
-        key = (nq, np.dtype(dtype).str)
+    def pool_values(self):
+        """gv[P, W] of the ground pools for this context's lanes (computed once)."""
+        if self._gv is None:
+            self._gv = self.pools.values(self.optics)
+        return self._gv
+
+    def constants(self, nq, dtype=np.float32, padded=True):
+        """Transport constants (the reference's Kernel.setup, `src/sosfuse2/solve/source_solver.py:1243-1339`).
+        padded: include the pool arrays gm (per-cell tensor moments) and the per-cell zero-padded gd, gv (read by the
+        oracle; the device path computes its own from the lane table, sos_ground_pools).
+        Everything but the Gauss-Legendre nodes is shared by every node count."""
+        key = (nq, np.dtype(dtype).str, padded)
         c = self._constants.get(key)
         if c is not None:
             return c
+        base = self._base.get(np.dtype(dtype).str)
+        if base is None:
+            base = self._base[np.dtype(dtype).str] = self._base_constants(dtype)
+        u, wu = geometry.gl(nq)
+        c = dict(base, u=u, wu=wu)
+        if padded:
+            F = np.dtype(dtype).type
+            if self._padded is None:
+                self._padded = self.pools.padded(self.optics, self.geo.ns, self.pool_values())
+                self._gm = self.pools.tensor(self.optics, self.geo.ns, self.pool_values())
+            gd, gv = self._padded
+            c.update(gd=gd.astype(F), gv=gv.astype(F), gm=self._gm.astype(F))
+        self._constants[key] = c
+        return c
+
+    def _base_constants(self, dtype):
         F = np.dtype(dtype).type
         g, o, st = self.geo, self.optics, self.state
-        u, wu = geometry.gl(nq)
         prof = atmosphere.profiles(st, g.h)
-        gd, gv = self.pools.padded(o, g.ns)
-        c = dict(
+        return dict(
             F=F,
             W=o.W,
             L=self.kernel.L,
             K=self.K,
             top=float(g.domain.top_km),
-            u=u,
-            wu=wu,
             h=g.h.astype(F),
             chi=g.chi.astype(F),
             hr=prof[:, 0].astype(F),
@@ -90,16 +115,11 @@ class Context:
             strato_density=atmosphere.strato_profile(st, g.h).astype(F),
             sun_h=self.sun.h.astype(F),
             sun_col=self.sun.col.astype(F),
-            gm=self.pools.tensor(o, g.ns).astype(F),
-            gd=gd.astype(F),
-            gv=gv.astype(F),
             profile=dict(air_H=st.air_height_km, aer_H=st.aerosol_height_km, o3_c=st.ozone_center_km,
                          o3_s=st.ozone_sigma_km, no2_c=st.no2_center_km, no2_s=st.no2_sigma_km,
                          st_c=st.sulfate_center_km, st_s=st.sulfate_sigma_km),
             combine=True,
         )
-        self._constants[key] = c
-        return c
 
     def eigen(self):
         """Per-row composition factors [rows, W] that turn incident moments into the source

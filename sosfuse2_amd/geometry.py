@@ -447,27 +447,32 @@ class GroundPools:
         tau = self.column[:, :5] @ optics.ext + self.column[:, 5:6] * optics.strato_ext[None, :]
         return optics.albedo[None, :] * self.factor[:, None] * np.exp(-tau)
 
-    def tensor(self, optics, cells):
+    def tensor(self, optics, cells, gv=None):
         """gm[C, 4, W]: the Rayleigh tensor moments of the ground-reflected light at each cell, times 3/(8 pi) ext_R
-        (`engine_core.py:135-144`)."""
-        gv = self.values(optics)
+        (`engine_core.py:135-144`). gv: the pool values when already at hand. The pools of a cell are contiguous, so
+        the per-cell sums are one sparse product with the cells' CSR pattern (off)."""
+        import scipy.sparse
+
+        gv = self.values(optics) if gv is None else gv
         loc = self.loc
         factor = np.stack(
             [(1 - loc[:, 0] ** 2) / 2, (1 - loc[:, 1] ** 2) / 2, (1 - loc[:, 2] ** 2) / 2, -loc[:, 0] * loc[:, 2] / 2],
             -1,
         )
+        P = len(loc)
         out = np.zeros((cells, 4, optics.W))
-        contrib = factor[:, :, None] * gv[:, None, :]
-        seg = np.repeat(np.arange(cells), np.diff(self.off))
-        np.add.at(out, seg, contrib)
+        if P:
+            for j in range(4):
+                seg = scipy.sparse.csr_matrix((factor[:, j], np.arange(P), self.off[: cells + 1]), shape=(cells, P))
+                out[:, j] = seg @ gv
         return out * (3 / (8 * math.pi)) * optics.ext[0][None, None, :]
 
-    def padded(self, optics, cells):
+    def padded(self, optics, cells, gv=None):
         """gd[C, ng, 3], gv[C, ng, W] zero-padded per cell (`source_solver.py:1281-1287`)."""
         ng = int(max(np.diff(self.off), default=0))
         gd = np.zeros((cells, max(ng, 1), 3))
         gvp = np.zeros((cells, max(ng, 1), optics.W))
-        gv = self.values(optics)
+        gv = self.values(optics) if gv is None else gv
         for c in range(cells):
             a, b = self.off[c : c + 2]
             gd[c, : b - a] = loc = self.loc[a:b]

@@ -41,6 +41,7 @@ _SIGNATURES = {
     'sos_ground_records': [ctypes.POINTER(Grid), c_i32, ctypes.POINTER(c_f32), ctypes.POINTER(c_f32), c_i64, P, P, P, P,
                            P, P, P],
     'sos_plan_basis': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, P],
+    'sos_ground_pools': [ctypes.POINTER(Grid), P, P, P, P, P, P, P],
     'sos_sh_records': [ctypes.POINTER(Grid), P, P, c_i32, c_i64, P, P, P, P, P, P, P, P, P],
     'sos_project': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, c_i32, P, P, P],
     'sos_compose': [ctypes.POINTER(Grid), c_i32, P, P, P, P, P],

@@ -174,7 +174,7 @@ class DeviceConstants:
     """Device copy of one context's constants for one node count (the reference's Kernel.setup + constants_device)."""
 
     def __init__(self, ctx, nq, L, G, device):
-        c = ctx.constants(nq)
+        c = ctx.constants(nq, padded=False)
         o = ctx.optics
         t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(device)
         self.G, self.L, self.nq = G, L, nq
@@ -194,11 +194,12 @@ class DeviceConstants:
         self.ha, self.sd = t(c['ha'], 'f4'), t(c['strato_density'], 'f4')
         self.sun_h, self.sun_col = t(c['sun_h'], 'f4'), t(c['sun_col'], 'f4')
         pools = ctx.pools
-        self.goff = t(pools.off, 'i8')
-        self.gdir = t(pools.loc if len(pools.loc) else np.zeros((1, 3)), 'f4')
-        gv = pools.values(o) if len(pools.loc) else np.zeros((1, o.W))
-        self.gval = t(_pad_lanes(gv, G), 'f4')
-        self.gm = t(_pad_lanes(c['gm'], G), 'f4')
+        pd = pool_device(pools, device)
+        self.goff, self.gdir = pd['off'], pd['loc32']
+        P = len(pools.loc)
+        # the state-dependent pool values and tensor moments are computed on the device (sos_ground_pools, below)
+        self.gval = torch.zeros((max(P, 1), G), dtype=torch.float32, device=device)
+        self.gm = torch.zeros((ctx.geo.ns, 4, G), dtype=torch.float32, device=device)
         self.factor = t(_pad_lanes(ctx.eigen(), G), 'f4')
         # one asymmetry per launch (a single-state chunk): first light may share phase evaluations across lanes
         self.uniform_phase = bool(np.all(lanes[6] == lanes[6][0]) and np.all(lanes[11] == lanes[11][0]))
@@ -220,6 +221,31 @@ class DeviceConstants:
             *(hip.ptr(x) for x in (self.gl_u, self.gl_w, self.h, self.chi, self.hr, self.lane)),
             len(c['sun_h']), c['sun_col'].shape[1], hip.ptr(self.sun_h), hip.ptr(self.sun_col), hip.ptr(self.goff),
             hip.ptr(self.gdir), hip.ptr(self.gval), hip.ptr(self.gm))
+        if P:
+            hip.call('sos_ground_pools', self.struct, hip.ptr(pd['off']), hip.ptr(pd['column']), hip.ptr(pd['factor']),
+                     hip.ptr(pd["loc"]), hip.ptr(self.gval), hip.ptr(self.gm),
+                     torch.cuda.current_stream(device).cuda_stream)
+
+
+_POOLS = {}
+
+
+def pool_device(pools, device):
+    """Device copy of a grid's ground pools (state-independent: CSR offsets, FP64 column vectors, geometric factors
+    and directions, FP32 directions for the transport), made once per pools object."""
+    key = (id(pools), str(device))
+    hit = _POOLS.get(key)
+    if hit is not None and hit[0] is pools:
+        return hit[1]
+    t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(device)
+    P = len(pools.loc)
+    d = dict(off=t(pools.off, 'i8'),
+             loc32=t(pools.loc if P else np.zeros((1, 3)), 'f4'),
+             loc=t(pools.loc if P else np.zeros((1, 3)), 'f8'),
+             column=t(pools.column if P else np.zeros((1, 6)), 'f8'),
+             factor=t(pools.factor if P else np.zeros(1), 'f8'))
+    _POOLS[key] = (pools, d)
+    return d
 
 
 class Kernel:
@@ -227,6 +253,7 @@ class Kernel:
     'cupy'; this build adds 'hip' and runs nothing else)."""
 
     backend = 'hip'
+    CONTEXTS = 4  # contexts whose device constants stay cached
 
     def __init__(self, backend='hip', L=16, block_size=256, device='cuda', mixed=False, records=True):
         if backend != 'hip':
@@ -265,9 +292,16 @@ class Kernel:
         return x.detach().cpu().numpy() if torch.is_tensor(x) else np.asarray(x)
 
     def constants(self, ctx, nq):
+        """Device constants of a context for a node count, cached for the most recent CONTEXTS contexts (the cache
+        holds the context, so an id is not reused while its entry lives)."""
         key = (id(ctx), nq)
         dc = self._constants.get(key)
         if dc is None:
+            live = list(dict.fromkeys(k[0] for k in self._constants))
+            if id(ctx) not in live and len(live) >= self.CONTEXTS:
+                old = live[0]
+                for k in [k for k in self._constants if k[0] == old]:
+                    del self._constants[k]
             dc = DeviceConstants(ctx, nq, self.L, lane_count(ctx.W), self.device)
             self._constants[key] = (ctx, dc)
             return dc

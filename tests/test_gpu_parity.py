@@ -152,6 +152,28 @@ def test_fp32_table_within_1e4_of_fp64(gpu):
     assert err < 1e-4
 
 
+def test_ground_pools_on_device_match_host(gpu):
+    """sos_ground_pools (pool values and per-cell tensor moments from the lane table, FP64 sums) against the host's
+    GroundPools.values / tensor (the restatement of engine_core.py:103-147 the oracle reads), for a two-state lane
+    set: equal to FP32 rounding of the FP32 lane table's extinctions (relative 1e-6 of each lane's largest)."""
+    s, scope, _ = setup_case(3, 16)
+    states = [st for st, _, _ in atmosphere.sample_states(2, seed=7, top_km=100.0)]
+    o = atmosphere.concat_optics([atmosphere.lane_optics(states[0], [430.0, 520.0, 610.0]),
+                                  atmosphere.lane_optics(states[1], [450.0, 600.0])])
+    ctx = scope.context(o)
+    dc = s.kernel.constants(ctx, ctx.geo.path_n)
+    gv_host = ctx.pools.values(o)
+    gm_host = ctx.pools.tensor(o, ctx.geo.ns)
+    # padding lanes repeat the last lane (as the lane table does)
+    pad = np.r_[np.arange(o.W), np.full(dc.G - o.W, o.W - 1)]
+    gv_host, gm_host = gv_host[:, pad], gm_host[..., pad]
+    gv_dev = dc.gval.cpu().numpy().astype('f8')
+    gm_dev = dc.gm.cpu().numpy().astype('f8')
+    assert len(gv_host) > 0
+    assert np.all(np.abs(gv_dev - gv_host) <= 1e-6 * np.abs(gv_host).max(0) + 1e-30)
+    assert np.all(np.abs(gm_dev - gm_host) <= 1e-6 * np.abs(gm_host).max((0, 1)) + 1e-30)
+
+
 @pytest.mark.parametrize('L', [8, 32])
 def test_basis_table_matches_oracle_basis(gpu, L):
     """sos_plan_basis against the oracle's FP64 basis (oracle/arith.py basis, the recurrence of arithmetic.py:86-109)
