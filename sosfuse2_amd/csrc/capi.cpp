@@ -153,6 +153,9 @@ int sos_ground_records(const sos_grid *g, int32_t groups, const float *group_g, 
     a.node_dir = reinterpret_cast<const float4 *>(node_dir_dev); a.goff = g->goff_dev; a.gdir = g->gdir_dev;
     a.gval = g->gval_dev; a.lane = g->lane_dev; a.records = records_dev;
     a.groups = groups;
+    a.pool_max = g->pool_max;
+    if (g->pool_max < 0 || g->pool_max > 640)
+        return arg_error("sos_ground_records: pool_max %d (0 to 640 pools per cell)", g->pool_max);
     for (int i = 0; i < 4; ++i) {
         a.gg[i] = i < groups ? group_g[i] : 0.0f;
         a.sgg[i] = i < groups ? group_sg[i] : 0.0f;

@@ -146,6 +146,7 @@ struct GroundArgs {
     float *records;            // [nodes * 4][G]
     int groups;                // phase groups: distinct (aerosol g, sulfate g) among the lanes, at most 4
     float gg[4], sgg[4];
+    int pool_max;              // largest pool count of a cell
 };
 
 struct GemmArgs {

@@ -15,22 +15,28 @@
 namespace sos {
 int set_error(const char *fmt, ...);
 
-template <int G>
-__global__ __launch_bounds__(G * 4) void ground_records_kernel(GroundArgs a) {
-    constexpr int V = 16, PC = 128, PS = PC + 4;
-    constexpr int NT = G * 4;
-    __shared__ float B[PC][G];
+// Layout: a workgroup takes an item (at most 2048 visits of one cell) in rounds of 32 visits. Each wavefront
+// (16-lane N-tile nt, 16-visit M-tile mt) holds the cell's pool radiances of its lanes as the MFMA B operand in
+// registers (NCH chunks of PC pools, loaded once per item), so the pools are staged once however many there are;
+// per round and chunk the 32 x PC phase tile of each phase group is generated in LDS by all threads and multiplied.
+template <int G, int NCH>
+__global__ __launch_bounds__(64 * (G / 16) * 2) void ground_records_kernel(GroundArgs a) {
+    constexpr int NW = G / 16;               // N-tiles
+    constexpr int NT = 64 * NW * 2;          // two M-tiles per N-tile
+    constexpr int V = 32, PC = 160, PS = PC + 4;
     __shared__ float PT[V][PS], PU[V][PS];
-    __shared__ float gd[PC][3];
+    __shared__ float gd[NCH * PC][3];
     __shared__ float4 sdir[V];
     __shared__ int sid[V];
-    const int tid = threadIdx.x, lane = tid & 63, nt = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nt = wave % NW, mt = wave / NW;
     const int col = nt * 16 + (lane & 15);
+    const int kk = lane >> 4, row = mt * 16 + (lane & 15);
     const int64_t item = blockIdx.x;
     const int cell = a.item_cell[item];
     const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
     const int64_t p0 = a.goff[cell];
-    const int np = (int)(a.goff[cell + 1] - p0);
+    const int np = (int)min((int64_t)(NCH * PC), a.goff[cell + 1] - p0);
     const int chunks = (np + PC - 1) / PC;
     const float asca = a.lane[SOS_LANE_ASCA * G + col], ssca = a.lane[SOS_LANE_SSCA * G + col];
     // this lane's phase group
@@ -40,19 +46,19 @@ __global__ __launch_bounds__(G * 4) void ground_records_kernel(GroundArgs a) {
         for (int i = 0; i < a.groups; ++i)
             if (a.gg[i] == lg && (!a.strato || a.sgg[i] == lsg)) { mygroup = i; break; }
     }
-    auto stage = [&](int ch) {
-        for (int e = tid; e < PC * G; e += NT) {
-            const int p = e / G, w = e % G;
-            const int pp = ch * PC + p;
-            B[p][w] = pp < np ? a.gval[(p0 + pp) * G + w] : 0.0f;
+    // B operand: bq[ch][s] = gval[pool ch PC + 4 s + kk][col] (0 past the cell's pools)
+    float bq[NCH][PC / 4];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch)
+#pragma unroll
+        for (int s = 0; s < PC / 4; ++s) {
+            const int pp = ch * PC + 4 * s + kk;
+            bq[ch][s] = pp < np ? a.gval[(p0 + pp) * G + col] : 0.0f;
         }
-        for (int e = tid; e < PC * 3; e += NT) {
-            const int p = e / 3, c = e % 3;
-            const int pp = ch * PC + p;
-            gd[p][c] = pp < np ? a.gdir[(p0 + pp) * 3 + c] : 0.0f;
-        }
-    };
-    if (chunks == 1) stage(0);
+    for (int e = tid; e < NCH * PC * 3; e += NT) {
+        const int p = e / 3;
+        gd[p][e % 3] = p < np ? a.gdir[(p0 + p) * 3 + e % 3] : 0.0f;
+    }
     for (int64_t r0 = v0; r0 < v1; r0 += V) {
         const int nv = (int)min((int64_t)V, v1 - r0);
         if (tid < V) {
@@ -67,11 +73,9 @@ __global__ __launch_bounds__(G * 4) void ground_records_kernel(GroundArgs a) {
         }
         __syncthreads();
         floatx4 acct = floatx4{0.f, 0.f, 0.f, 0.f}, accs = floatx4{0.f, 0.f, 0.f, 0.f};
-        for (int ch = 0; ch < chunks; ++ch) {
-            if (chunks > 1) {
-                stage(ch);
-                __syncthreads();
-            }
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            if (ch >= chunks) break;
             const int n = min(PC, np - ch * PC);
             for (int grp = 0; grp < a.groups; ++grp) {
                 const HgConst ht = hg_const(a.gg[grp]), hs = hg_const(a.sgg[grp]);
@@ -80,8 +84,9 @@ __global__ __launch_bounds__(G * 4) void ground_records_kernel(GroundArgs a) {
                     float pt = 0.0f, ps = 0.0f;
                     if (v < nv && p < n) {
                         const float4 dv = sdir[v];
-                        const float m0 = dv.x * gd[p][2] + dv.y * gd[p][0];
-                        const float m1 = dv.z * gd[p][1];
+                        const float *g3 = gd[ch * PC + p];
+                        const float m0 = dv.x * g3[2] + dv.y * g3[0];
+                        const float m1 = dv.z * g3[1];
                         const float up = clampf(m0 + m1, -1.0f, 1.0f), dn = clampf(m0 - m1, -1.0f, 1.0f);
                         pt = (hg_fast(up, ht) + hg_fast(dn, ht)) * 0.5f;
                         if (a.strato) ps = (hg_fast(up, hs) + hg_fast(dn, hs)) * 0.5f;
@@ -93,12 +98,10 @@ __global__ __launch_bounds__(G * 4) void ground_records_kernel(GroundArgs a) {
                 // every wavefront runs the products (the MFMA needs the whole wavefront); lanes of other groups
                 // discard them
                 floatx4 t = floatx4{0.f, 0.f, 0.f, 0.f}, u = floatx4{0.f, 0.f, 0.f, 0.f};
-                const int kk = lane >> 4, row = lane & 15;
-#pragma unroll 8
+#pragma unroll
                 for (int s = 0; s < PC / 4; ++s) {
-                    const float b = B[4 * s + kk][col];
-                    t = __builtin_amdgcn_mfma_f32_16x16x4f32(PT[row][4 * s + kk], b, t, 0, 0, 0);
-                    if (a.strato) u = __builtin_amdgcn_mfma_f32_16x16x4f32(PU[row][4 * s + kk], b, u, 0, 0, 0);
+                    t = __builtin_amdgcn_mfma_f32_16x16x4f32(PT[row][4 * s + kk], bq[ch][s], t, 0, 0, 0);
+                    if (a.strato) u = __builtin_amdgcn_mfma_f32_16x16x4f32(PU[row][4 * s + kk], bq[ch][s], u, 0, 0, 0);
                 }
                 if (grp == mygroup) {
                     acct = acct + t;
@@ -109,13 +112,13 @@ __global__ __launch_bounds__(G * 4) void ground_records_kernel(GroundArgs a) {
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int row = 4 * (lane >> 4) + v;
-            if (row < nv) {
+            const int r = mt * 16 + 4 * (lane >> 4) + v;
+            if (r < nv) {
                 float prof[5], st;
-                profiles(a.shape, sdir[row].w, prof, st);
+                profiles(a.shape, sdir[r].w, prof, st);
                 float rec = (prof[1] * asca) * acct[v];
                 if (a.strato) rec = rec + (st * ssca) * accs[v];
-                a.records[(int64_t)sid[row] * G + col] = rec;
+                a.records[(int64_t)sid[r] * G + col] = rec;
             }
         }
         __syncthreads();
@@ -163,20 +166,33 @@ int pool_values_launch(int cells, int G, const int64_t *off, const double *colum
     return SOS_OK;
 }
 
-template <int G>
+template <int G, int NCH>
 static int ground_launch(const GroundArgs &a, hipStream_t s) {
     if (a.items <= 0) return SOS_OK;
-    hipLaunchKernelGGL((ground_records_kernel<G>), dim3((unsigned)a.items), dim3(G * 4), 0, s, a);
+    hipLaunchKernelGGL((ground_records_kernel<G, NCH>), dim3((unsigned)a.items), dim3(64 * (G / 16) * 2), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error("ground_records launch: %s", hipGetErrorString(e));
     return SOS_OK;
 }
 
+template <int G>
+static int ground_chunks(const GroundArgs &a, hipStream_t s) {
+    // pools of a cell held in registers in chunks of 160
+    switch ((a.pool_max + 159) / 160) {
+        case 0:
+        case 1: return ground_launch<G, 1>(a, s);
+        case 2: return ground_launch<G, 2>(a, s);
+        case 3: return ground_launch<G, 3>(a, s);
+        case 4: return ground_launch<G, 4>(a, s);
+        default: return set_error("ground_records: %d pools in a cell (at most 640)", a.pool_max);
+    }
+}
+
 int ground_records_dispatch(const GroundArgs &a, hipStream_t s) {
     switch (a.G) {
-        case 16: return ground_launch<16>(a, s);
-        case 32: return ground_launch<32>(a, s);
-        default: return set_error("ground_records: lanes %d (16 or 32)", a.G);
+        case 16: return ground_chunks<16>(a, s);
+        case 32: return ground_chunks<32>(a, s);
+        default: return set_error("ground_records: lanes %d", a.G);
     }
 }
 

@@ -29,6 +29,7 @@ class Grid(ctypes.Structure):
         ('gl_u_dev', P), ('gl_w_dev', P), ('h_dev', P), ('chi_dev', P), ('hr_dev', P), ('lane_dev', P),
         ('sun_nh', c_i32), ('sun_nx', c_i32), ('sun_h_dev', P), ('sun_col_dev', P), ('goff_dev', P),
         ('gdir_dev', P), ('gval_dev', P), ('gm_dev', P), ('blocks_per_cu', c_i32), ('sh_tile', c_i32),
+        ('pool_max', c_i32),
     ]
 
 

@@ -220,7 +220,8 @@ class DeviceConstants:
             p['no2_c'], p['no2_s'], p['st_c'], p['st_s'],
             *(hip.ptr(x) for x in (self.gl_u, self.gl_w, self.h, self.chi, self.hr, self.lane)),
             len(c['sun_h']), c['sun_col'].shape[1], hip.ptr(self.sun_h), hip.ptr(self.sun_col), hip.ptr(self.goff),
-            hip.ptr(self.gdir), hip.ptr(self.gval), hip.ptr(self.gm))
+            hip.ptr(self.gdir), hip.ptr(self.gval), hip.ptr(self.gm), 0, 0,
+            int(np.diff(pools.off).max(initial=0)))
         if P:
             hip.call('sos_ground_pools', self.struct, hip.ptr(pd['off']), hip.ptr(pd['column']), hip.ptr(pd['factor']),
                      hip.ptr(pd["loc"]), hip.ptr(self.gval), hip.ptr(self.gm),
