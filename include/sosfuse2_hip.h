@@ -159,7 +159,7 @@ int sos_ground_pools(const sos_grid *grid, const int64_t *off_dev, const double 
 
 /* Basis table of a bundle (fixed by its geometry): basis_dev [rays][KZ] (KZ = K rounded up to a multiple of 16)
  * holds fp32(x_m P_lm(mu)) in the m-major K order k' = m (L + 1) - m (m - 1) / 2 + l - m, zero for K <= k' < KZ,
- * with mu = fp32 of the ray direction's component along the sun axis and x_m = sqrt 2 for m > 0: the normalised
+ * with mu the ray direction's component along the sun axis (FP64) and x_m = sqrt 2 for m > 0: the normalised
  * associated-Legendre part of the SH basis (the FP64 recurrence of arithmetic.py:86-109), which is the same at every
  * node of a straight ray. sos_sh_records multiplies it by each node's cos(m phi). */
 int sos_plan_basis(const sos_grid *grid, const sos_bundle *bundle, float *basis_dev, void *stream);
@@ -178,9 +178,11 @@ int sos_sh_records(const sos_grid *grid, const void *plane_dev, const float *sca
                    const float *basis_dev, float *records_dev, void *stream);
 
 /* Moments [groups][4 + K][lanes] and ground flux [groups][lanes] of the per-ray radiances rad_dev
- * [rays][lanes][4] (the bundle's groups are the destination cells). ground_only writes the flux only. */
+ * [rays][lanes][4] (the bundle's groups are the destination cells); the SH moments use the ray's basis row from
+ * basis_dev (the bundle's table of sos_plan_basis) times cos(m phi) of the ray's azimuth. ground_only writes the flux
+ * only (basis_dev may then be NULL). */
 int sos_project(const sos_grid *grid, const sos_bundle *bundle, const float *rad_dev, int32_t ground_only,
-                float *moments_dev, float *flux_dev, void *stream);
+                const float *basis_dev, float *moments_dev, float *flux_dev, void *stream);
 
 /* source[cells][rows][lanes] = factor[rows][lanes] * (moments + first)[cells][row'][lanes] with row' the moment of
  * each row; values below FLT_MIN in magnitude become 0. first_dev may be NULL. */

@@ -255,14 +255,15 @@ int sos_transport(const sos_grid *g, const sos_bundle *b, int32_t flags, int32_t
     return transport_dispatch(a, g->lanes, g->L, (hipStream_t)stream);
 }
 
-int sos_project(const sos_grid *g, const sos_bundle *b, const float *rad_dev, int32_t ground_only, float *moments_dev,
-                float *flux_dev, void *stream) {
+int sos_project(const sos_grid *g, const sos_bundle *b, const float *rad_dev, int32_t ground_only,
+                const float *basis_dev, float *moments_dev, float *flux_dev, void *stream) {
     if (!g || !b || !rad_dev || !flux_dev) return arg_error("sos_project: null argument");
-    if (!ground_only && !moments_dev) return arg_error("sos_project: moments output missing");
+    if (!ground_only && (!moments_dev || !basis_dev)) return arg_error("sos_project: moments output or basis missing");
     if (!lanes_ok(g->lanes) || !order_ok(g->L)) return arg_error("sos_project: lanes %d / L %d", g->lanes, g->L);
     ProjectArgs a;
     a.groups = b->groups; a.group_off = b->group_off_dev; a.dirs = b->dirs_dev; a.dloc = b->dloc_dev;
     a.weights = b->weights_dev; a.rad = rad_dev; a.lane = g->lane_dev; a.ground_only = ground_only;
+    a.basis = basis_dev;
     a.moments = moments_dev; a.flux = flux_dev;
     return project_dispatch(a, g->lanes, g->L, (hipStream_t)stream);
 }

@@ -112,6 +112,7 @@ struct ProjectArgs {
     const int64_t *group_off;
     const double *dirs;
     const float *dloc, *weights, *rad, *lane;
+    const float *basis;       // [rays][KZ] the bundle's basis table (sos_plan_basis)
     int ground_only;
     float *moments, *flux;
 };

@@ -25,8 +25,8 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
     __shared__ float Z[16][KP];
     __shared__ float V[16][G];
     __shared__ float cmv[16][L + 1];
-    __shared__ double seed[16][L + 1];
-    __shared__ double mus[16];
+    __shared__ short kl[KP];            // l-major position kidx(l, m) of m-major position k' (K for the padding)
+    __shared__ unsigned char km[KP];    // m of k'
     const int cell = blockIdx.x;
     const int tid = threadIdx.x;
     const int64_t r0 = a.group_off[cell], r1 = a.group_off[cell + 1];
@@ -48,6 +48,13 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
         }
     }
     if (a.ground_only) return;
+    for (int k = tid; k < KP; k += 256) {
+        int m = 0;
+        if (k < K)
+            while (moff<L>(m + 1) <= k) ++m;
+        km[k] = (unsigned char)m;
+        kl[k] = (short)(k < K ? kidx(k - moff<L>(m) + m, m) : K);
+    }
     const int wave = tid >> 6, lane = tid & 63;
     floatx4 acc[TPW][NT];
 #pragma unroll
@@ -60,24 +67,18 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
     for (int64_t c0 = r0; c0 < r1; c0 += 16) {
         const int n = (int)min((int64_t)16, r1 - c0);
         if (tid < 16) {
+            // cos(m phi) of the ray's azimuth about the sun axis (FP32 Chebyshev, arithmetic.py:112-119)
             const int i = tid;
-            double mu = 0.0, st = 0.0;
             float cp = 1.0f;
             if (i < n) {
                 const int64_t r = c0 + i;
                 const double dx = a.dirs[3 * r], dz = a.dirs[3 * r + 2];
-                mu = fmin(1.0, fmax(-1.0, dz));
-                st = sqrt(fmax(0.0, 1.0 - dz * dz));
+                const double st = sqrt(fmax(0.0, 1.0 - dz * dz));
                 cp = (float)(st > 1e-12 ? dx / fmax(st, 1e-30) : 1.0);
                 cp = clampf(cp, -1.0f, 1.0f);
             }
-            mus[i] = mu;
-            const double stc = sqrt(fmax(0.0, 1.0 - mu * mu));
-            double diag = 0.28209479177387814;
             float c0m = 1.0f, c1m = cp;
             for (int m = 0; m <= L; ++m) {
-                if (m) diag = (diag * -sqrt((2.0 * m + 1) / (2.0 * m))) * stc;
-                seed[i][m] = diag;
                 float cm;
                 if (m == 0) cm = 1.0f;
                 else if (m == 1) cm = cp;
@@ -96,29 +97,17 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
             V[i][w] = v;
         }
         __syncthreads();
-        for (int e = tid; e < 16 * (L + 1); e += 256) {
-            const int i = e / (L + 1), m = e % (L + 1);
-            const double mu = mus[i];
-            const float mult = m ? 1.41421356237309515f : 1.0f;
-            const float cm = cmv[i][m];
-            double z0 = seed[i][m];
-            if (i >= n) {
-                for (int l = m; l <= L; ++l) Z[i][kidx(l, m)] = 0.0f;
-                continue;
-            }
-            Z[i][kidx(m, m)] = ((float)(mult * z0)) * cm;
-            if (m < L) {
-                double z1 = (sqrt(2.0 * m + 3) * mu) * z0;
-                Z[i][kidx(m + 1, m)] = ((float)(mult * z1)) * cm;
-                for (int l = m + 2; l <= L; ++l) {
-                    const double aa = sqrt((4.0 * l * l - 1) / ((double)l * l - (double)m * m));
-                    const double bb = sqrt((2.0 * l + 1) * ((l - 1.0) * (l - 1.0) - (double)m * m) /
-                                           ((2.0 * l - 3) * ((double)l * l - (double)m * m)));
-                    const double z2 = (aa * mu) * z1 - bb * z0;
-                    Z[i][kidx(l, m)] = ((float)(mult * z2)) * cm;
-                    z0 = z1;
-                    z1 = z2;
-                }
+        // basis rows: the ray's row of the bundle's basis table (fp32(x_m P_lm(mu))) times cos(m phi), into the
+        // l-major order of the moments
+        for (int e = tid; e < 16 * (KP / 4); e += 256) {
+            const int i = e / (KP / 4), q = e % (KP / 4);
+            float4 z = float4{0.0f, 0.0f, 0.0f, 0.0f};
+            if (i < n) z = reinterpret_cast<const float4 *>(a.basis + (c0 + i) * KP)[q];
+            const float zz[4] = {z.x, z.y, z.z, z.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = 4 * q + u;
+                Z[i][kl[k]] = zz[u] * cmv[i][km[k]];
             }
         }
         if constexpr (KP > K)

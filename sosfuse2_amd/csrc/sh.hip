@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void plan_basis_kernel(int64_t rays, const dou
     for (int e = tid; e < V * (KZ - K); e += NT) Z[e / (KZ - K)][K + e % (KZ - K)] = 0.0f;
     __syncthreads();
     if (tid < V) {
-        const double mu = tid < nv ? (double)(float)dirs[3 * (r0 + tid) + 2] : 0.0;
+        const double mu = tid < nv ? fmin(1.0, fmax(-1.0, dirs[3 * (r0 + tid) + 2])) : 0.0;
         smu[tid] = mu;
         visit_seeds<L, V>(tid, mu, 1.0f, dd, seed, cmv);
     }

@@ -44,7 +44,7 @@ _SIGNATURES = {
     'sos_plan_basis': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, P],
     'sos_ground_pools': [ctypes.POINTER(Grid), P, P, P, P, P, P, P],
     'sos_sh_records': [ctypes.POINTER(Grid), P, P, c_i32, c_i64, P, P, P, P, P, P, P, P, P],
-    'sos_project': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, c_i32, P, P, P],
+    'sos_project': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, c_i32, P, P, P, P],
     'sos_compose': [ctypes.POINTER(Grid), c_i32, P, P, P, P, P],
     'sos_make_plane': [ctypes.POINTER(Grid), c_i32, P, P, P, c_i32, P, P, P],
     'sos_source_metrics': [c_i64, c_i32, P, P, c_u32, P, P],

@@ -409,6 +409,7 @@ class Kernel:
         moments = None
         if phase == 0:
             moments = torch.empty((db.groups, 4 + self.K, dc.G), dtype=torch.float32, device=self.device)
-        hip.call('sos_project', dc.struct, db.struct, hip.ptr(rad), int(phase == 1), hip.ptr(moments), hip.ptr(flux),
-                 self.stream())
+        basis = self.plan(db, dc).basis_table(self, dc, db) if phase == 0 else None
+        hip.call('sos_project', dc.struct, db.struct, hip.ptr(rad), int(phase == 1), hip.ptr(basis), hip.ptr(moments),
+                 hip.ptr(flux), self.stream())
         return moments, flux, None

@@ -260,16 +260,24 @@ def train(net, X, T, steps, batch=4096, lr=1e-3, seed=0, log=None):
     return hist
 
 
-def warm_start(net, s, solver, lit=1e-3):
-    """x0 [cells, rows, G] for Solver.solve: in sunlit cells (first-light trace >= lit x the lane's largest) the
-    first-light source + the network's multiple-scattering increment, with the tensor diagonal kept at or above its
-    first-light value; elsewhere 0, the cold start's value. Shadow cells are left out because their traces are tiny:
-    the log-space corner blend magnifies a relative error there, and a guess the iteration must undo costs more
-    evaluations than it saves. Lanes beyond W repeat the last lane (the padding rule of the lane table)."""
+def warm_start(net, s, solver, lit=1e-3, rows='all'):
+    """x0 [cells, rows, G] for Solver.solve: the first-light source, plus in sunlit cells (first-light trace >= lit x
+    the lane's largest) the network's multiple-scattering increment, with the tensor diagonal kept at or above its
+    first-light value. Shadow cells keep the first-light source because their traces are tiny: the log-space corner
+    blend magnifies a relative error there, and a guess the iteration must undo costs more evaluations than it saves.
+    rows='low' takes the increment only for the tensor rows and the isotropic (l = 0) SH rows and keeps the
+    first-light values of the higher orders, whose predictions are the least accurate. Lanes beyond W repeat the
+    last lane (the padding rule of the lane table)."""
     first = first_source(solver)
     X, scale = features(s, first)
     Y = net.predict(X)
     C, W = s.geo.ns, s.W
+    if rows == 'low':
+        keep = torch.zeros(first.shape[1], dtype=torch.bool, device=first.device)
+        keep[0:11] = True
+        if first.shape[1] > 10 + s.K:
+            keep[10 + s.K] = True
+        Y = Y * keep[None, :].to(Y.dtype)
     x0 = torch.empty_like(first)
     for w in range(first.shape[2]):
         wl = min(w, W - 1)
@@ -277,4 +285,4 @@ def warm_start(net, s, solver, lit=1e-3):
     x0[:, 0:3] = torch.maximum(x0[:, 0:3], first[:, 0:3])
     trace = first[:, 0:3].sum(1)
     sunlit = trace.abs() >= lit * trace.abs().amax(0, keepdim=True)
-    return torch.where(sunlit[:, None, :], x0, torch.zeros_like(x0))
+    return torch.where(sunlit[:, None, :], x0, first)

@@ -68,9 +68,12 @@ def test_training_lowers_the_loss(gpu):
 
 def test_warm_start_saves_evaluations(gpu):
     """Train on solves of seeded states of the batched distribution (small grid, three wavelengths), then solve
-    held-out states cold and warm-started: both reach the same tables (both iterate to 1e-5). The evaluation counts
-    of both are printed: that is what the network gains (or loses) over the cold start on this grid; the test bounds
-    the loss at 50 % more evaluations in total, so a warm start can never stall the solve."""
+    held-out states cold and warm-started: both reach the same tables (both iterate to 1e-5). What the network gains
+    is measured two ways and printed: its start is closer to the converged source than the first-light start (the
+    relative L2 distance over the sunlit cells' tensor and isotropic rows, asserted at least 2x smaller), and the
+    evaluation counts of both solves (the iteration's gain; the test bounds a loss at 50 % more evaluations in total,
+    so a warm start can never stall the solve). The start is the low-order form (tensor and isotropic rows): on this
+    coarse grid the iteration of some states is erratic, and the full form's higher orders cost evaluations there."""
     cfg = dict(nr=10, h=[0.0, 0.2, 0.6, 1.5, 3.0, 6.0, 12.0, 25.0, 50.0, 100.0], chi_grid='tiny', nmu=1, nphi=2,
                path_n=4, L=16, top_km=100.0, observer_km=0.1, query_n=6, anderson_depth=4)
     wl = atmosphere.log_spaced_wavelengths(3, 420, 680)
@@ -96,7 +99,7 @@ def test_warm_start_saves_evaluations(gpu):
     hs = [-3.0, -6.0]
     el = 90 - (np.arange(3) + 0.5) * 30
     az = (np.arange(4) + 0.5) * 45
-    cold, warm = [], []
+    cold, warm, d_first, d_warm = [], [], [], []
     for st in test_states:
         s = scope.context(atmosphere.lane_optics(st, wl))
         try:
@@ -104,12 +107,22 @@ def test_warm_start_saves_evaluations(gpu):
         except RuntimeError:
             continue
         sv = scope.solver(s, log=None)
-        xw, iw = sv.solve(60, 1e-5, x0=learned.warm_start(net, s, sv))
+        x0 = learned.warm_start(net, s, sv, rows='low')
+        first = learned.first_source(sv)
+        tr = first[:, 0:3].sum(1)
+        sunlit = (tr.abs() >= 1e-3 * tr.abs().amax(0, keepdim=True))[:, None, : s.W]
+        ref = xc[:, 0:11, : s.W]
+        dist = lambda y: float(torch.sqrt(((y[:, 0:11, : s.W] - ref) ** 2 * sunlit).sum() / ((ref ** 2) * sunlit).sum()))
+        d_first.append(dist(first))
+        d_warm.append(dist(x0))
+        xw, iw = sv.solve(60, 1e-5, x0=x0)
         cold.append(ic['evaluations'])
         warm.append(iw['evaluations'])
         tc, _ = api.make_table(s, xc, hs, el, az, log=None)
         tw, _ = api.make_table(s, xw, hs, el, az, log=None)
         assert np.max(np.abs(tw[..., 0] - tc[..., 0]) / tc[..., 0]) < 1e-4
     print(f'evaluations cold {cold} warm {warm}')
+    print(f'distance to the converged source: first light {np.round(d_first, 4)}, network {np.round(d_warm, 4)}')
     assert len(cold) >= 4
+    assert np.mean(d_warm) < 0.5 * np.mean(d_first)
     assert sum(warm) <= 1.5 * sum(cold)
