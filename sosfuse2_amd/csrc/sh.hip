@@ -207,17 +207,17 @@ __device__ inline void cos_chain(float cp, float *c) {
 }
 
 // Per-round node data, prepared two rounds ahead by 16 threads: the node ids (record rows), the rays (basis rows)
-// and cos(m phi) of the round's nodes, in slot r & 3.
-template <int L, int V>
+// and cos(m phi) of the round's nodes, in slot r mod S.
+template <int L, int V, int S = 4>
 struct RoundMeta {
-    int id[4][V];
-    int ray[4][V];
-    float cm[4][V][L + 1];
+    int id[S][V];
+    int ray[S][V];
+    float cm[S][V][L + 1];
 };
 
 // Thread v < V: node v of the round starting at sorted position r0 into slot q (id -1 and ray -1 past v1).
-template <int L, int V>
-__device__ inline void meta_fill(RoundMeta<L, V> &mt, int q, int v, bool in, int id, int ray, float cp) {
+template <int L, int V, int S>
+__device__ inline void meta_fill(RoundMeta<L, V, S> &mt, int q, int v, bool in, int id, int ray, float cp) {
     mt.id[q][v] = in ? id : -1;
     mt.ray[q][v] = in ? ray : -1;
     cos_chain<L>(in ? cp : 1.0f, mt.cm[q][v]);
@@ -254,30 +254,46 @@ __device__ inline void rows_to_z(const float4 *pr, const Meta &mt, int q, const 
 }
 
 // FP32 MFMA products of one round: d = Z rows (16 nodes) x this wavefront's B operand (b[4 i + u] = plane column at
-// k = 16 i + 4 (lane / 16) + u), four independent accumulators. The A operands (one ds_read_b128 per group of four
-// products) are read D groups ahead, and the schedule is pinned to (four products, one read) per group so that the
-// LDS latency stays behind the products in flight.
+// k = 16 i + 4 (lane / 16) + u). The K range is taken as two halves (groups 0 .. KQH - 1 and KQH .. KQ - 1, a group
+// being four MFMA k-steps) processed side by side, each into two accumulators (even and odd u), and the result is
+// (half 0) + (half 1): the same order as the quad form, whose wavefronts each hold one half, so every form gives
+// the same records bit for bit. The A operands (one ds_read_b128 per group) are read D group pairs ahead, with the
+// schedule pinned to (eight products, two reads) per pair so that the LDS latency stays behind the products.
 template <int KQ, int ZS, int D>
 __device__ inline floatx4 sh_products(const float *Zt, const float *b, int lane) {
+    constexpr int KQH = (KQ + 1) / 2;
     floatx4 acc[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
     const float *zrow = Zt + (lane & 15) * ZS + 4 * (lane >> 4);
-    floatx4 av[D];
+    floatx4 av[D][2];
 #pragma unroll
-    for (int j = 0; j < D; ++j) {
-        av[j] = *reinterpret_cast<const floatx4 *>(zrow + 16 * j);
-        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // reads of groups 0 .. D - 1 first
+    for (int j = 0; j < D && j < KQH; ++j) {
+        av[j][0] = *reinterpret_cast<const floatx4 *>(zrow + 16 * j);
+        if (KQH + j < KQ) av[j][1] = *reinterpret_cast<const floatx4 *>(zrow + 16 * (KQH + j));
+        if (KQH + j < KQ) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // reads of pairs 0 .. D - 1 first
+        else __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
     }
 #pragma unroll
-    for (int i = 0; i < KQ; ++i) {
-        const floatx4 cur = av[i % D];
-        if (i + D < KQ) av[i % D] = *reinterpret_cast<const floatx4 *>(zrow + 16 * (i + D));
+    for (int i = 0; i < KQH; ++i) {
+        const floatx4 c0 = av[i % D][0], c1 = av[i % D][1];
+        if (i + D < KQH) {
+            av[i % D][0] = *reinterpret_cast<const floatx4 *>(zrow + 16 * (i + D));
+            if (KQH + i + D < KQ) av[i % D][1] = *reinterpret_cast<const floatx4 *>(zrow + 16 * (KQH + i + D));
+        }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur[u], b[4 * i + u], acc[u], 0, 0, 0);
-        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);                  // the four products of group i
-        if (i + D < KQ) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // then the read of group i + D
+        for (int u = 0; u < 4; ++u) {
+            acc[u & 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(c0[u], b[4 * i + u], acc[u & 1], 0, 0, 0);
+            if (KQH + i < KQ)
+                acc[2 + (u & 1)] =
+                    __builtin_amdgcn_mfma_f32_16x16x4f32(c1[u], b[4 * (KQH + i) + u], acc[2 + (u & 1)], 0, 0, 0);
+        }
+        if (KQH + i < KQ) __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);  // the products of pair i
+        else __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+        if (i + D < KQH) {  // then the reads of pair i + D
+            if (KQH + i + D < KQ) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+            else __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
     }
     return (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
@@ -315,13 +331,14 @@ __device__ inline NodeRef node_ref(const ShArgs &a, int64_t r, int64_t v1) {
 // FP32 SH records, one workgroup per work item (a run of nodes of one base cell), 4 x G/16 wavefronts: wavefront
 // (c, nt) holds corner c's coefficients of lanes 16 nt .. 16 nt + 15 in registers (whole K, about 250 VGPRs, two
 // wavefronts per SIMD, one workgroup per CU). Per round of 16 nodes the basis tile is the nodes' basis rows (read
-// from the bundle's table a round ahead) times their cos(m phi). Two forms, a tuning choice (the records are bitwise
-// the same):
+// from the bundle's table a round ahead) times their cos(m phi). Three forms, a tuning choice (the records are
+// bitwise the same):
 //   sh_records_kernel (flow): tile r is formed from the rows fetched during round r - 1, the rows of round r + 1 are
 //     fetched, 16 threads of the last wavefront prepare the node data of round r + 2, one barrier, then the
 //     products of all wavefronts; tiles are double-buffered and node data four-buffered, so the fetches run under
 //     the products and a wavefront that finishes its products goes on to the next tile at once;
-//   sh_records_step_kernel: node data, barrier, fetch and tile, barrier, products, barrier.
+//   sh_records_step_kernel: node data, barrier, fetch and tile, barrier, products, barrier;
+//   sh_records_quad_kernel: the flow's rounds on twice the wavefronts, each holding half of the K range.
 template <int G, int L>
 __global__ __launch_bounds__(G * 16) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void sh_records_kernel(ShArgs a) {
@@ -381,6 +398,128 @@ void sh_records_kernel(ShArgs a) {
         for (int v = 0; v < 4; ++v) {
             const int row = 4 * (lane >> 4) + v;
             if (row < nv) a.records[((int64_t)ids[row] * 4 + corner) * G + col] = d[v];
+        }
+    }
+}
+
+// Products over the groups g0 .. g0 + ng - 1 only (one wavefront's half of the K range) into two accumulators (even
+// and odd u), in the order of one half of sh_products; the steps past ng (an odd group count leaves the second half
+// one short) multiply zeros and add exactly nothing.
+template <int NG, int ZS, int D>
+__device__ inline floatx4 sh_products_part(const float *Zt, const float *b, int lane, int g0, int ng) {
+    floatx4 acc[2];
+    acc[0] = acc[1] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const float *zrow = Zt + (lane & 15) * ZS + 4 * (lane >> 4) + 16 * g0;
+    floatx4 av[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        av[j] = j < ng ? *reinterpret_cast<const floatx4 *>(zrow + 16 * j) : floatx4{0.f, 0.f, 0.f, 0.f};
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < NG; ++i) {
+        // groups past ng (the shorter second half when the group count is odd) read nothing: their B is zero
+        const floatx4 cur = av[i % D];
+        if (i + D < NG) av[i % D] = i + D < ng ? *reinterpret_cast<const floatx4 *>(zrow + 16 * (i + D)) : floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            acc[u & 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur[u], b[4 * i + u], acc[u & 1], 0, 0, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+        if (i + D < NG) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+    return acc[0] + acc[1];
+}
+
+// Quad form: 8 G/16 wavefronts (4 corners x G/16 lane tiles x 2 halves of the K range), 72 B registers each at
+// L = 32, four wavefronts per SIMD at 32 lanes. Per round every thread fetches its share of the round's basis rows
+// and forms the tile (the other wavefronts of its SIMD, still in their products, cover the fetch), 16 threads
+// prepare the node data of round r + 2, one barrier, then the products. The second half's partial products of
+// round r go through LDS and are added by the first half after the barrier of round r + 1, so the sum is
+// (half 0) + (half 1) as in sh_products. Tiles and partials are double-buffered; the node ids of round r are read
+// after the barrier of round r + 1, so node data takes eight slots.
+template <int G, int L>
+__global__ __launch_bounds__(G * 32) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void sh_records_quad_kernel(ShArgs a) {
+    constexpr int K = (L + 1) * (L + 2) / 2;
+    constexpr int KZ = (K + 15) / 16 * 16;
+    constexpr int KQ = KZ / 16;
+    constexpr int KQH = (KQ + 1) / 2;
+    constexpr int NWB = G / 16;
+    constexpr int NC = 4 * NWB;               // (corner, tile) pairs
+    constexpr int NT = 64 * NC * 2;
+    constexpr int V = 16;
+    constexpr int ZS = KZ + 4;
+    constexpr int SW = NT - 64;
+    constexpr int PT = (V * KZ / 4 + NT - 1) / NT;
+    __shared__ __attribute__((aligned(16))) float Z[2][V][ZS];
+    __shared__ RoundMeta<L, V, 8> mt;  // ids of round r are read after the barrier of round r + 1: eight slots
+    __shared__ __attribute__((aligned(4))) unsigned char km[KZ];
+    __shared__ floatx4 red[2][NC][64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int kh = wave / NC, wc = wave % NC;
+    const int corner = wc / NWB, nt = wc % NWB;
+    const int col = nt * 16 + (lane & 15);
+    const int64_t item = blockIdx.x;
+    const int cell = corner_cell(a.item_cell[item], corner, a.nc);
+    const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
+    const int rounds = (int)((v1 - v0 + V - 1) / V);
+    const bool meta = tid >= SW && tid < SW + V;
+    stage_km<L, KZ>(km, NT);
+    constexpr int NG = KQH;
+    const int g0 = kh * KQH;
+    const int ng = min(KQH, KQ - g0);  // groups of this half (KQH or fewer for the second half)
+    float b[4 * NG];
+    {
+        const float *pl = reinterpret_cast<const float *>(a.plane) + (int64_t)cell * K * G + col;
+#pragma unroll
+        for (int i = 0; i < NG; ++i)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = 16 * (g0 + i) + 4 * (lane >> 4) + u;
+                b[4 * i + u] = (i < ng && k < K) ? pl[(int64_t)k * G] : 0.0f;
+            }
+    }
+    if (tid < 2 * V) {
+        const NodeRef n = node_ref(a, v0 + tid, v1);
+        meta_fill<L, V, 8>(mt, tid / V, tid % V, n.id >= 0, n.id, n.ray, n.cp);
+    }
+    NodeRef nn{-1, -1, 1.0f};
+    if (meta) nn = node_ref(a, v0 + 2 * V + (tid - SW), v1);
+    __syncthreads();
+    floatx4 dprev = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int r = 0; r <= rounds; ++r) {
+        const int cur = r & 1;
+        const int64_t r0 = v0 + (int64_t)r * V;
+        if (r < rounds) {
+            float4 pr[PT];
+            fetch_rows<V, KZ, NT, PT>(pr, a.basis, mt, r & 7, tid);
+            float (*Zc)[ZS] = Z[cur];
+            rows_to_z<V, KZ, NT, PT>(pr, mt, r & 7, km, tid, [&](int v, int k4, float4 z) {
+                reinterpret_cast<float4 *>(&Zc[v][0])[k4] = z;
+            });
+            if (meta && r + 2 < rounds) {
+                const int v = tid - SW;
+                meta_fill<L, V, 8>(mt, (r + 2) & 7, v, nn.id >= 0, nn.id, nn.ray, nn.cp);
+                nn = node_ref(a, r0 + 3 * V + v, v1);
+            }
+        }
+        __syncthreads();
+        // records of round r - 1: the first half adds the second half's partials
+        if (r > 0 && kh == 0) {
+            const int64_t p0 = v0 + (int64_t)(r - 1) * V;
+            const int nvp = (int)min((int64_t)V, v1 - p0);
+            const floatx4 d = dprev + red[(r - 1) & 1][wc][lane];
+            const int *ids = mt.id[(r - 1) & 7];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 4 * (lane >> 4) + v;
+                if (row < nvp) a.records[((int64_t)ids[row] * 4 + corner) * G + col] = d[v];
+            }
+        }
+        if (r < rounds) {
+            const floatx4 d = sh_products_part<NG, ZS, 2>(&Z[cur][0][0], b, lane, g0, ng);
+            if (kh == 1) red[cur][wc][lane] = d;
+            else dprev = d;
         }
     }
 }
@@ -621,6 +760,8 @@ static int sh_launch(const ShArgs &a, bool half, hipStream_t s) {
             hipLaunchKernelGGL((sh_records_split_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 16), 0, s, a);
         else if (a.tile == 1)
             hipLaunchKernelGGL((sh_records_step_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 16), 0, s, a);
+        else if (a.tile == 2)
+            hipLaunchKernelGGL((sh_records_quad_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 32), 0, s, a);
         else hipLaunchKernelGGL((sh_records_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 16), 0, s, a);
     }
     hipError_t e = hipGetLastError();

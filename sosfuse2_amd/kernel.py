@@ -280,7 +280,7 @@ class Kernel:
         self.metrics = dict(samples=0, launches=0)
         self.timed = None  # list of (name, start event, end event) while timing is on
         # launch shapes chosen by the autotuner (sosfuse2_amd.tune); results do not depend on them
-        self.tuning = dict(blocks_per_cu=0, node_chunk=1024, sh_tile=0)
+        self.tuning = dict(blocks_per_cu=0, node_chunk=1024, sh_tile=2)
         self.counters = None  # device uint64 [2] while counting is on
 
     def sync(self):

@@ -518,14 +518,14 @@ def test_autotuner_keeps_results(gpu):
 
 
 def test_sh_record_forms_are_bitwise_equal(gpu):
-    """The two FP32 SH-records workgroup forms (flow and stepwise) and three work-item sizes give the same records
-    bit for bit at L = 32 on 32 lanes (same basis arithmetic, same MFMA accumulation order)."""
+    """The three FP32 SH-records workgroup forms (flow, stepwise and quad) and three work-item sizes give the same
+    records bit for bit at L = 32 on 32 lanes (same basis arithmetic, same MFMA accumulation order)."""
     s, scope, _ = setup_case(20, 32)
     sv = scope.solver(s, log=None)
     f = torch.from_numpy(np.ascontiguousarray(padded(physical_field(s, 20), 32))).cuda()
     ref = None
     for nc in (256, 1024, 4096):
-        for st in (0, 1):
+        for st in (0, 1, 2):
             s.kernel.tuning.update(node_chunk=nc, sh_tile=st)
             g = sv.apply(f)
             if ref is None:
