@@ -403,24 +403,26 @@ void sh_records_kernel(ShArgs a) {
 }
 
 // Products over the groups g0 .. g0 + ng - 1 only (one wavefront's half of the K range) into two accumulators (even
-// and odd u), in the order of one half of sh_products; the steps past ng (an odd group count leaves the second half
-// one short) multiply zeros and add exactly nothing.
+// and odd u), in the order of one half of sh_products. The steps past ng (an odd group count leaves the second half
+// one short) read the half's last group again, finite values against a zero B: they add exactly nothing. The reads
+// are unconditional (no select on ng), so each keeps its alias information.
 template <int NG, int ZS, int D>
-__device__ inline floatx4 sh_products_part(const float *Zt, const float *b, int lane, int g0, int ng) {
+__device__ inline floatx4 sh_products_part(const float *Z, int tile, const float *b, int lane, int g0, int ng) {
     floatx4 acc[2];
     acc[0] = acc[1] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const float *zrow = Zt + (lane & 15) * ZS + 4 * (lane >> 4) + 16 * g0;
+    // every read indexes Z itself (one integer offset), which keeps the reads' LDS alias information
+    const int zrow = tile + (lane & 15) * ZS + 4 * (lane >> 4) + 16 * g0;
+    const int glast = ng - 1;
     floatx4 av[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-        av[j] = j < ng ? *reinterpret_cast<const floatx4 *>(zrow + 16 * j) : floatx4{0.f, 0.f, 0.f, 0.f};
+        av[j] = *reinterpret_cast<const floatx4 *>(Z + (zrow + 16 * min(j, glast)));
         __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
     }
 #pragma unroll
     for (int i = 0; i < NG; ++i) {
-        // groups past ng (the shorter second half when the group count is odd) read nothing: their B is zero
         const floatx4 cur = av[i % D];
-        if (i + D < NG) av[i % D] = i + D < ng ? *reinterpret_cast<const floatx4 *>(zrow + 16 * (i + D)) : floatx4{0.f, 0.f, 0.f, 0.f};
+        if (i + D < NG) av[i % D] = *reinterpret_cast<const floatx4 *>(Z + (zrow + 16 * min(i + D, glast)));
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             acc[u & 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur[u], b[4 * i + u], acc[u & 1], 0, 0, 0);
@@ -428,6 +430,131 @@ __device__ inline floatx4 sh_products_part(const float *Zt, const float *b, int 
         if (i + D < NG) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
     }
     return acc[0] + acc[1];
+}
+
+// Node data of the round whose first sorted position is first into slot (3 V ints): lanes 0 .. 3 V - 1 of one
+// wavefront (loader), one LDS-direct 4-byte load each, lane l writing slot[l]; src is the lane's array (node ids,
+// rays or cos phi bits, by lane / V). Positions past the item read its last node (absent nodes are flagged by
+// position where the data is used).
+template <int V>
+__device__ inline void quad_fetch_nodes(int *slot, const int32_t *src, int64_t first, int64_t v1, bool loader, int lane) {
+    if (loader) {
+        const int64_t pos = min(first + lane % V, v1 - 1);
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + pos),
+                                         (__attribute__((address_space(3))) void *)slot, 4, 0, 0);
+    }
+}
+
+// Basis rows of one round (rays[v] of its 16 nodes) straight into raw with LDS-direct 16-byte loads: lane l of a
+// wavefront writes 16 bytes at the wavefront's base + 16 l, so thread tid's chunk j (c = tid + j NT: row c / (KZ / 4))
+// lands at raw[c]. Each thread later reads back only its own chunks, so its own wait for the loads is all the
+// ordering needed. Absent nodes (ray -1) read row 0: their products are never stored.
+template <int V, int KZ, int NT, int PT>
+__device__ inline void quad_fetch_rows(float4 *raw, const float *basis, const int *rays, int tid) {
+    asm volatile("" : "+v"(tid));  // the chunk positions are recomputed per call, not held across the rounds
+#pragma unroll
+    for (int j = 0; j < PT; ++j) {
+        const int c = tid + j * NT;
+        if (c < V * (KZ / 4)) {  // the same for the whole wavefront: V KZ / 4 is a multiple of 64
+            const int ray = max(rays[c / (KZ / 4)], 0);
+            const float *src = basis + (int64_t)ray * KZ + 4 * (c % (KZ / 4));
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                             (__attribute__((address_space(3))) void *)&raw[j * NT + (tid & ~63)], 16, 0, 0);
+        }
+    }
+}
+
+// The rounds of the quad form (below). Every LDS array comes in as a restrict pointer, so that each LDS access
+// carries alias-scope information: the compiler's wait insertion can then tell the reads of Z, the node data and the
+// partials from the LDS-direct loads into raw and ndata still in flight, instead of waiting for those loads before
+// every LDS read.
+template <int G, int L>
+__device__ __attribute__((always_inline)) inline void quad_rounds(
+    const ShArgs &a, float *__restrict__ Z, float4 *__restrict__ raw, int *__restrict__ ndata,
+    RoundMeta<L, 16, 8> *__restrict__ mtp, const unsigned char *__restrict__ km, floatx4 *__restrict__ red,
+    const float *b, int64_t v0, int64_t v1, int corner, int col, int kh, int wc, int g0, int ng) {
+    constexpr int K = (L + 1) * (L + 2) / 2;
+    constexpr int KZ = (K + 15) / 16 * 16;
+    constexpr int KQ = KZ / 16;
+    constexpr int KQH = (KQ + 1) / 2;
+    constexpr int NWB = G / 16;
+    constexpr int NC = 4 * NWB;
+    constexpr int NT = 64 * NC * 2;
+    constexpr int V = 16;
+    constexpr int ZS = KZ + 4;
+    constexpr int SW = NT - 64;
+    constexpr int PT = (V * KZ / 4 + NT - 1) / NT;
+    constexpr int NG = KQH;
+    RoundMeta<L, V, 8> &mt = *mtp;
+    const int32_t *node = a.node, *node_ray = a.node_ray, *node_cp = reinterpret_cast<const int32_t *>(a.node_cp);
+    (void)node_cp;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int rounds = (int)((v1 - v0 + V - 1) / V);
+    const bool meta = tid >= SW && tid < SW + V;
+    if (tid < 2 * V) {
+        const NodeRef n = node_ref(a, v0 + tid, v1);
+        meta_fill<L, V, 8>(mt, tid / V, tid % V, n.id >= 0, n.id, n.ray, n.cp);
+    }
+    const bool nlane = tid >= SW && lane < 3 * V;  // the node-data loaders
+    const int64_t ndel = lane < V ? 0 : lane < 2 * V ? (int64_t)(node_ray - node) : (int64_t)(node_cp - node);
+    if (rounds > 2) quad_fetch_nodes<V>(ndata + 2 * 3 * V, node + ndel, v0 + 2 * V, v1, nlane, lane);
+    __syncthreads();
+    const float *basis = a.basis;
+    if (rounds > 0) quad_fetch_rows<V, KZ, NT, PT>(raw, basis, mt.ray[0], tid);
+    floatx4 dprev = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int r = 0; r <= rounds; ++r) {
+        const int cur = r & 1;
+        const int64_t r0 = v0 + (int64_t)r * V;
+        if (r < rounds) {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this thread's rows of round r are in raw
+            float *Zc = Z + cur * V * ZS;
+            int t = tid;
+            asm volatile("" : "+v"(t));  // chunk positions recomputed per round (registers)
+#pragma unroll
+            for (int j = 0; j < PT; ++j) {
+                const int c = t + j * NT;
+                if (c < V * (KZ / 4)) {
+                    const int v = c / (KZ / 4), k4 = c % (KZ / 4);
+                    const float4 p = raw[c];
+                    const uchar4 m = reinterpret_cast<const uchar4 *>(km)[k4];
+                    const float *cm = mt.cm[r & 7][v];
+                    reinterpret_cast<float4 *>(Zc + v * ZS)[k4] =
+                        float4{p.x * cm[m.x], p.y * cm[m.y], p.z * cm[m.z], p.w * cm[m.w]};
+                }
+            }
+            if (meta && r + 2 < rounds) {  // from ndata, fetched during round r - 1 (or before the loop)
+                const int v = lane;
+                const int *nd = ndata + ((r + 2) & 3) * 3 * V;
+                meta_fill<L, V, 8>(mt, (r + 2) & 7, v, r0 + 2 * V + v < v1, nd[v], nd[V + v], __int_as_float(nd[2 * V + v]));
+            }
+            if (r + 3 < rounds)  // slot (r + 3) & 3 was last read in round r - 3
+                quad_fetch_nodes<V>(ndata + ((r + 3) & 3) * 3 * V, node + ndel, r0 + 3 * V, v1, nlane, lane);
+            // the reads of raw above have returned (their values are in Z): the rows of round r + 1 may land
+            if (r + 1 < rounds) quad_fetch_rows<V, KZ, NT, PT>(raw, basis, mt.ray[(r + 1) & 7], tid);
+        }
+        // Workgroup barrier over the LDS only: the wavefronts share Z, the node data and the partials, all in LDS,
+        // and nothing in global memory, so the barrier waits for this wavefront's LDS operations and leaves its
+        // loads into raw in flight (a __syncthreads would wait for them too, as LDS writes).
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        // records of round r - 1: the first half adds the second half's partials
+        if (r > 0 && kh == 0) {
+            const int64_t p0 = v0 + (int64_t)(r - 1) * V;
+            const int nvp = (int)min((int64_t)V, v1 - p0);
+            const floatx4 d = dprev + red[(((r - 1) & 1) * NC + wc) * 64 + lane];
+            const int *ids = mt.id[(r - 1) & 7];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 4 * (lane >> 4) + v;
+                if (row < nvp) a.records[((int64_t)ids[row] * 4 + corner) * G + col] = d[v];
+            }
+        }
+        if (r < rounds) {
+            const floatx4 d = sh_products_part<NG, ZS, 2>(Z, cur * V * ZS, b, lane, g0, ng);
+            if (kh == 1) red[(cur * NC + wc) * 64 + lane] = d;
+            else dprev = d;
+        }
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // no load into LDS outlives the workgroup
 }
 
 // Quad form: 8 G/16 wavefronts (4 corners x G/16 lane tiles x 2 halves of the K range), 72 B registers each at
@@ -449,10 +576,10 @@ void sh_records_quad_kernel(ShArgs a) {
     constexpr int NT = 64 * NC * 2;
     constexpr int V = 16;
     constexpr int ZS = KZ + 4;
-    constexpr int SW = NT - 64;
-    constexpr int PT = (V * KZ / 4 + NT - 1) / NT;
     __shared__ __attribute__((aligned(16))) float Z[2][V][ZS];
+    __shared__ float4 raw[V * KZ / 4];  // the round's basis rows as fetched: chunk c = tid + j NT of thread tid
     __shared__ RoundMeta<L, V, 8> mt;  // ids of round r are read after the barrier of round r + 1: eight slots
+    __shared__ int ndata[4][3 * V];    // node id, ray and cos phi (bits) of round r in slot r & 3, as fetched
     __shared__ __attribute__((aligned(4))) unsigned char km[KZ];
     __shared__ floatx4 red[2][NC][64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -462,8 +589,6 @@ void sh_records_quad_kernel(ShArgs a) {
     const int64_t item = blockIdx.x;
     const int cell = corner_cell(a.item_cell[item], corner, a.nc);
     const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
-    const int rounds = (int)((v1 - v0 + V - 1) / V);
-    const bool meta = tid >= SW && tid < SW + V;
     stage_km<L, KZ>(km, NT);
     constexpr int NG = KQH;
     const int g0 = kh * KQH;
@@ -479,49 +604,7 @@ void sh_records_quad_kernel(ShArgs a) {
                 b[4 * i + u] = (i < ng && k < K) ? pl[(int64_t)k * G] : 0.0f;
             }
     }
-    if (tid < 2 * V) {
-        const NodeRef n = node_ref(a, v0 + tid, v1);
-        meta_fill<L, V, 8>(mt, tid / V, tid % V, n.id >= 0, n.id, n.ray, n.cp);
-    }
-    NodeRef nn{-1, -1, 1.0f};
-    if (meta) nn = node_ref(a, v0 + 2 * V + (tid - SW), v1);
-    __syncthreads();
-    floatx4 dprev = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int r = 0; r <= rounds; ++r) {
-        const int cur = r & 1;
-        const int64_t r0 = v0 + (int64_t)r * V;
-        if (r < rounds) {
-            float4 pr[PT];
-            fetch_rows<V, KZ, NT, PT>(pr, a.basis, mt, r & 7, tid);
-            float (*Zc)[ZS] = Z[cur];
-            rows_to_z<V, KZ, NT, PT>(pr, mt, r & 7, km, tid, [&](int v, int k4, float4 z) {
-                reinterpret_cast<float4 *>(&Zc[v][0])[k4] = z;
-            });
-            if (meta && r + 2 < rounds) {
-                const int v = tid - SW;
-                meta_fill<L, V, 8>(mt, (r + 2) & 7, v, nn.id >= 0, nn.id, nn.ray, nn.cp);
-                nn = node_ref(a, r0 + 3 * V + v, v1);
-            }
-        }
-        __syncthreads();
-        // records of round r - 1: the first half adds the second half's partials
-        if (r > 0 && kh == 0) {
-            const int64_t p0 = v0 + (int64_t)(r - 1) * V;
-            const int nvp = (int)min((int64_t)V, v1 - p0);
-            const floatx4 d = dprev + red[(r - 1) & 1][wc][lane];
-            const int *ids = mt.id[(r - 1) & 7];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = 4 * (lane >> 4) + v;
-                if (row < nvp) a.records[((int64_t)ids[row] * 4 + corner) * G + col] = d[v];
-            }
-        }
-        if (r < rounds) {
-            const floatx4 d = sh_products_part<NG, ZS, 2>(&Z[cur][0][0], b, lane, g0, ng);
-            if (kh == 1) red[cur][wc][lane] = d;
-            else dprev = d;
-        }
-    }
+    quad_rounds<G, L>(a, &Z[0][0][0], raw, &ndata[0][0], &mt, km, &red[0][0][0], b, v0, v1, corner, col, kh, wc, g0, ng);
 }
 
 template <int G, int L>
