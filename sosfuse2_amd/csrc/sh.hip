@@ -662,6 +662,7 @@ void sh_records_step_kernel(ShArgs a) {
 // cos(m phi)) by 2^13 on its way into LDS; the records are scaled back. Each k-step of 16 costs three 8-pass
 // MFMAs instead of four 32-pass FP32 MFMAs; the error per record is about 2^-21 of sum_k |Z_k c_k|, the FP32 level.
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 
 __device__ inline half4_t split_hi(float4 z) {
     return half4_t{(_Float16)z.x, (_Float16)z.y, (_Float16)z.z, (_Float16)z.w};
@@ -702,6 +703,36 @@ __device__ inline floatx4 split_products(const _Float16 *hrow, const _Float16 *l
     return am + (a1 + a2) * (1.0f / 2048.0f);
 }
 
+// The same with gfx950's K = 32 FP16 MFMA (v_mfma_f32_16x16x32_f16: twice the k-depth per product at the rate of
+// the K = 16 form): bh[s], bl[s] hold K positions 32 s + 8 (lane / 16) + 0..7 of the lane's column, the A rows are
+// read 16 bytes (8 positions) at a time.
+template <int KS2, int D>
+__device__ inline floatx4 split_products32(const _Float16 *hrow, const _Float16 *lrow, const half8_t *bh,
+                                           const half8_t *bl) {
+    floatx4 am = floatx4{0.f, 0.f, 0.f, 0.f}, a1 = am, a2 = am;
+    half8_t ph[D], pl[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        ph[j] = *reinterpret_cast<const half8_t *>(hrow + 32 * j);
+        pl[j] = *reinterpret_cast<const half8_t *>(lrow + 32 * j);
+        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < KS2; ++i) {
+        const half8_t ah = ph[i % D], al = pl[i % D];
+        if (i + D < KS2) {
+            ph[i % D] = *reinterpret_cast<const half8_t *>(hrow + 32 * (i + D));
+            pl[i % D] = *reinterpret_cast<const half8_t *>(lrow + 32 * (i + D));
+        }
+        am = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[i], am, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[i], a1, 0, 0, 0);
+        a2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[i], a2, 0, 0, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
+        if (i + D < KS2) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+    }
+    return am + (a1 + a2) * (1.0f / 2048.0f);
+}
+
 // Split-FP16 SH records in the flow form of sh_records_kernel: 4 x G/16 wavefronts, wavefront (corner, N-tile)
 // holding the whole K range of its hi / lo B operands (144 VGPRs at L = 32 in the quad layout [cell][KZ/4][G][4],
 // about 240 in all: two wavefronts per SIMD).
@@ -734,16 +765,33 @@ void sh_records_split_kernel(ShArgs a) {
     // the basis tile is stored times ZSCALE = 2^13 (|Z| <= sqrt(2 (2L + 1) / 4 pi) < 3.3, so |Z| 2^13 < 2^15): its
     // entries down to 2^-27 are FP16 normal numbers
     constexpr float ZSCALE = 8192.0f;
-    // B operands from the quad layout: bh[i], bl[i] hold K positions 16 i + 4 (lane / 16) + 0..3 of column col
-    half4_t bh[KS], bl[KS];
+    // B operands from the quad layout [cell][KZ/4][G][4]. K = 32 products (KZ a multiple of 32): bh[s], bl[s] hold
+    // K positions 32 s + 8 (lane / 16) + 0..7 of column col (quads 8 s + 2 (lane / 16) and the next). K = 16
+    // products otherwise (L = 8): b4h[i], b4l[i] hold positions 16 i + 4 (lane / 16) + 0..3.
+    constexpr bool K32 = KZ % 32 == 0;
+    constexpr int KS2 = K32 ? KZ / 32 : 1;
+    constexpr int KS4 = K32 ? 1 : KS;
+    half8_t bh[KS2], bl[KS2];
+    half4_t b4h[KS4], b4l[KS4];
     {
         const half4_t *ph = reinterpret_cast<const half4_t *>(a.plane) + (int64_t)cell * (KZ / 4) * G + col;
         const half4_t *pl = ph + a.plane_lo / 4;
+        if constexpr (K32) {
 #pragma unroll
-        for (int i = 0; i < KS; ++i) {
-            const int q = 4 * i + (lane >> 4);
-            bh[i] = ph[(int64_t)q * G];
-            bl[i] = pl[(int64_t)q * G];
+            for (int i = 0; i < KS2; ++i) {
+                const int q = 8 * i + 2 * (lane >> 4);
+                const half4_t h0 = ph[(int64_t)q * G], h1 = ph[(int64_t)(q + 1) * G];
+                const half4_t l0 = pl[(int64_t)q * G], l1 = pl[(int64_t)(q + 1) * G];
+                bh[i] = half8_t{h0[0], h0[1], h0[2], h0[3], h1[0], h1[1], h1[2], h1[3]};
+                bl[i] = half8_t{l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3]};
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < KS; ++i) {
+                const int q = 4 * i + (lane >> 4);
+                b4h[i] = ph[(int64_t)q * G];
+                b4l[i] = pl[(int64_t)q * G];
+            }
         }
     }
     if (tid < 2 * V) {
@@ -774,8 +822,13 @@ void sh_records_split_kernel(ShArgs a) {
             nn = node_ref(a, r0 + 3 * V + v, v1);
         }
         __syncthreads();
-        const floatx4 d = split_products<KS, 3>(&Zh[cur][lane & 15][4 * (lane >> 4)],
-                                                &Zl[cur][lane & 15][4 * (lane >> 4)], bh, bl);
+        floatx4 d;
+        if constexpr (K32)
+            d = split_products32<KS2, 3>(&Zh[cur][lane & 15][8 * (lane >> 4)], &Zl[cur][lane & 15][8 * (lane >> 4)], bh,
+                                         bl);
+        else
+            d = split_products<KS, 3>(&Zh[cur][lane & 15][4 * (lane >> 4)], &Zl[cur][lane & 15][4 * (lane >> 4)], b4h,
+                                      b4l);
         const int *ids = mt.id[r & 3];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
