@@ -703,39 +703,136 @@ __device__ inline floatx4 split_products(const _Float16 *hrow, const _Float16 *l
     return am + (a1 + a2) * (1.0f / 2048.0f);
 }
 
-// The same with gfx950's K = 32 FP16 MFMA (v_mfma_f32_16x16x32_f16: twice the k-depth per product at the rate of
-// the K = 16 form): bh[s], bl[s] hold K positions 32 s + 8 (lane / 16) + 0..7 of the lane's column, the A rows are
-// read 16 bytes (8 positions) at a time.
-template <int KS2, int D>
-__device__ inline floatx4 split_products32(const _Float16 *hrow, const _Float16 *lrow, const half8_t *bh,
-                                           const half8_t *bl) {
+// Split-FP16 products (K = 32 MFMA) over the steps g0 .. g0 + ng - 1 only (one wavefront's half of the K range),
+// tile at offset tile of Zh / Zl (halves); steps past ng read the half's last step again against a zero B.
+template <int NS, int ZS, int D>
+__device__ inline floatx4 split_products_part(const _Float16 *Zh, const _Float16 *Zl, int tile, const half8_t *bh,
+                                              const half8_t *bl, int lane, int g0, int ng) {
     floatx4 am = floatx4{0.f, 0.f, 0.f, 0.f}, a1 = am, a2 = am;
+    int zrow = tile + (lane & 15) * ZS + 8 * (lane >> 4) + 32 * g0;
+    asm volatile("" : "+v"(zrow));  // one opaque offset: each read one constant offset away from Zh / Zl
+    const int glast = ng - 1;
     half8_t ph[D], pl[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-        ph[j] = *reinterpret_cast<const half8_t *>(hrow + 32 * j);
-        pl[j] = *reinterpret_cast<const half8_t *>(lrow + 32 * j);
+        ph[j] = *reinterpret_cast<const half8_t *>(Zh + (zrow + 32 * min(j, glast)));
+        pl[j] = *reinterpret_cast<const half8_t *>(Zl + (zrow + 32 * min(j, glast)));
         __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
     }
 #pragma unroll
-    for (int i = 0; i < KS2; ++i) {
+    for (int i = 0; i < NS; ++i) {
         const half8_t ah = ph[i % D], al = pl[i % D];
-        if (i + D < KS2) {
-            ph[i % D] = *reinterpret_cast<const half8_t *>(hrow + 32 * (i + D));
-            pl[i % D] = *reinterpret_cast<const half8_t *>(lrow + 32 * (i + D));
+        if (i + D < NS) {
+            ph[i % D] = *reinterpret_cast<const half8_t *>(Zh + (zrow + 32 * min(i + D, glast)));
+            pl[i % D] = *reinterpret_cast<const half8_t *>(Zl + (zrow + 32 * min(i + D, glast)));
         }
         am = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[i], am, 0, 0, 0);
         a1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[i], a1, 0, 0, 0);
         a2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[i], a2, 0, 0, 0);
         __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
-        if (i + D < KS2) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+        if (i + D < NS) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
     }
     return am + (a1 + a2) * (1.0f / 2048.0f);
 }
 
-// Split-FP16 SH records in the flow form of sh_records_kernel: 4 x G/16 wavefronts, wavefront (corner, N-tile)
-// holding the whole K range of its hi / lo B operands (144 VGPRs at L = 32 in the quad layout [cell][KZ/4][G][4],
-// about 240 in all: two wavefronts per SIMD).
+// The rounds of the split-FP16 quad form: the FP32 quad form's rounds (quad_rounds) with the tile stored as split
+// FP16 (hi, lo) times 2^13 and the K = 32 FP16 products.
+template <int G, int L>
+__device__ __attribute__((always_inline)) inline void quad_rounds_split(
+    const ShArgs &a, _Float16 *__restrict__ Zh, _Float16 *__restrict__ Zl, float4 *__restrict__ raw, int *__restrict__ ndata,
+    RoundMeta<L, 16, 8> *__restrict__ mtp, const unsigned char *__restrict__ km, floatx4 *__restrict__ red,
+    const half8_t *bh, const half8_t *bl, float scale, int64_t v0, int64_t v1, int corner, int col, int kh, int wc,
+    int g0, int ng) {
+    constexpr int K = (L + 1) * (L + 2) / 2;
+    constexpr int KZ = (K + 15) / 16 * 16;
+    constexpr int KS2 = KZ / 32;
+    constexpr int NS = (KS2 + 1) / 2;
+    constexpr int NWB = G / 16;
+    constexpr int NC = 4 * NWB;
+    constexpr int NT = 64 * NC * 2;
+    constexpr int V = 16;
+    constexpr int ZS = KZ + 8;  // halves
+    constexpr float ZSCALE = 8192.0f;
+    constexpr int SW = NT - 64;
+    constexpr int PT = (V * KZ / 4 + NT - 1) / NT;
+    RoundMeta<L, V, 8> &mt = *mtp;
+    const int32_t *node = a.node, *node_ray = a.node_ray, *node_cp = reinterpret_cast<const int32_t *>(a.node_cp);
+    (void)node_cp;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int rounds = (int)((v1 - v0 + V - 1) / V);
+    const bool meta = tid >= SW && tid < SW + V;
+    if (tid < 2 * V) {
+        const NodeRef n = node_ref(a, v0 + tid, v1);
+        meta_fill<L, V, 8>(mt, tid / V, tid % V, n.id >= 0, n.id, n.ray, n.cp);
+    }
+    const bool nlane = tid >= SW && lane < 3 * V;  // the node-data loaders
+    const int64_t ndel = lane < V ? 0 : lane < 2 * V ? (int64_t)(node_ray - node) : (int64_t)(node_cp - node);
+    if (rounds > 2) quad_fetch_nodes<V>(ndata + 2 * 3 * V, node + ndel, v0 + 2 * V, v1, nlane, lane);
+    __syncthreads();
+    const float *basis = a.basis;
+    if (rounds > 0) quad_fetch_rows<V, KZ, NT, PT>(raw, basis, mt.ray[0], tid);
+    floatx4 dprev = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int r = 0; r <= rounds; ++r) {
+        const int cur = r & 1;
+        const int64_t r0 = v0 + (int64_t)r * V;
+        if (r < rounds) {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this thread's rows of round r are in raw
+            const int zt = cur * V * ZS;
+            int t = tid;
+            asm volatile("" : "+v"(t));  // chunk positions recomputed per round (registers)
+#pragma unroll
+            for (int j = 0; j < PT; ++j) {
+                const int c = t + j * NT;
+                if (c < V * (KZ / 4)) {
+                    const int v = c / (KZ / 4), k4 = c % (KZ / 4);
+                    const float4 p = raw[c];
+                    const uchar4 m = reinterpret_cast<const uchar4 *>(km)[k4];
+                    const float *cm = mt.cm[r & 7][v];
+                    const float4 z = float4{(p.x * cm[m.x]) * ZSCALE, (p.y * cm[m.y]) * ZSCALE,
+                                            (p.z * cm[m.z]) * ZSCALE, (p.w * cm[m.w]) * ZSCALE};
+                    const half4_t h = split_hi(z);
+                    reinterpret_cast<half4_t *>(Zh + zt + v * ZS)[k4] = h;
+                    reinterpret_cast<half4_t *>(Zl + zt + v * ZS)[k4] = split_lo(z, h);
+                }
+            }
+            if (meta && r + 2 < rounds) {  // from ndata, fetched during round r - 1 (or before the loop)
+                const int v = lane;
+                const int *nd = ndata + ((r + 2) & 3) * 3 * V;
+                meta_fill<L, V, 8>(mt, (r + 2) & 7, v, r0 + 2 * V + v < v1, nd[v], nd[V + v], __int_as_float(nd[2 * V + v]));
+            }
+            if (r + 3 < rounds)  // slot (r + 3) & 3 was last read in round r - 3
+                quad_fetch_nodes<V>(ndata + ((r + 3) & 3) * 3 * V, node + ndel, r0 + 3 * V, v1, nlane, lane);
+            // the reads of raw above have returned (their values are in Z): the rows of round r + 1 may land
+            if (r + 1 < rounds) quad_fetch_rows<V, KZ, NT, PT>(raw, basis, mt.ray[(r + 1) & 7], tid);
+        }
+        // Workgroup barrier over the LDS only: the wavefronts share Z, the node data and the partials, all in LDS,
+        // and nothing in global memory, so the barrier waits for this wavefront's LDS operations and leaves its
+        // loads into raw in flight (a __syncthreads would wait for them too, as LDS writes).
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        // records of round r - 1: the first half adds the second half's partials
+        if (r > 0 && kh == 0) {
+            const int64_t p0 = v0 + (int64_t)(r - 1) * V;
+            const int nvp = (int)min((int64_t)V, v1 - p0);
+            const floatx4 d = dprev + red[(((r - 1) & 1) * NC + wc) * 64 + lane];
+            const int *ids = mt.id[(r - 1) & 7];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 4 * (lane >> 4) + v;
+                if (row < nvp) a.records[((int64_t)ids[row] * 4 + corner) * G + col] = (d[v] * scale) * (1.0f / ZSCALE);
+            }
+        }
+        if (r < rounds) {
+            const floatx4 d = split_products_part<NS, ZS, 2>(Zh, Zl, cur * V * ZS, bh, bl, lane, g0, ng);
+            if (kh == 1) red[(cur * NC + wc) * 64 + lane] = d;
+            else dprev = d;
+        }
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // no load into LDS outlives the workgroup
+}
+
+// Split-FP16 SH records in the flow form of sh_records_kernel, for a K range that is not a multiple of 32 (L = 8;
+// the other degrees take the quad form below): 4 x G/16 wavefronts, wavefront (corner, N-tile) holding the whole K
+// range of its hi / lo B operands in the quad layout [cell][KZ/4][G][4], K = 16 FP16 products.
 template <int G, int L>
 __global__ __launch_bounds__(G * 16) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void sh_records_split_kernel(ShArgs a) {
@@ -765,33 +862,16 @@ void sh_records_split_kernel(ShArgs a) {
     // the basis tile is stored times ZSCALE = 2^13 (|Z| <= sqrt(2 (2L + 1) / 4 pi) < 3.3, so |Z| 2^13 < 2^15): its
     // entries down to 2^-27 are FP16 normal numbers
     constexpr float ZSCALE = 8192.0f;
-    // B operands from the quad layout [cell][KZ/4][G][4]. K = 32 products (KZ a multiple of 32): bh[s], bl[s] hold
-    // K positions 32 s + 8 (lane / 16) + 0..7 of column col (quads 8 s + 2 (lane / 16) and the next). K = 16
-    // products otherwise (L = 8): b4h[i], b4l[i] hold positions 16 i + 4 (lane / 16) + 0..3.
-    constexpr bool K32 = KZ % 32 == 0;
-    constexpr int KS2 = K32 ? KZ / 32 : 1;
-    constexpr int KS4 = K32 ? 1 : KS;
-    half8_t bh[KS2], bl[KS2];
-    half4_t b4h[KS4], b4l[KS4];
+    // B operands from the quad layout: bh[i], bl[i] hold K positions 16 i + 4 (lane / 16) + 0..3 of column col
+    half4_t bh[KS], bl[KS];
     {
         const half4_t *ph = reinterpret_cast<const half4_t *>(a.plane) + (int64_t)cell * (KZ / 4) * G + col;
         const half4_t *pl = ph + a.plane_lo / 4;
-        if constexpr (K32) {
 #pragma unroll
-            for (int i = 0; i < KS2; ++i) {
-                const int q = 8 * i + 2 * (lane >> 4);
-                const half4_t h0 = ph[(int64_t)q * G], h1 = ph[(int64_t)(q + 1) * G];
-                const half4_t l0 = pl[(int64_t)q * G], l1 = pl[(int64_t)(q + 1) * G];
-                bh[i] = half8_t{h0[0], h0[1], h0[2], h0[3], h1[0], h1[1], h1[2], h1[3]};
-                bl[i] = half8_t{l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3]};
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < KS; ++i) {
-                const int q = 4 * i + (lane >> 4);
-                b4h[i] = ph[(int64_t)q * G];
-                b4l[i] = pl[(int64_t)q * G];
-            }
+        for (int i = 0; i < KS; ++i) {
+            const int q = 4 * i + (lane >> 4);
+            bh[i] = ph[(int64_t)q * G];
+            bl[i] = pl[(int64_t)q * G];
         }
     }
     if (tid < 2 * V) {
@@ -822,13 +902,8 @@ void sh_records_split_kernel(ShArgs a) {
             nn = node_ref(a, r0 + 3 * V + v, v1);
         }
         __syncthreads();
-        floatx4 d;
-        if constexpr (K32)
-            d = split_products32<KS2, 3>(&Zh[cur][lane & 15][8 * (lane >> 4)], &Zl[cur][lane & 15][8 * (lane >> 4)], bh,
-                                         bl);
-        else
-            d = split_products<KS, 3>(&Zh[cur][lane & 15][4 * (lane >> 4)], &Zl[cur][lane & 15][4 * (lane >> 4)], b4h,
-                                      b4l);
+        const floatx4 d = split_products<KS, 3>(&Zh[cur][lane & 15][4 * (lane >> 4)],
+                                                &Zl[cur][lane & 15][4 * (lane >> 4)], bh, bl);
         const int *ids = mt.id[r & 3];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -836,6 +911,64 @@ void sh_records_split_kernel(ShArgs a) {
             if (row < nv) a.records[((int64_t)ids[row] * 4 + corner) * G + col] = (d[v] * scale) * (1.0f / ZSCALE);
         }
     }
+}
+
+// Split-FP16 SH records in the quad form (KZ a multiple of 32, L = 16 and 32): 8 G/16 wavefronts (4 corners x G/16
+// lane tiles x 2 halves of the K range), each holding its half of the hi / lo B operands (72 VGPRs at L = 32), the
+// rounds of quad_rounds_split.
+template <int G, int L>
+__global__ __launch_bounds__(G * 32) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void sh_records_split_quad_kernel(ShArgs a) {
+    constexpr int K = (L + 1) * (L + 2) / 2;
+    constexpr int KZ = (K + 15) / 16 * 16;
+    static_assert(KZ % 32 == 0, "K = 32 products need KZ a multiple of 32");
+    constexpr int KS2 = KZ / 32;
+    constexpr int NS = (KS2 + 1) / 2;
+    constexpr int NWB = G / 16;
+    constexpr int NC = 4 * NWB;
+    constexpr int NT = 64 * NC * 2;
+    constexpr int V = 16;
+    constexpr int ZS = KZ + 8;
+    __shared__ __attribute__((aligned(16))) _Float16 Zh[2][V][ZS];
+    __shared__ __attribute__((aligned(16))) _Float16 Zl[2][V][ZS];
+    __shared__ float4 raw[V * KZ / 4];
+    __shared__ RoundMeta<L, V, 8> mt;
+    __shared__ int ndata[4][3 * V];
+    __shared__ __attribute__((aligned(4))) unsigned char km[KZ];
+    __shared__ floatx4 red[2][NC][64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int kh = wave / NC, wc = wave % NC;
+    const int corner = wc / NWB, nt = wc % NWB;
+    const int col = nt * 16 + (lane & 15);
+    const int64_t item = blockIdx.x;
+    const int cell = corner_cell(a.item_cell[item], corner, a.nc);
+    const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
+    stage_km<L, KZ>(km, NT);
+    const float scale = a.scale[(int64_t)cell * G + col];
+    const int g0 = kh * NS;
+    const int ng = min(NS, KS2 - g0);
+    // bh[i], bl[i]: K positions 32 (g0 + i) + 8 (lane / 16) + 0..7 of column col (quads 8 (g0 + i) + 2 (lane / 16)
+    // and the next of the layout [cell][KZ/4][G][4]); zero past the half's steps
+    half8_t bh[NS], bl[NS];
+    {
+        const half4_t *ph = reinterpret_cast<const half4_t *>(a.plane) + (int64_t)cell * (KZ / 4) * G + col;
+        const half4_t *pl = ph + a.plane_lo / 4;
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            half4_t h0{}, h1{}, l0{}, l1{};
+            if (i < ng) {
+                const int q = 8 * (g0 + i) + 2 * (lane >> 4);
+                h0 = ph[(int64_t)q * G];
+                h1 = ph[(int64_t)(q + 1) * G];
+                l0 = pl[(int64_t)q * G];
+                l1 = pl[(int64_t)(q + 1) * G];
+            }
+            bh[i] = half8_t{h0[0], h0[1], h0[2], h0[3], h1[0], h1[1], h1[2], h1[3]};
+            bl[i] = half8_t{l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3]};
+        }
+    }
+    quad_rounds_split<G, L>(a, &Zh[0][0][0], &Zl[0][0][0], raw, &ndata[0][0], &mt, km, &red[0][0][0], bh, bl, scale,
+                            v0, v1, corner, col, kh, wc, g0, ng);
 }
 
 // One lane (W = 1): one thread per node, dot products of the node's basis row (table row times cos(m phi)) against
@@ -892,8 +1025,12 @@ static int sh_launch(const ShArgs &a, bool half, hipStream_t s) {
         if (half) hipLaunchKernelGGL((sh_records_scalar<L, true>), dim3((unsigned)a.items), dim3(256), 0, s, a);
         else hipLaunchKernelGGL((sh_records_scalar<L, false>), dim3((unsigned)a.items), dim3(256), 0, s, a);
     } else {
-        if (half)
-            hipLaunchKernelGGL((sh_records_split_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 16), 0, s, a);
+        if (half) {
+            if constexpr (((L + 1) * (L + 2) / 2 + 15) / 16 * 16 % 32 == 0)
+                hipLaunchKernelGGL((sh_records_split_quad_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 32), 0, s, a);
+            else
+                hipLaunchKernelGGL((sh_records_split_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 16), 0, s, a);
+        }
         else if (a.tile == 1)
             hipLaunchKernelGGL((sh_records_step_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 16), 0, s, a);
         else if (a.tile == 2)
