@@ -101,7 +101,9 @@ typedef struct sos_grid {
     int32_t sh_tile;           /* sos_sh_records FP32 workgroup form: 0 flow (double-buffered basis tiles, one
                                   barrier per round of 16 nodes), 1 stepwise (seeds, basis and products separated by
                                   barriers), 2 quad (the flow's rounds on twice the wavefronts, each holding half of
-                                  the K range); a tuning knob, the records are bitwise the same */
+                                  the K range); a tuning knob, the records are bitwise the same. The split-FP16
+                                  records (mixed precision) take the quad form whatever its value (the flow form at
+                                  L = 8) */
     int32_t pool_max;          /* largest pool count of a cell (goff_dev differences); sos_ground_records holds a
                                   cell's pools in registers and takes at most 640 */
 } sos_grid;
