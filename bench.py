@@ -6,7 +6,7 @@ visit plans, node geometry, basis tables) is resident in HBM before the timed re
 
 Prints one JSON line (the bench contract) with a roofline entry for the dominant kernel (the SH contraction of the
 warm pass, sh_records) and a CPU baseline: the NumPy FP32 oracle timed on a bounded sample of the same work on this
-host. --mixed runs the mixed-precision form (FP16 SH contraction on v_mfma_f32_16x16x16f16).
+host. --mixed runs the mixed-precision form (split-FP16 SH contraction on v_mfma_f32_16x16x32_f16).
 """
 
 import argparse
@@ -309,7 +309,7 @@ def main():
             out['error_vs_fp32'] = dict(err, evaluations_fp32=info_o['evaluations'])
         else:
             out['mixed'] = dict(tables_per_hour=round(3600.0 / t_other, 2), evaluations=info_o['evaluations'],
-                                error_vs_fp32=err, form='split-FP16 SH contraction (v_mfma_f32_16x16x16f16)')
+                                error_vs_fp32=err, form='split-FP16 SH contraction (v_mfma_f32_16x16x32_f16)')
         log(f'mixed vs FP32 table: {err}')
     if 'batched' in variants and args.batched_states > 0:
         out['batched'] = batched_variant(scope, args.batched_states, el, az, tol)
