@@ -517,12 +517,15 @@ def test_autotuner_keeps_results(gpu):
     assert again == best and t2 == {}
 
 
-def test_sh_record_forms_are_bitwise_equal(gpu):
+@pytest.mark.parametrize('W,L', [(20, 32), (3, 16), (5, 8)])
+def test_sh_record_forms_are_bitwise_equal(gpu, W, L):
     """The three FP32 SH-records workgroup forms (flow, stepwise and quad) and three work-item sizes give the same
-    records bit for bit at L = 32 on 32 lanes (same basis arithmetic, same MFMA accumulation order)."""
-    s, scope, _ = setup_case(20, 32)
+    records bit for bit (same basis arithmetic, same MFMA accumulation order): L = 32 on 32 lanes, L = 16 and L = 8
+    (an odd count of K groups: the second K half is one group short) on 16 lanes."""
+    s, scope, _ = setup_case(W, L)
     sv = scope.solver(s, log=None)
-    f = torch.from_numpy(np.ascontiguousarray(padded(physical_field(s, 20), 32))).cuda()
+    G = 32 if W > 16 else 16
+    f = torch.from_numpy(np.ascontiguousarray(padded(physical_field(s, W), G))).cuda()
     ref = None
     for nc in (256, 1024, 4096):
         for st in (0, 1, 2):
