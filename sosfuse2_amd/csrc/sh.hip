@@ -487,7 +487,6 @@ __device__ __attribute__((always_inline)) inline void quad_rounds(
     constexpr int NG = KQH;
     RoundMeta<L, V, 8> &mt = *mtp;
     const int32_t *node = a.node, *node_ray = a.node_ray, *node_cp = reinterpret_cast<const int32_t *>(a.node_cp);
-    (void)node_cp;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int rounds = (int)((v1 - v0 + V - 1) / V);
     const bool meta = tid >= SW && tid < SW + V;
@@ -757,7 +756,6 @@ __device__ __attribute__((always_inline)) inline void quad_rounds_split(
     constexpr int PT = (V * KZ / 4 + NT - 1) / NT;
     RoundMeta<L, V, 8> &mt = *mtp;
     const int32_t *node = a.node, *node_ray = a.node_ray, *node_cp = reinterpret_cast<const int32_t *>(a.node_cp);
-    (void)node_cp;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int rounds = (int)((v1 - v0 + V - 1) / V);
     const bool meta = tid >= SW && tid < SW + V;
