@@ -100,6 +100,7 @@ __global__ __launch_bounds__(64 * (G / 16) * 2) void ground_records_kernel(Groun
                 floatx4 t = floatx4{0.f, 0.f, 0.f, 0.f}, u = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int s = 0; s < PC / 4; ++s) {
+                    if (4 * s >= n) break;  // past the chunk's pools both operands are zero
                     t = __builtin_amdgcn_mfma_f32_16x16x4f32(PT[row][4 * s + kk], bq[ch][s], t, 0, 0, 0);
                     if (a.strato) u = __builtin_amdgcn_mfma_f32_16x16x4f32(PU[row][4 * s + kk], bq[ch][s], u, 0, 0, 0);
                 }
