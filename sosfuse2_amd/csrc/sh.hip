@@ -557,12 +557,13 @@ __device__ __attribute__((always_inline)) inline void quad_rounds(
 }
 
 // Quad form: 8 G/16 wavefronts (4 corners x G/16 lane tiles x 2 halves of the K range), 72 B registers each at
-// L = 32, four wavefronts per SIMD at 32 lanes. Per round every thread fetches its share of the round's basis rows
-// and forms the tile (the other wavefronts of its SIMD, still in their products, cover the fetch), 16 threads
-// prepare the node data of round r + 2, one barrier, then the products. The second half's partial products of
-// round r go through LDS and are added by the first half after the barrier of round r + 1, so the sum is
-// (half 0) + (half 1) as in sh_products. Tiles and partials are double-buffered; the node ids of round r are read
-// after the barrier of round r + 1, so node data takes eight slots.
+// L = 32, four wavefronts per SIMD at 32 lanes. The basis rows of round r + 1 and the node data of round r + 3 are
+// fetched before the barrier of round r with LDS-direct loads (no registers held), so they land during the products
+// of round r. Per round every thread forms its chunks of the tile from the rows it fetched itself (its own wait is
+// the only one needed), 16 threads prepare the node data of round r + 2, one LDS-only barrier, then the products.
+// The second half's partial products of round r go through LDS and are added by the first half after the barrier
+// of round r + 1, so the sum is (half 0) + (half 1) as in sh_products. Tiles and partials are double-buffered; the
+// node ids of round r are read after the barrier of round r + 1, so node data takes eight slots.
 template <int G, int L>
 __global__ __launch_bounds__(G * 32) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void sh_records_quad_kernel(ShArgs a) {
