@@ -190,8 +190,9 @@ def main():
                          'warm-started against cold (adds minutes)')
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f'bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); '
-              'the work shards by state and wavelength chunk with no exchange, see DESIGN.md', file=sys.stderr)
+        print(f'bench.py runs on one GPU: there is no multi-GPU path (asked for --gpus {args.gpus}); '
+              'the work would shard by state and wavelength chunk, one process per GPU, with no exchange',
+              file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
