@@ -173,8 +173,8 @@ int sos_plan_basis(const sos_grid *grid, const sos_bundle *bundle, float *basis_
  * node_dev[v0[i] .. v1[i]) whose base cell is item_cell[i]; node_dev holds node ids sorted by base cell and
  * node_ray_dev / node_cp_dev the ray and the node's cos(phi) in the same order; basis_dev is the bundle's table from
  * sos_plan_basis. half = 1: the plane is the split FP16 plane of sos_make_plane with its per-(cell, lane) scale_dev;
- * Z rows are split the same way and the products run as three v_mfma_f32_16x16x16f16 (hi hi, hi lo, lo hi) with
- * FP32 accumulation (mixed precision, FP32-level accuracy; one lane recombines the plane in FP32). */
+ * Z rows are split the same way and the products run as three v_mfma_f32_16x16x32_f16 (hi hi, hi lo, lo hi; the
+ * K = 16 form v_mfma_f32_16x16x16f16 at L = 8) with FP32 accumulation (mixed precision, FP32-level accuracy). */
 int sos_sh_records(const sos_grid *grid, const void *plane_dev, const float *scale_dev, int32_t half, int64_t items,
                    const int32_t *item_cell_dev, const int64_t *item_v0_dev, const int64_t *item_v1_dev,
                    const int32_t *node_dev, const int32_t *node_ray_dev, const float *node_cp_dev,
@@ -251,6 +251,7 @@ int sos_mlp_adam(int64_t n, float *p_dev, const float *g_dev, float *m_dev, floa
                  float beta1, float beta2, float eps, void *stream);
 
 const char *sos_last_error(void);
+/* Compute units, XCDs (hipDeviceAttributeNumberOfXccs) and the gfx architecture name of the current device. */
 int sos_device_info(int32_t *cus, int32_t *xcds, char *arch, int32_t arch_len);
 
 #ifdef __cplusplus

@@ -23,6 +23,13 @@ from .arith import TRACE_MIN32, count
 from .transport import PAIRS4, PAIRS6, project, transport
 
 
+def monitor_elevations(top_km):
+    """(stop, record) solar elevations of the observer monitor (`src/sosfuse2/foundation/top.py:55-57`): five stop
+    elevations, plus -15 when the top is above 80 km, which also records -18 for diagnostics only."""
+    extended = float(top_km) > 80.0
+    return [2.0, -3.0, -6.0, -9.0, -12.0] + ([-15.0] if extended else []), ([-18.0] if extended else [])
+
+
 def anderson_weights(products):
     """Regularised Gram solve, normalised weights, guard |w| < 8 (source_iteration.py:25-33)."""
     gram = np.array(products, dtype=np.float64, copy=True)
@@ -142,11 +149,12 @@ class OracleSolver:
         v, _ = self.query_raw(h_s, el, az, field, first)
         return ((v[..., 0] + v[..., 1]) + v[..., 2]).astype('f8') * self.ctx.solar[None, :]
 
-    def boxes(self, field):
-        """Observer monitor [n_solar, 7, W] (engine_core.py:228-238)."""
+    def boxes(self, field, record=False):
+        """Observer monitor [n_solar, 7, W] of the stop elevations, or of the record ones (engine_core.py:228-238)."""
         el, az, ww = self.ctx.box_directions()
         out = []
-        for h in self.ctx.monitors:
+        stop, rec = monitor_elevations(self.ctx.geo.domain.top_km)
+        for h in rec if record else stop:
             v = self.query(h, el, az, field, False).reshape(len(self.ctx.directions), 16, self.c['W'])
             out.append(np.einsum('dnw,n->dw', v, ww))
         return np.stack(out)

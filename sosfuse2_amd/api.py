@@ -84,8 +84,13 @@ def create_configured(output, cfg, wavelengths, backend, state, block_size=256, 
         Path(output).mkdir(parents=True, exist_ok=True)
     if scope is None:
         cfg = dict(cfg, sun_scale=sun_scale)
-        kernel = Kernel(backend=backend, L=int(cfg['L']), block_size=block_size, mixed=mixed, records=records)
-        scope = Scope(cfg, state, kernel)
+        if backend != 'hip':
+            raise ValueError(f"E_BACKEND: backend {backend!r}; this engine runs 'hip' only")
+        # the host preparation (process pools) runs before the kernel brings up the GPU runtime
+        scope = Scope(cfg, state, None)
+        scope.kernel = Kernel(backend=backend, L=int(cfg['L']), block_size=block_size, mixed=mixed, records=records)
+    else:
+        atmosphere.check_profile_shapes(state, scope.state)
     wl = np.asarray(wavelengths, float)
     if wl.ndim != 1 or len(wl) < 1:
         raise ValueError('E_WAVELENGTHS: a one-dimensional wavelength list is required')

@@ -229,16 +229,26 @@ def lane_optics(state, wl):
     )
 
 
+PROFILE_KEYS = ('top_km', 'air_height_km', 'aerosol_height_km', 'ozone_center_km', 'ozone_sigma_km', 'no2_center_km',
+                'no2_sigma_km', 'sulfate_center_km', 'sulfate_sigma_km')
+
+
+def check_profile_shapes(state, base):
+    """Raise E_BATCH_PROFILE unless `state` has the profile shapes and top of `base`: the geometry, sun table, ground
+    pools and node geometry of a grid are built from one set of profiles, and each lane's extinction is normalised by
+    its own state's columns, so a state of other shapes would get wrong optical depths."""
+    for key in PROFILE_KEYS:
+        if getattr(state, key) != getattr(base, key):
+            raise ValueError(f'E_BATCH_PROFILE: state {state.name!r} differs from {base.name!r} in {key}')
+
+
 def concat_optics(parts):
     """Pack the lanes of several states into one chunk. All parts must share the profile shapes (aerosol height,
     ozone/NO2/sulfate centres and widths, top), since the transport evaluates one set of profiles per sample."""
     base = parts[0].states[0]
     for p in parts:
-        s = p.states[0]
-        for key in ('top_km', 'air_height_km', 'aerosol_height_km', 'ozone_center_km', 'ozone_sigma_km',
-                    'no2_center_km', 'no2_sigma_km', 'sulfate_center_km', 'sulfate_sigma_km'):
-            if getattr(s, key) != getattr(base, key):
-                raise ValueError(f'E_BATCH_PROFILE: states differ in {key}')
+        for s in p.states:
+            check_profile_shapes(s, base)
     states = []
     lane_state = []
     for p in parts:

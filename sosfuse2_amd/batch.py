@@ -59,16 +59,19 @@ def pack(items, lanes=32, max_states=4):
     return chunks
 
 
-def solve(scope, items, el, az, tolerance=1e-5, max_eval=60, lanes=32, log=None, x0=None):
+def solve(scope, items, el, az, tolerance=1e-5, max_eval=60, lanes=32, log=None, x0=None, graph=True):
     """Solve every item and return per item (table [h_s, el, az, W_i, 3] I/Q/U, info). `x0(s)` may give a warm
-    start for a chunk context (the learned warm start); the iteration still runs to `tolerance`."""
+    start for a chunk context (the learned warm start); the iteration still runs to `tolerance`. graph: each chunk's
+    evaluation runs as one captured HIP graph, replayed every iteration."""
+    for it in items:
+        atmosphere.check_profile_shapes(it.state, scope.state)
     out = [None] * len(items)
     for ch in pack(items, lanes):
         t0 = time.perf_counter()
         optics = atmosphere.concat_optics([atmosphere.lane_optics(items[i].state, items[i].wavelengths)
                                            for i in ch.members])
         s = scope.context(optics)
-        solver = scope.solver(s, log=None)
+        solver = scope.solver(s, log=None, graph=graph)
         start = x0(s, solver) if x0 is not None else None
         x, info = solver.solve(max_eval, tolerance, x0=start)
         hs = sorted({h for i in ch.members for h in items[i].elevations}, reverse=True)

@@ -79,7 +79,11 @@ int sos_device_info(int32_t *cus, int32_t *xcds, char *arch, int32_t arch_len) {
     hipDeviceProp_t p;
     if (hipGetDeviceProperties(&p, dev) != hipSuccess) return set_error("hipGetDeviceProperties failed");
     if (cus) *cus = p.multiProcessorCount;
-    if (xcds) *xcds = 8;
+    if (xcds) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess || n < 1) n = 1;
+        *xcds = n;
+    }
     if (arch && arch_len > 0) {
         strncpy(arch, p.gcnArchName, arch_len - 1);
         arch[arch_len - 1] = 0;

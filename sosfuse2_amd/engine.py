@@ -18,7 +18,7 @@ class Context:
     """One solve chunk: a grid, the optics of its W lanes, and the per-grid geometry."""
 
     def __init__(self, grid, optics, kernel, sun=None, pools=None, observer_km=0.1, profile_state=None,
-                 monitors=(2, -3, -6, -9, -12), directions=None):
+                 monitors=None, directions=None, record_monitors=None):
         self.geo = grid
         self.optics = optics
         self.kernel = kernel
@@ -32,7 +32,10 @@ class Context:
         self.sun = sun if sun is not None else geometry.sun_table(self.state, grid.domain)
         self.pools = pools if pools is not None else geometry.ground_pools(self.state, grid, self.sun)
         self.prepare_seconds = time.perf_counter() - t
-        self.monitors = tuple(monitors)
+        stop, record = grid.domain.monitors()
+        # stop monitors decide convergence; record monitors are evaluated and logged only
+        self.monitors = tuple(stop if monitors is None else monitors)
+        self.record_monitors = tuple(record if record_monitors is None else record_monitors)
         # Observer boxes of the convergence monitor (`solve/engine.py:435-443`): 7 directions x 4 x 4 GL boxes.
         self.directions = directions or [(2, 0), (10, 0), (30, 0), (10, 180), (30, 180), (10, 90), (89, 0)]
         self._constants = {}
@@ -56,7 +59,7 @@ class Context:
     def with_optics(self, optics):
         """The same grid and geometry with other lanes (another state or wavelength chunk)."""
         return Context(self.geo, optics, self.kernel, self.sun, self.pools, self.observer_km, self.state,
-                       self.monitors, self.directions)
+                       self.monitors, self.directions, self.record_monitors)
 
     # -------------------------------------------------------------------------------------------- constants
     def pool_values(self):

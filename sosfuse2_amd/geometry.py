@@ -37,6 +37,13 @@ class Domain:
             (NORMAL_TOP_KM, self.top_km) if self.top_km > NORMAL_TOP_KM else (self.top_km,)
         )
 
+    def monitors(self):
+        """Solar elevations (deg) of the observer monitor (`src/sosfuse2/foundation/top.py:55-57`): the stop set
+        decides convergence and feeds the Anderson residual; a top above 80 km adds -15 to it and records -18 for
+        diagnostics only."""
+        extended = self.top_km > NORMAL_TOP_KM
+        return (2.0, -3.0, -6.0, -9.0, -12.0) + ((-15.0,) if extended else ()), ((-18.0,) if extended else ())
+
 
 def gl(n):
     """Gauss-Legendre nodes and weights on [0, 1] (`shell_segments.py:21-34`)."""
@@ -517,17 +524,35 @@ def _cells_job(args):
     return [with_end_columns(state, cell_pack(grid, c), grid.domain) for c in cells]
 
 
+def gpu_runtime_started():
+    """Whether this process has brought up the GPU runtime (the HIP library of this package, or torch's CUDA state).
+    A process pool started after that forks a process holding the GPU runtime and replaces it with a new interpreter,
+    which the GPU host may refuse, leaving the pool waiting for workers that never start."""
+    import sys
+
+    hip_mod = sys.modules.get(__package__ + '.hip')
+    if hip_mod is not None and hip_mod.started():
+        return True
+    torch = sys.modules.get('torch')
+    return bool(torch is not None and torch.cuda.is_initialized())
+
+
 def _parallel(kind, state, grid, sun, workers, qscale=1):
-    """Run a per-cell preparation over a spawn process pool (numpy only in the workers), in cell order."""
+    """Run a per-cell preparation over a spawn process pool (numpy only in the workers), in cell order; a thread
+    pool once the process has started the GPU runtime (gpu_runtime_started)."""
     cells = list(range(grid.ns))
     if workers <= 1 or grid.ns < 64:
         return [_cells_job((kind, state, grid, sun, cells, qscale))]
-    from concurrent.futures import ProcessPoolExecutor
+    from concurrent.futures import ProcessPoolExecutor, ThreadPoolExecutor
     import multiprocessing as mp
 
     chunk = max(1, (grid.ns + 4 * workers - 1) // (4 * workers))
     jobs = [(kind, state, grid, sun, cells[i : i + chunk], qscale) for i in range(0, grid.ns, chunk)]
-    with ProcessPoolExecutor(workers, mp_context=mp.get_context('spawn')) as pool:
+    if gpu_runtime_started():
+        pool = ThreadPoolExecutor(workers)
+    else:
+        pool = ProcessPoolExecutor(workers, mp_context=mp.get_context('spawn'))
+    with pool:
         return list(pool.map(_cells_job, jobs))
 
 

@@ -95,8 +95,22 @@ def lib():
     return _lib
 
 
+_started = False
+
+
+def mark_started():
+    """Record that this process has started the GPU runtime (Kernel creation, any library call)."""
+    global _started
+    _started = True
+
+
+def started():
+    return _started
+
+
 def call(name, *args):
     """Call an entry point; a nonzero status raises with the library's message."""
+    mark_started()
     rc = getattr(lib(), name)(*args)
     if rc != 0:
         raise RuntimeError(f'E_HIP_{name.upper()}: status {rc}: {lib().sos_last_error().decode()}')

@@ -70,6 +70,18 @@ class DeviceBundle:
             self.edge_off, self.edges, self.lower, self.end_j, self.end_f, self.end_col, self.hits))
 
 
+INDEX_LIMIT = 2**31 - 1  # node and (node, corner) visit ids are int32 on the device
+
+
+def check_plan_size(nodes):
+    """Raise E_PLAN_SIZE when the 4 x nodes (node, corner) visit ids of a plan would not fit the device's int32 ids
+    (they would wrap to negative values and index the records out of bounds)."""
+    if 4 * int(nodes) > INDEX_LIMIT:
+        raise OverflowError(f'E_PLAN_SIZE: {nodes} quadrature nodes give {4 * int(nodes)} (node, corner) visits, '
+                            f'more than the {INDEX_LIMIT} int32 visit ids allow; use a coarser quadrature')
+    return int(nodes)
+
+
 class Plan:
     """Visit plan of a bundle (fixed by its geometry): every quadrature node's cell, mu, cos(phi) and direction; the
     nodes sorted by their base cell with work items of at most `node_chunk` nodes of one base cell (SH records,
@@ -85,7 +97,7 @@ class Plan:
         self.node_off = torch.zeros(db.rays + 1, dtype=torch.int64, device=dev)
         self.node_off[1:] = torch.cumsum(counts[: db.rays], 0)
         db.struct.node_off_dev = hip.ptr(self.node_off)
-        self.nodes = N = int(self.node_off[-1].item())
+        self.nodes = N = check_plan_size(self.node_off[-1].item())
         self.node_cell = torch.empty(max(N, 1), dtype=torch.int32, device=dev)
         self.node_mu = torch.empty(max(N, 1), dtype=torch.float32, device=dev)
         self.node_cp = torch.empty(max(N, 1), dtype=torch.float32, device=dev)
@@ -262,6 +274,7 @@ class Kernel:
         layout.count(L)
         if L == 24:
             raise ValueError('E_SH_L: the HIP kernels are built for L = 8, 16 and 32')
+        hip.mark_started()
         if not torch.cuda.is_available():
             raise RuntimeError('E_HIP_DEVICE: no GPU visible to torch')
         if mixed and not records:

@@ -86,7 +86,7 @@ class Solver:
         self.G = lane_count(s.W)
         self.timings = dict(apply_seconds=[], setup_seconds=0.0)
         el, az, ww = s.box_directions()
-        self._box = observer_bundle(s, s.monitors, el, az)
+        self._box = observer_bundle(s, tuple(s.monitors) + tuple(getattr(s, 'record_monitors', ())), el, az)
         self._ww = ww
 
     @property
@@ -148,14 +148,14 @@ class Solver:
         return out
 
     def boxes(self, field):
-        """Observer monitor [n_solar, 7, G] (`engine_core.py:228-238`)."""
+        """Observer monitor [stop + record solar elevations, 7, G] (`engine_core.py:228-238`)."""
         host, db = self._box
         rad = self.kernel.radiance(db, self.s, field, False, None, 4)
         I = ((rad[..., 0] + rad[..., 1]) + rad[..., 2]).double().cpu().numpy()
         solar = np.r_[self.s.solar, np.full(self.G - self.s.W, self.s.solar[-1])] if self.G > self.s.W else self.s.solar
         I = I * solar[None, :]
         nd = len(self.s.directions)
-        return np.einsum('hdnw,n->hdw', I.reshape(len(self.s.monitors), nd, 16, self.G), self._ww)
+        return np.einsum('hdnw,n->hdw', I.reshape(-1, nd, 16, self.G), self._ww)
 
     def metrics(self, g, x, done):
         out = torch.empty((3, self.G), dtype=torch.float32, device=self.device)
@@ -198,12 +198,20 @@ class Solver:
                 mask = torch.from_numpy(done).to(self.device)
                 g = torch.where(mask[None, None, :], x, g)
             new = self.boxes(g)
+            # the stop monitors decide convergence and enter the residual; record monitors are logged only
+            # (`source_iteration.py:369-391`)
+            nstop = len(s.monitors)
+            rec_old, rec_new = old[nstop:], new[nstop:]
+            old, new = old[:nstop], new[:nstop]
             rr = np.max(abs(new - old) / np.maximum(abs(new), 1e-30), axis=(0, 1))
             if iteration >= 2:
                 done |= np.maximum(rr, ss) <= tolerance
             history.append(dict(evaluation=iteration, observer_relative=rr[:W].tolist(),
                                 source_relative=ss[:W].tolist(), converged=done[:W].tolist(),
                                 seconds=time.perf_counter() - t0))
+            if len(rec_new):
+                history[-1]['record_observer_relative'] = np.max(
+                    abs(rec_new - rec_old) / np.maximum(abs(rec_new), 1e-30), axis=1)[:, :W].tolist()
             if self.log:
                 self.log(f'SOSFUSE EVAL {iteration} obs={max(rr[:W]):.6g} source={max(ss[:W]):.6g} '
                          f'done={done[:W].sum()}/{W}')

@@ -66,7 +66,7 @@ def test_training_lowers_the_loss(gpu):
     assert hist[-1] < hist[0] / 10
 
 
-def test_warm_start_saves_evaluations(gpu):
+def test_warm_start_is_closer_and_reaches_the_same_tables(gpu):
     """Train on solves of seeded states of the batched distribution (small grid, three wavelengths), then solve
     held-out states cold and warm-started: both reach the same tables (both iterate to 1e-5). What the network gains
     is measured two ways and printed: its start is closer to the converged source than the first-light start (the
