@@ -11,7 +11,10 @@ Follows:
 - `query` / `boxes` = `src/sosfuse2/solve/engine_core.py:208-238`; `make_table` = `src/sosfuse2/outputs/radiance_table.py:167-219`.
 
 Inputs (grid, optics, packs, constants) come from the product's host preparation, so the oracle checks the transport
-and the iteration, not the input geometry.
+and the iteration, not the input geometry. The composition factors are restated here from the optical constants
+(`composition_factors`), not taken from the product; tests/test_warm_pins.py pins them by the Funk-Hecke theorem
+(composition then reconstruction of P_l(mu) radiance gives g^l P_l) and the Rayleigh rows by the Legendre moments of
+the Rayleigh phase function; `monitor_elevations` restates the monitor rule of the reference's domain.
 """
 
 import math
@@ -65,6 +68,45 @@ def stokes(comp, view, F=np.float64):
     return np.stack([I, quad(ev, ev) - quad(eh, eh), 2 * quad(ev, eh)], -1)
 
 
+def composition_factors(c):
+    """Per-row factors [rows, W] turning incident moments [4 + K] into source rows (source_iteration.py:176-208):
+    the Rayleigh tensor by F(3 / (8 pi)) ext_R; rows 4-9 (the first six SH moments, degrees 0, 1, 1, 2, 2, 2) by the
+    Rayleigh eigenvalues rf = 1 (l = 0), 0 (l = 1), 0.1 (l = 2) (engine_core.py:93-94) times ext_R; the aerosol plane
+    by g^l asca (engine_core.py:95); a sulfate plane by g_s^l sca_s. Each factor is the product of FP32-rounded
+    parts taken in FP64, as the reference multiplies FP64 moments by FP32 arrays."""
+    f4 = lambda a: np.asarray(a, 'f8').astype('f4').astype('f8')
+    L = c['L']
+    K = count(L)
+    deg = np.array([l for l in range(L + 1) for m in range(l + 1)], 'f8')
+    rf = (deg[:6] == 0) + 0.1 * (deg[:6] == 2)
+    ext_r = f4(c['ext'][0])
+    rows = 10 + K * (2 if c['strato'] else 1)
+    out = np.zeros((rows, len(ext_r)))
+    out[:4] = f4(np.float32(3 / (8 * math.pi)) * np.asarray(c['ext'][0], 'f4'))[None, :]
+    out[4:10] = f4(rf)[:, None] * ext_r[None, :]
+    out[10 : 10 + K] = f4(np.asarray(c['g'], 'f8')[None, :] ** deg[:, None]) * f4(c['asca'])[None, :]
+    if c['strato']:
+        out[10 + K :] = f4(np.asarray(c['strato_g'], 'f8')[None, :] ** deg[:, None]) * f4(c['strato_sca'])[None, :]
+    return out
+
+
+def compose(moment, c, F):
+    """Source field [cells, rows, W] from incident moments [cells, 4 + K, W]; FP32 flushes |source| < FLT_MIN to 0
+    (source_iteration.py:209-212)."""
+    fac = composition_factors(c)
+    K = count(c['L'])
+    moment = np.asarray(moment, 'f8')
+    out = np.empty((moment.shape[0], fac.shape[0], moment.shape[2]), F)
+    out[:, :4] = (moment[:, :4] * fac[None, :4]).astype(F)
+    out[:, 4:10] = (moment[:, 4:10] * fac[None, 4:10]).astype(F)
+    out[:, 10 : 10 + K] = (moment[:, 4:] * fac[None, 10 : 10 + K]).astype(F)
+    if c['strato']:
+        out[:, 10 + K :] = (moment[:, 4:] * fac[None, 10 + K :]).astype(F)
+    if F is np.float32:
+        out[np.abs(out) < TRACE_MIN32] = 0
+    return out
+
+
 class OracleSolver:
     """CPU solve of one chunk. dtype float32 = the FP32-FTZ restatement; float64 = the FP64 twin."""
 
@@ -115,17 +157,7 @@ class OracleSolver:
 
     def compose(self, moment):
         """Source rows from incident moments (source_iteration.py:176-212)."""
-        eig = self.ctx.eigen().astype(self.F).astype('f8')
-        K = count(self.c['L'])
-        out = np.empty((moment.shape[0], self.ctx.rows, self.c['W']), self.F)
-        out[:, :4] = (moment[:, :4] * eig[None, :4]).astype(self.F)
-        out[:, 4:10] = (moment[:, 4:10] * eig[None, 4:10]).astype(self.F)
-        out[:, 10 : 10 + K] = (moment[:, 4:] * eig[None, 10 : 10 + K]).astype(self.F)
-        if self.c['strato']:
-            out[:, 10 + K :] = (moment[:, 4:] * eig[None, 10 + K :]).astype(self.F)
-        if self.F is np.float32:
-            out[np.abs(out) < TRACE_MIN32] = 0
-        return out
+        return compose(moment, self.c, self.F)
 
     def apply(self, field):
         self.initialize()

@@ -21,14 +21,15 @@ Deliberate differences from the reference, each shared with the HIP kernels (so 
 - phase 2 may return six tensor components (xx, yy, zz, xz, xy, yz) so that Stokes Q and U exist; the reference
   forms the first four only (`numpy_reference.py:14`).
 
-What pins each part (tests/test_oracle_pins.py): `Simpson` with `sample`'s path coordinates and weights by fourth-order
-convergence to an FP64 integral and the attenuation closure; `sun_columns` by FP64 path integrals toward the sun;
-`hg`, the log-space blend, the SH basis and cos(m phi) by closed forms; `project` by the projection of isotropic light.
-The first-light assembly of `_transport` (sampling, attenuation, sun columns, the Rayleigh tensor with
-depolarisation) is pinned by the textbook single-scattering integral of a pure Rayleigh atmosphere. The warm
-assembly (the corner blend of the source tensor and SH values) is parity unpinned against reference outputs: the
-reference's own golden tables need its atmosphere and geometry inputs, which only an import of the reference could
-produce; it is pinned through its parts and by the FP64 twin.
+What pins each part (tests/test_oracle_pins.py, tests/test_warm_pins.py): `Simpson` with `sample`'s path coordinates
+and weights by fourth-order convergence to an FP64 integral and the attenuation closure; `sun_columns` by FP64 path
+integrals toward the sun; `hg`, the log-space blend, the SH basis and cos(m phi) by closed forms; `project` by the
+projection of isotropic light. The first-light assembly of `_transport` is pinned by the textbook single-scattering
+integral of a pure Rayleigh atmosphere, and with aerosol, sulfate and a Lambertian ground by the FP64 single-
+scattering source at cell centres (direct sun, and the sunlit ground integrated over every direction that sees it).
+The warm assembly is pinned by FP64 ray integrals of closed-form sources: an isotropic source (tensor and SH (0, 0)
+parts), a zz-only cell tensor (the rotation into the ray frame) and an (l = 1, m = 1) plane coefficient (cos(m phi) at
+the node); the boundary term by E exp(-tau_end) on ground-hitting rays.
 """
 
 import math
