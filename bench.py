@@ -32,8 +32,8 @@ def log(*a):
 
 
 def measured_traffic(kernel, config, mixed):
-    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 counter summary of this workload
-    (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a copy of known size); None when absent."""
+    """HBM bytes per launch of a kernel, from the committed rocprofv3 counter summary of this workload (FETCH_SIZE +
+    WRITE_SIZE, scaled by the same counters on a copy of known size); None when absent."""
     name = f'{kernel}_{config}_{"fp16" if mixed else "fp32"}_traffic.txt'
     path = ROOT / 'profiles' / name
     if not path.exists():
@@ -42,6 +42,30 @@ def measured_traffic(kernel, config, mixed):
         if line.startswith('bytes_per_launch:'):
             return float(line.split(':', 1)[1])
     return None
+
+
+def angular_pairs(plan):
+    """P of the transport's SH work: the (ray, corner cell) refills of the reference's per-ray SH cache
+    (numpy_reference.py:579-600, counted as `angular_pairs` at :597). The four corners of a node are its base cell
+    plus fixed offsets, so a ray refills all four exactly when its base cell differs from its previous node's:
+    P = 4 x (runs of equal base cell along the rays)."""
+    import torch
+
+    n = plan.nodes
+    cell = plan.node_cell[:n].long()
+    first = plan.node_off[:-1][plan.node_off[:-1] < plan.node_off[1:]]
+    new = torch.ones(n, dtype=torch.bool, device=cell.device)
+    new[1:] = cell[1:] != cell[:-1]
+    new[first] = True
+    return 4 * int(new.sum().item())
+
+
+def consumer_bytes(plan, rays, cells, G):
+    """Bytes the warm consumer (transport_geom_kernel) must move per launch given its inputs: the SH records
+    [4 nodes][G] and the node geometry [nodes][32] (FP32), the tensor rows of the source [cells][4][G], the per-ray
+    constants (start, direction, edges, end columns: about 120 bytes) and the radiance written [rays][G][4]."""
+    n = plan.nodes
+    return 4.0 * (4 * n * G + 32 * n + 4 * cells * G + 4 * rays * G) + 120.0 * rays
 
 
 def calibration_copy():
@@ -57,40 +81,57 @@ def calibration_copy():
     del x, y
 
 
-def batched_variant(scope, n, el, az, tol, seed=2026):
-    """Tables/hour of the batched form: n states of the batched distribution (uneven wavelength and elevation
-    counts) packed into 32-lane chunks, solved and queried; the plain form's rate of the same states solved one by
-    one is estimated from its lanes (each state alone is one chunk)."""
+def normalised_tables(items):
+    """Table equivalents of a set of states: lanes x solar elevations over the full table's 30 x 25."""
+    return sum(len(it.wavelengths) * len(it.elevations) for it in items) / (30 * 25)
+
+
+def batched_run(scope, items, el, az, tol, mixed=False, x0=None, graph=True):
+    """Solve and query every state in packed chunks (one captured HIP graph per chunk's evaluation)."""
     import torch
 
     from sosfuse2_amd import batch
 
+    k = scope.kernel
+    k.mixed, was = mixed, k.mixed
+    try:
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        res = batch.solve(scope, items, el, az, tolerance=tol, max_eval=60, lanes=32, log=log, x0=x0, graph=graph)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t
+    finally:
+        k.mixed = was
+    n = len(items)
+    return dict(states=n, seconds=round(elapsed, 2), tables_per_hour=round(3600.0 * n / elapsed, 2),
+                normalised_tables_per_hour=round(3600.0 * normalised_tables(items) / elapsed, 2),
+                evaluations=[r[1]['evaluations'] for r in res]), res
+
+
+def batched_variant(scope, n, el, az, tol, seed=2026):
+    """The batched form: n states of the batched distribution (uneven wavelength and elevation counts) packed into
+    32-lane chunks, each chunk's evaluation replayed as a captured HIP graph, FP32. The normalised rate counts table
+    equivalents (lanes x elevations over 30 x 25), comparable with the plain table rate."""
+    from sosfuse2_amd import batch
+
     items = batch.items_from_seed(n, seed, top_km=120.0)
-    chunks = batch.pack(items, 32)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    res = batch.solve(scope, items, el, az, tolerance=tol, max_eval=60, lanes=32, log=log)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t
-    return dict(states=n, seed=seed, chunks=len(chunks), lanes=sum(len(it.wavelengths) for it in items),
-                elevations=sum(len(it.elevations) for it in items), seconds=round(elapsed, 2),
-                tables_per_hour=round(3600.0 * n / elapsed, 2),
-                evaluations=[r[1]['evaluations'] for r in res])
+    out, _ = batched_run(scope, items, el, az, tol)
+    return dict(out, seed=seed, chunks=len(batch.pack(items, 32)), lanes=sum(len(it.wavelengths) for it in items),
+                elevations=sum(len(it.elevations) for it in items),
+                table_equivalents=round(normalised_tables(items), 3)), items
 
 
-def learned_variant(scope, ntrain, ntest, el, az, tol):
-    """The learned warm start in the batched form: train the network on the solves of ntrain states, then solve
-    ntest held-out states cold and warm-started (both to the same tolerance) and compare evaluations and time."""
+def train_network(scope, ntrain, tol, seed=7, steps=1500):
+    """Train the warm-start network on the converged sources of ntrain states of the batched distribution."""
     import numpy as np
     import torch
 
     from sosfuse2_amd import atmosphere, batch, learned
 
     t0 = time.perf_counter()
-    train_items = batch.items_from_seed(ntrain, 7, top_km=120.0)
+    train_items = batch.items_from_seed(ntrain, seed, top_km=120.0)
     rng = np.random.default_rng(0)
     Xs, Ts = [], []
-
     for ch in batch.pack(train_items, 32):
         optics = atmosphere.concat_optics([atmosphere.lane_optics(train_items[i].state, train_items[i].wavelengths)
                                            for i in ch.members])
@@ -100,77 +141,169 @@ def learned_variant(scope, ntrain, ntest, el, az, tol):
         X, T = learned.samples(s, sv, x, cells_per_lane=2048, rng=rng)
         Xs.append(X)
         Ts.append(T)
-        log(f'LEARN data chunk {ch.members}: {info["evaluations"]} evaluations')
     X, T = torch.cat(Xs), torch.cat(Ts)
     gen = time.perf_counter() - t0
     net = learned.Network(learned.sizes_for(s.rows), seed=0)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    hist = learned.train(net, X, T, 1500, batch=8192, lr=1e-3, log=log)
+    hist = learned.train(net, X, T, steps, batch=8192, lr=1e-3, log=None)
     torch.cuda.synchronize()
-    train_s = time.perf_counter() - t1
-    del X, T, Xs, Ts
-    test_items = batch.items_from_seed(ntest, 1234, top_km=120.0)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    cold = batch.solve(scope, test_items, el, az, tolerance=tol, max_eval=60, lanes=32)
-    torch.cuda.synchronize()
-    t_cold = time.perf_counter() - t
-    out = dict(train_states=ntrain, test_states=ntest, data_seconds=round(gen, 1), train_seconds=round(train_s, 1),
-               loss_first=hist[0], loss_last=hist[-1],
-               cold=dict(seconds=round(t_cold, 2), tables_per_hour=round(3600.0 * ntest / t_cold, 2),
-                         evaluations=[r[1]['evaluations'] for r in cold]))
-    # the warm start over all source rows, and over the tensor and isotropic rows only
-    for key, rows in (('warm', 'all'), ('warm_low_orders', 'low')):
-        t = time.perf_counter()
-        warm = batch.solve(scope, test_items, el, az, tolerance=tol, max_eval=60, lanes=32,
-                           x0=lambda s, sv: learned.warm_start(net, s, sv, rows=rows))
-        torch.cuda.synchronize()
-        t_warm = time.perf_counter() - t
-        worst = max(float(np.max(np.abs(w[0][..., 0] - c[0][..., 0]) / np.abs(c[0][..., 0])))
-                    for w, c in zip(warm, cold))
-        out[key] = dict(seconds=round(t_warm, 2), tables_per_hour=round(3600.0 * ntest / t_warm, 2),
-                        evaluations=[r[1]['evaluations'] for r in warm], max_I_difference_vs_cold=worst)
-    return out
+    info = dict(train_states=ntrain, samples=int(X.shape[0]), data_seconds=round(gen, 1),
+                train_seconds=round(time.perf_counter() - t1, 1), loss_first=hist[0], loss_last=hist[-1])
+    return net, info
 
 
-def cpu_baseline(s, scope, evaluations, seconds_budget=20.0):
-    """Time the FP32 oracle (NumPy, one core) on a bounded sample of cells of the same workload and extrapolate to
-    one table (evaluations x warm+ground transport and projection, plus first light)."""
+def learned_variant(scope, items, el, az, tol, ntrain, cold_mixed):
+    """The learned + mixed-precision + batched form on the batched states: the network (trained on other states)
+    seeds each chunk, the SH contraction runs in split FP16, every chunk replays a captured graph; the iteration
+    still runs to the same tolerance. Compared with the same states cold in mixed precision (the start's gain) and
+    against the plain table rate."""
+    import numpy as np
+
+    from sosfuse2_amd import learned
+
+    net, info = train_network(scope, ntrain, tol)
+    out, res = batched_run(scope, items, el, az, tol, mixed=True,
+                           x0=lambda s, sv: learned.warm_start(net, s, sv, rows='low'))
+    worst = max(float(np.max(np.abs(w[0][..., 0] - c[0][..., 0]) / np.abs(c[0][..., 0])))
+                for w, c in zip(res, cold_mixed))
+    return dict(out, training=info, max_I_difference_vs_cold=worst,
+                evaluations_total=int(sum(out['evaluations'])),
+                evaluations_total_cold=int(sum(r[1]['evaluations'] for r in cold_mixed)))
+
+
+_CPU = {}
+
+
+def _cpu_cell(job):
+    """Warm (with a boundary) and first-light transport + projection of every `step`-th ray of one config1 cell by
+    the FP32 oracle, in a forked worker that inherited the host context; returns (rays, warm s, first-light s)."""
     import numpy as np
 
     from oracle.transport import PAIRS4, project, transport
-    from sosfuse2_amd import geometry
 
-    os.environ.setdefault('OMP_NUM_THREADS', '1')
-    c = s.constants(s.geo.path_n)
-    rng = np.random.default_rng(0)
-    field = (np.abs(rng.normal(size=(s.geo.ns, s.rows, s.W))) * 1e-4).astype('f4')
-    cells = list(range(0, s.geo.ns, max(1, s.geo.ns // 24)))
-    total_rays = sum(len(p['d']) for p in scope.packs)
-    done_rays = 0
-    warm_t = first_t = 0.0
-    t_start = time.perf_counter()
-    for cell in cells:
-        g = scope.packs[cell]
+    cell, step = job
+    c, packs, field, nc = _CPU['c1'], _CPU['packs1'], _CPU['field1'], _CPU['nc1']
+    g = packs[cell]
+    n = len(g['d'])
+    g = {k: (v[::step] if isinstance(v, np.ndarray) and v.shape[:1] == (n,) else v) for k, v in g.items()}
+    t = time.perf_counter()
+    v = transport(g, c, field, False, np.zeros((nc, c['W'])), PAIRS4)
+    project(g, v, c)
+    warm = time.perf_counter() - t
+    t = time.perf_counter()
+    v = transport(g, c, field, True, None, PAIRS4)
+    project(g, v, c)
+    return len(g['d']), warm, time.perf_counter() - t
+
+
+def _ftz_active(_):
+    import numpy as np
+
+    from oracle.arith import flush_to_zero
+
+    tiny = np.float32(np.finfo(np.float32).tiny)
+    with flush_to_zero():
+        return bool(tiny * np.float32(0.5) == 0)
+
+
+class CpuBaseline:
+    """The FP32 NumPy oracle on this host's cores: a process pool forked from this process before it starts the GPU
+    runtime (the workers inherit the host contexts and never touch the GPU), one worker per core of the affinity
+    mask (at most 16). Two measurements: config1 (the bench workload) on a bounded sample of cells, extrapolated to
+    a table, and config0 end to end (solve to 1e-5 at Anderson depth 5 and its 2 x 4 x 6 table)."""
+
+    def __init__(self, scope1, ctx1, cfg0, state0):
+        import multiprocessing as mp
+        from types import SimpleNamespace
+
+        import numpy as np
+
+        from oracle import solve as osolve
+        from sosfuse2_amd import api, atmosphere
+
+        self.cores = max(1, min(16, len(os.sched_getaffinity(0))))
+        rng = np.random.default_rng(0)
+        ns = ctx1.geo.ns
+        _CPU.update(c1=ctx1.constants(ctx1.geo.path_n), packs1=scope1.packs, nc1=ctx1.geo.nc,
+                    field1=(np.abs(rng.normal(size=(ns, ctx1.rows, ctx1.W))) * 1e-4).astype('f4'))
+        self.scope1, self.ctx1 = scope1, ctx1
+        wl0, self.hs0, self.el0, self.az0 = api.axes(cfg0)
+        self.cfg0 = cfg0
+        scope0 = api.Scope(dict(cfg0, sun_scale=1, workers=1), state0, SimpleNamespace(L=int(cfg0['L'])))
+        self.ctx0 = scope0.context(atmosphere.lane_optics(state0, wl0))
+        self.packs0 = scope0.packs
+        osolve._worker_init(self.ctx0.constants(self.ctx0.geo.path_n), self.ctx0.constants(self.ctx0.geo.query_n),
+                            self.packs0)
+        self.pool = mp.get_context('fork').Pool(self.cores)
+        self.pool.map(_ftz_active, range(self.cores))  # the workers exist before the GPU runtime starts
+
+    def run(self, evaluations, per_core=1, step=1):
+        import numpy as np
+
+        from oracle.solve import OracleSolver
+
+        ctx1 = self.ctx1
+        pick = np.unique(np.linspace(0, ctx1.geo.ns - 1, per_core * self.cores).astype(int)).tolist()
+        total_rays = sum(len(p['d']) for p in self.scope1.packs)
         t = time.perf_counter()
-        v = transport(g, c, field, False, np.zeros((s.geo.nc, s.W)), PAIRS4)
-        project(g, v, c)
-        warm_t += time.perf_counter() - t
+        parts = self.pool.map(_cpu_cell, [(c, step) for c in pick], chunksize=1)
+        wall = time.perf_counter() - t
+        rays = sum(p[0] for p in parts)
+        warm, first = sum(p[1] for p in parts), sum(p[2] for p in parts)
+        # one table = first light + evaluations x (warm pass + ground pass, the ground pass being the first row of
+        # cells); CPU seconds scale with rays, spread over the pool's cores
+        table_cpu = total_rays / rays * (first + evaluations * warm * (1 + 1.0 / ctx1.geo.nh))
+        table_s = table_cpu / self.cores
+        ftz = all(self.pool.map(_ftz_active, range(self.cores)))
         t = time.perf_counter()
-        v = transport(g, c, field, True, None, PAIRS4)
-        project(g, v, c)
-        first_t += time.perf_counter() - t
-        done_rays += len(g['d'])
-        if time.perf_counter() - t_start > seconds_budget:
-            break
-    scale = total_rays / done_rays
-    table_s = scale * (first_t + evaluations * warm_t * (1 + 1.0 / s.geo.nh))
-    return dict(value=3600.0 / table_s, unit='tables/hour', cores=1, kind='port',
-                sample=f'FP32 NumPy oracle (flush-to-zero), warm and first-light transport + projection of '
-                       f'{done_rays} rays in {len(cells[:len(cells)])} cells sampled across the grid, all {s.W} '
-                       f'wavelengths, extrapolated to {total_rays} rays x {evaluations} evaluations per table '
-                       f'({table_s:.0f} s per table)')
+        o = OracleSolver(self.ctx0, np.float32, depth=int(self.cfg0['anderson_depth']), packs=self.packs0,
+                         pool=self.pool, workers=self.cores)
+        x, info = o.solve(60, float(self.cfg0['anderson_tol']))
+        table0 = o.table(x, self.hs0, self.el0, self.az0)
+        t0 = time.perf_counter() - t
+        self.pool.close()
+        threads = os.environ.get('OMP_NUM_THREADS', 'unset')
+        return dict(
+            value=3600.0 / table_s, unit='tables/hour', cores=self.cores, kind='port',
+            sample=f'FP32 NumPy oracle (flush-to-zero {"active" if ftz else "INACTIVE"}) in {self.cores} worker '
+                   f'processes (one per core of the affinity mask, OMP_NUM_THREADS={threads}): warm and first-light '
+                   f'transport + projection of {rays} rays ({"all" if step == 1 else f"every {step}th"} ray of '
+                   f'{len(pick)} cells spread over the config1 grid, one cell per worker), all {ctx1.W} wavelengths, {wall:.1f} s wall ({warm + first:.0f} CPU s), extrapolated to {total_rays} '
+                   f'rays x {evaluations} evaluations per table ({table_cpu:.0f} CPU s, {table_s:.0f} s per table '
+                   f'on {self.cores} cores)',
+            config0_end_to_end=dict(seconds=round(t0, 2), tables_per_hour=round(3600.0 / t0, 2),
+                                    evaluations=info['evaluations'], cores=self.cores,
+                                    what=f'config0 solved by the FP32 oracle to {self.cfg0["anderson_tol"]:g} '
+                                         f'(Anderson depth {self.cfg0["anderson_depth"]}) plus its '
+                                         f'{len(self.hs0)} x {len(self.el0)} x {len(self.az0)} table, cells and '
+                                         f'observer packs spread over the pool'),
+        )
+
+
+def config0_variant(tol_default=None):
+    """config0 on the GPU end to end (60 levels, 550 nm, L = 16, depth 5, tolerance 1e-5, its 2 x 4 x 6 table): the
+    workload the CPU baseline also solves end to end."""
+    import torch
+
+    from sosfuse2_amd import api, atmosphere
+
+    cfg = dict(api.CONFIG0)
+    wl, hs, el, az = api.axes(cfg)
+    s, scope = api.create_configured(None, cfg, wl, 'hip', atmosphere.config0_state())
+    tol = float(cfg['anderson_tol'])
+    scope.solver(s, log=None).solve(60, tol)  # warm-up: plans and constants
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        sc = scope.context(atmosphere.lane_optics(atmosphere.config0_state(), wl))
+        x, info = scope.solver(sc, log=None).solve(60, tol)
+        api.make_table(sc, x, hs, el, az, log=None)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t) / reps
+    return dict(seconds=round(sec, 4), tables_per_hour=round(3600.0 / sec, 1), evaluations=info['evaluations'],
+                cells=s.geo.ns)
 
 
 def main():
@@ -182,28 +315,38 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-tune', action='store_true', help='keep the default launch shapes (no autotuning)')
     ap.add_argument('--mixed', action='store_true', help='headline in the mixed-precision form (split-FP16 SH)')
-    ap.add_argument('--variants', default='mixed,batched',
-                    help='forms measured beside the headline: mixed, batched (comma list, or "none")')
-    ap.add_argument('--batched-states', type=int, default=8, help='states of the batched sample')
-    ap.add_argument('--learned', type=str, default='',
-                    help='TRAIN,TEST: train the warm-start network on TRAIN batched states, compare TEST states '
-                         'warm-started against cold (adds minutes)')
+    ap.add_argument('--variants', default='mixed,batched,learned,config0',
+                    help='forms measured beside the headline: mixed, batched, learned (learned + mixed + batched, '
+                         'needs batched), config0 (comma list, or "none")')
+    ap.add_argument('--batched-states', type=int, default=64, help='states of the batched sample')
+    ap.add_argument('--learned-train', type=int, default=16, help='training states of the warm-start network')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU: there is no multi-GPU path (asked for --gpus {args.gpus}); '
               'the work would shard by state and wavelength chunk, one process per GPU, with no exchange',
               file=sys.stderr)
         sys.exit(2)
+    from types import SimpleNamespace
+
     import numpy as np
     import torch
 
     from sosfuse2_amd import api, atmosphere
+    from sosfuse2_amd.kernel import Kernel
 
     cfg = dict(api.CONFIG1 if args.config == 'config1' else api.CONFIG0)
     state = atmosphere.config1_state() if args.config == 'config1' else atmosphere.config0_state()
     wl, hs, el, az = api.axes(cfg)
     t0 = time.perf_counter()
-    s, scope = api.create_configured(None, cfg, wl, 'hip', state, mixed=args.mixed)
+    # host preparation (grid, sun table, ground pools, cell packs) and the CPU baseline's worker processes come
+    # before anything starts the GPU runtime
+    scope = api.Scope(dict(cfg, sun_scale=1), state, SimpleNamespace(L=int(cfg['L'])))
+    cpu = None
+    if not args.no_cpu and args.config == 'config1':
+        cpu = CpuBaseline(scope, scope.context(atmosphere.lane_optics(state, wl)), dict(api.CONFIG0),
+                          atmosphere.config0_state())
+    scope.kernel = Kernel(backend='hip', L=int(cfg['L']), mixed=args.mixed)
+    s = scope.context(atmosphere.lane_optics(state, wl))
     solver = scope.solver(s, log=None)
     k = s.kernel
     k.plan(solver.geometry.bundle, k.constants(s, solver.geometry.bundle.nq))
@@ -249,26 +392,45 @@ def main():
     elapsed = time.perf_counter() - t
     timed, k.timed = k.timed, None
     per_step = elapsed / args.steps
-    warm = [a.elapsed_time(b) * 1e-3 for name, a, b in timed if name == 'warm']
     kinds = {}
     for name, a, b in timed:
         kinds.setdefault(name, []).append(a.elapsed_time(b) * 1e-3)
     log('launch seconds per step: ' + ', '.join(f'{n} {sum(v) / args.steps:.3f} ({len(v) // args.steps})'
                                                   for n, v in kinds.items()))
     launch_s = {n: round(sum(v) / args.steps, 4) for n, v in kinds.items()}
-    # the dominant kernel is the SH contraction of the warm pass: 2 K W flops per (node, corner) visit
-    sh = [x for x in kinds.get('sh_warm', [])]
+    # the dominant kernel is the SH contraction of the warm pass; its algorithmic work is the reference's:
+    # 2 K W flops per (ray, corner cell) refill of the per-ray SH cache (angular_pairs)
+    sh = kinds.get('sh_warm', [])
     t_sh = sum(sh) / max(1, len(sh))
-    plan = k.plan(solver.geometry.bundle, k.constants(s, solver.geometry.bundle.nq))
-    visits = int(plan.vis.numel())
-    flops = 2.0 * visits * k.K * s.W
+    db = solver.geometry.bundle
+    plan = k.plan(db, k.constants(s, db.nq))
+    pairs = angular_pairs(plan)
+    visits = int(plan.visits)
+    flops = 2.0 * pairs * k.K * s.W
     peak = PEAK_FP16 if args.mixed else PEAK_FP32
     achieved = flops / t_sh
     traffic = measured_traffic('sh_records', args.config, args.mixed)
     roofline = dict(bound='mfma', achieved=round(achieved / 1e12, 3), peak=peak / 1e12, unit='TFLOP/s',
-                    frac=round(achieved / peak, 4), traffic=traffic)
-    log(f'SH records (warm): {t_sh * 1e3:.2f} ms per launch, {visits} visits x K {k.K} x W {s.W} -> '
-        f'{flops / 1e12:.3f} TFLOP, {achieved / 1e12:.2f} TFLOP/s ({"FP16" if args.mixed else "FP32"} MFMA)')
+                    frac=round(achieved / peak, 4), traffic=traffic,
+                    kernel=f'sh_records ({"split-FP16" if args.mixed else "FP32"} MFMA), warm pass',
+                    work=f'2 K W per (ray, corner cell) refill: {pairs} refills (angular_pairs) x K {k.K} x W {s.W}; '
+                         f'the node-centric kernel computes {visits} (node, corner) visits, '
+                         f'{visits / pairs:.3f}x the refills',
+                    ms_per_launch=round(t_sh * 1e3, 3))
+    log(f'SH records (warm): {t_sh * 1e3:.2f} ms per launch, {pairs} refills ({visits} visits) x K {k.K} x W {s.W} '
+        f'-> {flops / 1e12:.3f} TFLOP, {achieved / 1e12:.2f} TFLOP/s ({"FP16" if args.mixed else "FP32"} MFMA)')
+    warm = kinds.get('warm', [])
+    t_c = sum(warm) / max(1, len(warm))
+    G = k.constants(s, db.nq).G
+    cbytes = consumer_bytes(plan, db.rays, s.geo.ns, G)
+    ctraffic = measured_traffic('transport_geom', args.config, args.mixed)
+    transport_roofline = dict(
+        kernel='transport_geom_kernel (warm consumer)', bound='hbm', achieved=round(cbytes / t_c / 1e9, 1),
+        peak=PEAK_HBM / 1e9, unit='GB/s', frac=round(cbytes / t_c / PEAK_HBM, 4), traffic=ctraffic,
+        measured=round(ctraffic / t_c / 1e9, 1) if ctraffic else None, ms_per_launch=round(t_c * 1e3, 3),
+        bytes=f'{cbytes / 1e9:.2f} GB per launch: records 4 x {plan.nodes} nodes x {G} lanes, node geometry 32 '
+              f'floats per node, tensor rows, ray constants and radiance out')
+    log(f'warm consumer: {t_c * 1e3:.2f} ms per launch, {cbytes / 1e9:.2f} GB -> {cbytes / t_c / 1e9:.0f} GB/s')
     out = dict(
         metric='radiance tables/hour/GPU at <=1e-4 rel err vs FP64; learned+batched vs plain',
         value=round(3600.0 / per_step, 2),
@@ -285,9 +447,10 @@ def main():
         config=dict(workload=f'{args.config}: {"mixed FP16 SH" if args.mixed else "plain FP32"} HIP, {s.geo.nh} levels x {s.geo.nc} chi, '
                              f'{s.W} wavelengths, L={k.L}, nmu={s.geo.nmu} nphi={s.geo.nphi} path_n={s.geo.path_n}, '
                              f'{len(hs)} solar elevations x {len(el)} x {len(az)} views, I/Q/U, tol {tol:g}',
-                    evaluations=evals, rays=solver.geometry.bundle.rays, prepare_seconds=round(prep, 1),
+                    evaluations=evals, rays=db.rays, prepare_seconds=round(prep, 1),
                     tuning=tuning),
         roofline=roofline,
+        transport_roofline=transport_roofline,
         launch_seconds_per_step=launch_s,
     )
     variants = [v for v in args.variants.split(',') if v and v != 'none']
@@ -312,13 +475,21 @@ def main():
             out['mixed'] = dict(tables_per_hour=round(3600.0 / t_other, 2), evaluations=info_o['evaluations'],
                                 error_vs_fp32=err, form='split-FP16 SH contraction (v_mfma_f32_16x16x32_f16)')
         log(f'mixed vs FP32 table: {err}')
-    if 'batched' in variants and args.batched_states > 0:
-        out['batched'] = batched_variant(scope, args.batched_states, el, az, tol)
-    if args.learned:
-        ntrain, ntest = (int(v) for v in args.learned.split(','))
-        out['learned'] = learned_variant(scope, ntrain, ntest, el, az, tol)
-    if not args.no_cpu:
-        out['cpu_baseline'] = cpu_baseline(s, scope, int(np.mean(evals)))
+    if 'batched' in variants and args.batched_states > 0 and args.config == 'config1':
+        out['batched'], items = batched_variant(scope, args.batched_states, el, az, tol)
+        out['batched']['vs_plain'] = round(out['batched']['normalised_tables_per_hour'] / out['value'], 3)
+        log(f'batched: {out["batched"]}')
+        if 'learned' in variants:
+            cold, cold_res = batched_run(scope, items, el, az, tol, mixed=True)
+            out['batched_mixed'] = dict(cold, vs_plain=round(cold['normalised_tables_per_hour'] / out['value'], 3))
+            lm = learned_variant(scope, items, el, az, tol, args.learned_train, cold_res)
+            lm['vs_plain'] = round(lm['normalised_tables_per_hour'] / out['value'], 3)
+            out['learned_mixed_batched'] = lm
+            log(f'learned + mixed + batched: {lm}')
+    if 'config0' in variants:
+        out['config0'] = config0_variant()
+    if cpu is not None:
+        out['cpu_baseline'] = cpu.run(int(round(np.mean(evals))))
     print(json.dumps(out), flush=True)
 
 

@@ -75,6 +75,9 @@ typedef struct sos_bundle {
     const float *node_sun_dev;    /* [nodes][12] sun geometry of first light (SOS_NODE_GEOMETRY with SOS_FIRST) */
 } sos_bundle;
 
+enum { SOS_PROFILE_SHAPES = 0, SOS_PROFILE_EXP = 1, SOS_PROFILE_GAUSS = 2, SOS_PROFILE_TRIANGLE = 3,
+       SOS_PROFILE_TABLE = 4, SOS_PROFILE_AIR_SQUARED = 5, SOS_PROFILE_ZERO = 6 };
+
 typedef struct sos_grid {
     int32_t nh, nc;            /* heights, solar zenith nodes; cells = nh * nc */
     int32_t L;                 /* SH order: 8, 16 or 32 */
@@ -106,6 +109,17 @@ typedef struct sos_grid {
                                   L = 8) */
     int32_t pool_max;          /* largest pool count of a cell (goff_dev differences); sos_ground_records holds a
                                   cell's pools in registers and takes at most 640 */
+    /* Profile kinds of air, aerosol, O3, NO2, O4 and the sulfate layer (numpy_reference.py:187-223 evaluates the
+     * reference's: exp(-h / 8), exp(-h / H), a triangle ozone, a tabulated NO2, air^2). SOS_PROFILE_SHAPES (0) takes
+     * the analytic shape of the fields above for that component (exponential air and aerosol, Gaussian O3, NO2 and
+     * sulfate, O4 as air^2); otherwise profile_a / profile_b are: SOS_PROFILE_EXP scale height (km) / -;
+     * SOS_PROFILE_GAUSS sigma / centre; SOS_PROFILE_TRIANGLE half width / centre; SOS_PROFILE_TABLE step (km) / -,
+     * values in row k of profile_table_dev [6][profile_table_n] from height 0; SOS_PROFILE_AIR_SQUARED the air
+     * profile's surface value / - (O4 = (air / surface)^2); SOS_PROFILE_ZERO. */
+    int32_t profile_kind[6];
+    float profile_a[6], profile_b[6];
+    const float *profile_table_dev;
+    int32_t profile_table_n;
 } sos_grid;
 
 /* Radiance tensor of every ray: out_dev [rays][lanes][pairs] (pairs 4: xx yy zz xz; 6: + xy yz), FP32.

@@ -107,10 +107,58 @@ def compose(moment, c, F):
     return out
 
 
-class OracleSolver:
-    """CPU solve of one chunk. dtype float32 = the FP32-FTZ restatement; float64 = the FP64 twin."""
+_WORKER = {}
 
-    def __init__(self, ctx, dtype=np.float32, depth=4, packs=None):
+
+def _worker_init(c, cq, packs):
+    """Process-pool worker state: the constants of both node counts and the cell packs, sent once."""
+    _WORKER.update(c=c, cq=cq, packs=packs)
+
+
+def _cells_job(args):
+    """Transport and projection of a run of cells in a pool worker (the per-cell loop of evaluate)."""
+    from sosfuse2_amd import geometry
+
+    cells, field, first, phase, boundary = args
+    c = _WORKER['c']
+    sel = [_WORKER['packs'][i] for i in cells]
+    cat = geometry.concat_packs(sel)
+    cat['end_columns'] = np.concatenate([p['end_columns'] for p in sel])
+    off = np.r_[0, np.cumsum([len(p['d']) for p in sel])]
+    stats = dict(samples=0, angular_pairs=0)
+    value = transport(cat, c, field, first, boundary, PAIRS4, stats)
+    K = count(c['L'])
+    moments = np.zeros((len(cells), 4 + K, c['W']), 'f8')
+    flux = np.zeros((len(cells), c['W']), 'f8')
+    for i in range(len(cells)):
+        sub = {k: cat[k][off[i] : off[i + 1]] for k in ('d', 'dloc', 'weight')}
+        moments[i], flux[i] = project(sub, value[off[i] : off[i + 1]], c, ground_only=phase == 1)
+    return moments, flux, stats
+
+
+def _query_job(args):
+    """Observer-ray transport of one pack in a pool worker."""
+    pack, field, first, pairs = args
+    stats = dict(samples=0, angular_pairs=0)
+    return transport(pack, _WORKER['cq'], field, first, None, pairs, stats), stats
+
+
+def make_pool(ctx, packs, dtype=np.float32, workers=1):
+    """A spawn process pool whose workers hold the context's constants and packs (for OracleSolver(pool=...)). The
+    pool must be created before the process starts a GPU runtime: its workers are new interpreters."""
+    import multiprocessing as mp
+
+    c = ctx.constants(ctx.geo.path_n, dtype)
+    cq = ctx.constants(ctx.geo.query_n, dtype)
+    return mp.get_context('spawn').Pool(workers, initializer=_worker_init, initargs=(c, cq, packs))
+
+
+class OracleSolver:
+    """CPU solve of one chunk. dtype float32 = the FP32-FTZ restatement; float64 = the FP64 twin. pool: a process
+    pool from make_pool over the same context and packs, which runs the per-cell transports and the observer
+    queries in parallel (the same arithmetic, cell by cell)."""
+
+    def __init__(self, ctx, dtype=np.float32, depth=4, packs=None, pool=None, workers=1):
         from sosfuse2_amd import geometry  # input preparation only
 
         self.ctx = ctx
@@ -125,6 +173,8 @@ class OracleSolver:
         self.packs = packs
         self.first_moments = None
         self.first_flux = None
+        self.pool = pool
+        self.workers = workers
         self.stats = dict(samples=0, angular_pairs=0)
         self.timings = dict(apply_seconds=[])
 
@@ -138,8 +188,20 @@ class OracleSolver:
             setattr(self, key, (cat, off))
         return getattr(self, key)
 
+    def _merge(self, stats):
+        for k, v in stats.items():
+            self.stats[k] += v
+
     def evaluate(self, field, first=False, phase=0, ground_only=False, boundary=None):
         count_ = self.ctx.geo.nc if ground_only else self.ctx.geo.ns
+        if self.pool is not None:
+            step = max(1, -(-count_ // (4 * self.workers)))
+            jobs = [(list(range(a, min(a + step, count_))), field, first, phase, boundary)
+                    for a in range(0, count_, step)]
+            parts = self.pool.map(_cells_job, jobs)
+            for _, _, st in parts:
+                self._merge(st)
+            return np.concatenate([p[0] for p in parts]), np.concatenate([p[1] for p in parts])
         cat, off = self._cat(count_)
         value = transport(cat, self.c, field, first, boundary, PAIRS4, self.stats)
         K = count(self.c['L'])
@@ -174,6 +236,11 @@ class OracleSolver:
     def query_raw(self, h_s, el, az, field, first, pairs=PAIRS4):
         packs = self.ctx.query_packs(h_s, el, az)
         cat = self.g.concat_packs(packs)
+        if self.pool is not None:
+            parts = self.pool.map(_query_job, [(p, field, first, pairs) for p in packs])
+            for _, st in parts:
+                self._merge(st)
+            return np.concatenate([p[0] for p in parts]), cat
         return transport(cat, self.cq, field, first, None, pairs, self.stats), cat
 
     def query(self, h_s, el, az, field, first=False):

@@ -42,21 +42,42 @@ PAIRS4 = ((0, 0), (1, 1), (2, 2), (0, 2))
 PAIRS6 = PAIRS4 + ((0, 1), (1, 2))
 
 
+# Profile kinds of a state's shapes (the product's atmosphere.ProfileSpec codes)
+_EXP, _GAUSS, _TRIANGLE, _TABLE, _AIR_SQUARED, _ZERO = 1, 2, 3, 4, 5, 6
+
+
+def _profile(spec, k, h, air, F):
+    """Profile k at FP heights h (>= 0) from a shape spec (kinds, a, b, table): exponential exp(-h / a), Gaussian
+    exp(-((h - b) / a)^2 / 2), triangle max(0, 1 - |h - b| / a) (numpy_reference.py:205), table interpolated as
+    interpolate32 (x = h / step, i = min(int x, n - 2), t[i] + (x - i)(t[i + 1] - t[i]); density.py:677-690), the
+    square of the air profile over its surface value (O4, numpy_reference.py:219-220), zero."""
+    kind, a, b = spec.kinds[k], F(spec.a[k]), F(spec.b[k])
+    if kind == _EXP:
+        return np.exp(-h / a)
+    if kind == _GAUSS:
+        t = (h - b) / a
+        return np.exp(F(-0.5) * (t * t))
+    if kind == _TRIANGLE:
+        return np.maximum(F(0), F(1) - np.abs(h - b) / a)
+    if kind == _TABLE:
+        row = np.asarray(spec.table[k], F)
+        x = np.maximum(h, F(0)) * (F(1) / a)
+        i = np.minimum(x.astype('i8'), len(row) - 2)
+        f = x - i.astype(F)
+        return row[i] + f * (row[i + 1] - row[i])
+    if kind == _AIR_SQUARED:
+        return (air * air) / (a * a)
+    if kind == _ZERO:
+        return np.zeros_like(h)
+    raise ValueError(f'E_PROFILE_KIND: {kind}')
+
+
 def _profiles(h, c, F):
     """Five component profiles and the sulfate profile at FP heights h (profile_at, numpy_reference.py:187-223)."""
-    p = c['profile']
-    air = np.exp(-h / F(p['air_H']))
-    aer = np.exp(-h / F(p['aer_H']))
-    t = (h - F(p['o3_c'])) / F(p['o3_s'])
-    o3 = np.exp(F(-0.5) * (t * t))
-    t = (h - F(p['no2_c'])) / F(p['no2_s'])
-    no2 = np.exp(F(-0.5) * (t * t))
-    prof = np.stack([air, aer, o3, no2, air * air], -1)
-    if c['strato']:
-        t = (h - F(p['st_c'])) / F(p['st_s'])
-        st = np.exp(F(-0.5) * (t * t))
-    else:
-        st = np.zeros_like(h)
+    spec = c['profile']['spec']
+    air = _profile(spec, 0, h, None, F)
+    prof = np.stack([air] + [_profile(spec, k, h, air, F) for k in (1, 2, 3, 4)], -1)
+    st = _profile(spec, 5, h, air, F) if c['strato'] else np.zeros_like(h)
     return prof, st
 
 

@@ -27,6 +27,53 @@ LOSCHMIDT_DU = 2.687e16  # molecules cm^-2 per Dobson unit
 KM_CM = 1e5
 
 
+# Profile kinds (the device's, include/sosfuse2_hip.h): exponential (a = scale height), Gaussian (a = sigma,
+# b = centre), triangle (a = half width, b = centre), table (a = step, values from height 0), the square of the air
+# profile over its surface value (a = surface value; O4), zero.
+EXP, GAUSS, TRIANGLE, TABLE, AIR_SQUARED, ZERO = 1, 2, 3, 4, 5, 6
+
+
+@dataclass(frozen=True)
+class ProfileSpec:
+    """Shapes of the five component profiles (air, aerosol, O3, NO2, O4) and the sulfate layer: kinds[k] with
+    parameters a[k], b[k]; `table` holds one row per component (tuples, so the spec is hashable) on a uniform grid of
+    step a[k] from height 0 for the TABLE kinds."""
+
+    kinds: tuple
+    a: tuple
+    b: tuple
+    table: tuple = None
+
+    def table_array(self):
+        return None if self.table is None else np.asarray(self.table, 'f8')
+
+
+def evaluate_profile(spec, k, h, air=None, dtype=np.float64):
+    """Profile k of a spec at heights h (>= 0) in dtype; air: the air profile for AIR_SQUARED."""
+    F = np.dtype(dtype).type
+    h = np.asarray(h, dtype)
+    kind, a, b = spec.kinds[k], F(spec.a[k]), F(spec.b[k])
+    if kind == EXP:
+        return np.exp(-h / a)
+    if kind == GAUSS:
+        t = (h - b) / a
+        return np.exp(F(-0.5) * (t * t))
+    if kind == TRIANGLE:
+        return np.maximum(F(0), F(1) - np.abs(h - b) / a)
+    if kind == TABLE:
+        row = np.asarray(spec.table[k], dtype)
+        x = np.maximum(h, F(0)) * (F(1) / a)
+        i = np.minimum(x.astype('i8'), len(row) - 2)
+        f = x - i.astype(dtype)
+        return row[i] + f * (row[i + 1] - row[i])
+    if kind == AIR_SQUARED:
+        q = air if air is not None else evaluate_profile(spec, 0, h, dtype=dtype)
+        return (q * q) / (a * a)
+    if kind == ZERO:
+        return np.zeros_like(h)
+    raise ValueError(f'E_PROFILE_KIND: {kind}')
+
+
 @dataclass(frozen=True)
 class State:
     """One atmosphere parameter set (a synthetic state of the configurations)."""
@@ -55,10 +102,22 @@ class State:
     sulfate_ssa: float = 1.0
     sulfate_angstrom: float = 1.0
     albedo: float = 0.1
+    profile_spec: ProfileSpec = None  # None: the analytic shapes of the fields above
 
     @property
     def strato(self):
         return self.sulfate_aod550 > 0
+
+    @property
+    def profiles(self):
+        """The ProfileSpec of this state: its own, or exponential air and aerosol, Gaussian ozone, NO2 and sulfate,
+        O4 as the square of the air profile."""
+        if self.profile_spec is not None:
+            return self.profile_spec
+        return ProfileSpec(kinds=(EXP, EXP, GAUSS, GAUSS, AIR_SQUARED, GAUSS),
+                           a=(self.air_height_km, self.aerosol_height_km, self.ozone_sigma_km, self.no2_sigma_km, 1.0,
+                              self.sulfate_sigma_km),
+                           b=(0.0, 0.0, self.ozone_center_km, self.no2_center_km, 0.0, self.sulfate_center_km))
 
     @property
     def mix(self):
@@ -105,25 +164,18 @@ def gaussian(h, center, sigma):
 
 
 def profiles(state, h):
-    """Component profiles [n, 5] at heights h (km), FP64. Air and aerosol are 1 at the ground; ozone and NO2 peak at 1;
-    O4 is the square of the air profile (`numpy_reference.py:219-220`)."""
+    """Component profiles [n, 5] at heights h (km), FP64, from the state's ProfileSpec. With the analytic shapes air
+    and aerosol are 1 at the ground, ozone and NO2 peak at 1 and O4 is the square of the air profile
+    (`numpy_reference.py:219-220`)."""
     h = np.maximum(np.asarray(h, 'f8'), 0.0)
-    air = np.exp(-h / state.air_height_km)
-    return np.stack(
-        [
-            air,
-            np.exp(-h / state.aerosol_height_km),
-            gaussian(h, state.ozone_center_km, state.ozone_sigma_km),
-            gaussian(h, state.no2_center_km, state.no2_sigma_km),
-            air * air,
-        ],
-        -1,
-    )
+    spec = state.profiles
+    air = evaluate_profile(spec, 0, h)
+    return np.stack([air] + [evaluate_profile(spec, k, h, air) for k in (1, 2, 3, 4)], -1)
 
 
 def strato_profile(state, h):
     """Sulfate layer profile, peak 1 (`numpy_reference.py:423-428` with norm 1)."""
-    return gaussian(np.maximum(np.asarray(h, 'f8'), 0.0), state.sulfate_center_km, state.sulfate_sigma_km)
+    return evaluate_profile(state.profiles, 5, np.maximum(np.asarray(h, 'f8'), 0.0))
 
 
 def _column_exp(H, top):
@@ -194,21 +246,40 @@ class Optics:
         return bool(np.any(self.strato_ext > 0))
 
 
+def profile_column(spec, k, top):
+    """Vertical column (km) of profile k of a spec from 0 to top: closed forms for the exponential and Gaussian kinds,
+    Gauss-Legendre on 400 pieces for the others (piecewise-linear shapes are integrated exactly on the table nodes
+    and at the triangle's corners, which fall on piece boundaries of the 0.5 km multiples used)."""
+    kind = spec.kinds[k]
+    if kind == EXP:
+        return _column_exp(spec.a[k], top)
+    if kind == GAUSS:
+        return _column_gauss(spec.b[k], spec.a[k], top)
+    x, w = np.polynomial.legendre.leggauss(8)
+    edges = np.linspace(0.0, top, 401)
+    h = ((edges[:-1] + edges[1:]) / 2)[:, None] + (np.diff(edges) / 2)[:, None] * x[None, :]
+    vals = evaluate_profile(spec, k, h.ravel(), evaluate_profile(spec, 0, h.ravel())).reshape(h.shape)
+    return float(np.sum(vals * w[None, :] * (np.diff(edges) / 2)[:, None]))
+
+
 def lane_optics(state, wl):
-    """Optics of one state at wavelengths wl (nm)."""
+    """Optics of one state at wavelengths wl (nm). Each component's extinction at unit profile is its column amount
+    over the column of its profile shape."""
     wl = np.atleast_1d(np.asarray(wl, 'f8'))
     top = state.top_km
-    ray = state.rayleigh_tau550 / _column_exp(state.air_height_km, top) * (550.0 / wl) ** 4
-    aer = state.aod550 / _column_exp(state.aerosol_height_km, top) * (550.0 / wl) ** state.angstrom
-    oz_col = state.ozone_du * LOSCHMIDT_DU / (_column_gauss(state.ozone_center_km, state.ozone_sigma_km, top) * KM_CM)
+    spec = state.profiles
+    ray = state.rayleigh_tau550 / profile_column(spec, 0, top) * (550.0 / wl) ** 4
+    aer = state.aod550 / profile_column(spec, 1, top) * (550.0 / wl) ** state.angstrom
+    oz_col = state.ozone_du * LOSCHMIDT_DU / (profile_column(spec, 2, top) * KM_CM)
     o3 = ozone_cross_section(wl) * oz_col * KM_CM
-    no2_n = state.no2_column / (_column_gauss(state.no2_center_km, state.no2_sigma_km, top) * KM_CM)
+    no2_cols = profile_column(spec, 3, top)
+    no2_n = state.no2_column / (no2_cols * KM_CM) if no2_cols > 0 else 0.0
     no2 = no2_cross_section(wl) * no2_n * KM_CM
     n_o2 = 0.20946 * 2.547e19  # O2 number density at the ground (cm^-3)
     o4 = o4_cross_section(wl) * n_o2 * n_o2 * KM_CM if state.o4 else np.zeros_like(wl)
     ext = np.stack([ray, aer, o3, no2, o4])
     if state.strato:
-        col = _column_gauss(state.sulfate_center_km, state.sulfate_sigma_km, top)
+        col = profile_column(spec, 5, top)
         sext = state.sulfate_aod550 / col * (550.0 / wl) ** state.sulfate_angstrom
     else:
         sext = np.zeros_like(wl)
@@ -230,7 +301,7 @@ def lane_optics(state, wl):
 
 
 PROFILE_KEYS = ('top_km', 'air_height_km', 'aerosol_height_km', 'ozone_center_km', 'ozone_sigma_km', 'no2_center_km',
-                'no2_sigma_km', 'sulfate_center_km', 'sulfate_sigma_km')
+                'no2_sigma_km', 'sulfate_center_km', 'sulfate_sigma_km', 'profile_spec')
 
 
 def check_profile_shapes(state, base):

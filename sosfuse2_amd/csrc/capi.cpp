@@ -91,6 +91,37 @@ int sos_device_info(int32_t *cus, int32_t *xcds, char *arch, int32_t arch_len) {
     return SOS_OK;
 }
 
+// The device form of a grid's profile kinds: SOS_PROFILE_SHAPES resolved to the analytic shape of its component,
+// scales turned into the inverses the device multiplies by.
+static ProfileShape profile_shape(const sos_grid *g) {
+    ProfileShape s;
+    const int def_kind[6] = {PROFILE_EXP, PROFILE_EXP, PROFILE_GAUSS, PROFILE_GAUSS, PROFILE_AIR_SQUARED, PROFILE_GAUSS};
+    const float def_a[6] = {1.0f / g->air_H, 1.0f / g->aer_H, 1.0f / g->o3_s, 1.0f / g->no2_s, 1.0f, 1.0f / g->st_s};
+    const float def_b[6] = {0.0f, 0.0f, g->o3_c, g->no2_c, 0.0f, g->st_c};
+    for (int k = 0; k < 6; ++k) {
+        const float a = g->profile_a[k], b = g->profile_b[k];
+        switch (g->profile_kind[k]) {
+        case SOS_PROFILE_EXP: s.kind[k] = PROFILE_EXP; s.a[k] = 1.0f / a; s.b[k] = 0.0f; break;
+        case SOS_PROFILE_GAUSS: s.kind[k] = PROFILE_GAUSS; s.a[k] = 1.0f / a; s.b[k] = b; break;
+        case SOS_PROFILE_TRIANGLE: s.kind[k] = PROFILE_TRIANGLE; s.a[k] = 1.0f / a; s.b[k] = b; break;
+        case SOS_PROFILE_TABLE: s.kind[k] = PROFILE_TABLE; s.a[k] = 1.0f / a; s.b[k] = 0.0f; break;
+        case SOS_PROFILE_AIR_SQUARED: s.kind[k] = PROFILE_AIR_SQUARED; s.a[k] = 1.0f / (a * a); s.b[k] = 0.0f; break;
+        case SOS_PROFILE_ZERO: s.kind[k] = PROFILE_ZERO; s.a[k] = 0.0f; s.b[k] = 0.0f; break;
+        default: s.kind[k] = def_kind[k]; s.a[k] = def_a[k]; s.b[k] = def_b[k]; break;
+        }
+    }
+    s.table = g->profile_table_dev;
+    s.table_n = g->profile_table_n;
+    return s;
+}
+
+// A grid whose profile kinds ask for a table must carry one of at least two nodes.
+static bool profile_table_ok(const sos_grid *g) {
+    for (int k = 0; k < 6; ++k)
+        if (g->profile_kind[k] == SOS_PROFILE_TABLE && (!g->profile_table_dev || g->profile_table_n < 2)) return false;
+    return true;
+}
+
 static void fill_args(TransportArgs &a, const sos_grid *g, const sos_bundle *b) {
     memset(&a, 0, sizeof(a));
     a.nh = g->nh; a.nc = g->nc; a.nq = g->nq; a.strato = g->strato; a.top = g->top_km;
@@ -123,8 +154,8 @@ int sos_plan_geometry(const sos_grid *g, const sos_bundle *b, const int64_t *nod
     if (!g || !b || !node_off_dev || !geom_dev) return arg_error("sos_plan_geometry: null argument");
     TransportArgs a;
     fill_args(a, g, b);
-    a.shape = ProfileShape{1.0f / g->air_H, 1.0f / g->aer_H, g->o3_c, 1.0f / g->o3_s,
-                           g->no2_c, 1.0f / g->no2_s, g->st_c, 1.0f / g->st_s};
+    if (!profile_table_ok(g)) return arg_error("profile table missing for a table profile kind");
+    a.shape = profile_shape(g);
     a.targets = b->targets_dev; a.dloc = b->dloc_dev; a.nc = g->nc; a.nh = g->nh;
     return plan_geometry_launch(a, node_off_dev, geom_dev, (hipStream_t)stream);
 }
@@ -134,8 +165,8 @@ int sos_plan_sun(const sos_grid *g, const sos_bundle *b, const int64_t *node_off
     if (!g->sun_col_dev || g->sun_nh < 2 || g->sun_nx < 2) return arg_error("sos_plan_sun: no sun table");
     TransportArgs a;
     fill_args(a, g, b);
-    a.shape = ProfileShape{1.0f / g->air_H, 1.0f / g->aer_H, g->o3_c, 1.0f / g->o3_s,
-                           g->no2_c, 1.0f / g->no2_s, g->st_c, 1.0f / g->st_s};
+    if (!profile_table_ok(g)) return arg_error("profile table missing for a table profile kind");
+    a.shape = profile_shape(g);
     a.sun_nh = g->sun_nh; a.sun_nx = g->sun_nx; a.sun_h = g->sun_h_dev; a.sun_col = g->sun_col_dev;
     return plan_sun_launch(a, node_off_dev, sun_dev, (hipStream_t)stream);
 }
@@ -151,8 +182,8 @@ int sos_ground_records(const sos_grid *g, int32_t groups, const float *group_g, 
     if (g->lanes != 16 && g->lanes != 32) return arg_error("sos_ground_records: lanes %d (16 or 32)", g->lanes);
     GroundArgs a;
     a.G = g->lanes; a.strato = g->strato;
-    a.shape = ProfileShape{1.0f / g->air_H, 1.0f / g->aer_H, g->o3_c, 1.0f / g->o3_s,
-                           g->no2_c, 1.0f / g->no2_s, g->st_c, 1.0f / g->st_s};
+    if (!profile_table_ok(g)) return arg_error("profile table missing for a table profile kind");
+    a.shape = profile_shape(g);
     a.items = items; a.item_cell = item_cell_dev; a.item_v0 = item_v0_dev; a.item_v1 = item_v1_dev; a.vis = vis_dev;
     a.node_dir = reinterpret_cast<const float4 *>(node_dir_dev); a.goff = g->goff_dev; a.gdir = g->gdir_dev;
     a.gval = g->gval_dev; a.lane = g->lane_dev; a.records = records_dev;
@@ -230,8 +261,8 @@ int sos_transport(const sos_grid *g, const sos_bundle *b, int32_t flags, int32_t
     a.nh = g->nh; a.nc = g->nc; a.nq = g->nq; a.strato = g->strato; a.flags = flags; a.pairs = pairs; a.rows = rows;
     a.blocks_per_cu = g->blocks_per_cu;
     a.sun_nh = g->sun_nh; a.sun_nx = g->sun_nx; a.top = g->top_km;
-    a.shape = ProfileShape{1.0f / g->air_H, 1.0f / g->aer_H, g->o3_c, 1.0f / g->o3_s,
-                           g->no2_c, 1.0f / g->no2_s, g->st_c, 1.0f / g->st_s};
+    if (!profile_table_ok(g)) return arg_error("profile table missing for a table profile kind");
+    a.shape = profile_shape(g);
     a.gl_u = g->gl_u_dev; a.gl_w = g->gl_w_dev; a.h = g->h_dev; a.chi = g->chi_dev; a.hr = g->hr_dev;
     a.lane = g->lane_dev; a.sun_h = g->sun_h_dev; a.sun_col = g->sun_col_dev; a.goff = g->goff_dev;
     a.gdir = g->gdir_dev; a.gval = g->gval_dev; a.gm = g->gm_dev;

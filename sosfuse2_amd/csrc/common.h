@@ -52,22 +52,55 @@ __device__ inline float hg_fast(float u, const HgConst &c) {
     return c.norm * ((r * r) * r);
 }
 
+// Height profiles of the five components (air, tropospheric aerosol, O3, NO2, O4) and the sulfate layer (index 5).
+// Each has a kind: exponential exp(-h a) (a = 1 / scale height), Gaussian exp(-((h - b) a)^2 / 2) (a = 1 / sigma,
+// b = centre), triangle max(0, 1 - |h - b| a) (a = 1 / half width), table (linear interpolation of table[k][i] at
+// x = h a, a = 1 / step, as numpy_reference.profile_at's interpolate32, density.py:677-690), square of the air
+// profile times a (O4 = (air / air_surface)^2, numpy_reference.py:219-220), or zero.
+enum ProfileKind { PROFILE_EXP = 0, PROFILE_GAUSS = 1, PROFILE_TRIANGLE = 2, PROFILE_TABLE = 3, PROFILE_AIR_SQUARED = 4,
+                   PROFILE_ZERO = 5 };
+
 struct ProfileShape {
-    float inv_air_H, inv_aer_H, o3_c, inv_o3_s, no2_c, inv_no2_s, st_c, inv_st_s;
+    int kind[6];
+    float a[6], b[6];
+    const float *table;  // [6][table_n] rows of the table kinds
+    int table_n;
 };
+
+__device__ inline float profile_value(const ProfileShape &s, int k, float h, float air) {
+    const float a = s.a[k], b = s.b[k];
+    switch (s.kind[k]) {
+    case PROFILE_EXP:
+        return expf(-h * a);
+    case PROFILE_GAUSS: {
+        const float t = (h - b) * a;
+        return expf(-0.5f * (t * t));
+    }
+    case PROFILE_TRIANGLE:
+        return fmaxf(0.0f, 1.0f - fabsf(h - b) * a);
+    case PROFILE_TABLE: {
+        const float x = fmaxf(h, 0.0f) * a;
+        const int i = min((int)x, s.table_n - 2);
+        const float f = x - (float)i;
+        const float *t = s.table + k * s.table_n;
+        return t[i] + f * (t[i + 1] - t[i]);
+    }
+    case PROFILE_AIR_SQUARED:
+        return (air * air) * a;
+    default:
+        return 0.0f;
+    }
+}
 
 // Five component profiles and the sulfate profile at height h.
 __device__ inline void profiles(const ProfileShape &s, float h, float *p, float &st) {
-    float air = expf(-h * s.inv_air_H);
+    const float air = profile_value(s, 0, h, 0.0f);
     p[0] = air;
-    p[1] = expf(-h * s.inv_aer_H);
-    float t = (h - s.o3_c) * s.inv_o3_s;
-    p[2] = expf(-0.5f * (t * t));
-    t = (h - s.no2_c) * s.inv_no2_s;
-    p[3] = expf(-0.5f * (t * t));
-    p[4] = air * air;
-    t = (h - s.st_c) * s.inv_st_s;
-    st = expf(-0.5f * (t * t));
+    p[1] = profile_value(s, 1, h, air);
+    p[2] = profile_value(s, 2, h, air);
+    p[3] = profile_value(s, 3, h, air);
+    p[4] = profile_value(s, 4, h, air);
+    st = profile_value(s, 5, h, air);
 }
 
 struct TransportArgs {

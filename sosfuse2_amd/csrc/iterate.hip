@@ -378,10 +378,16 @@ int residual_launch(int64_t cells, int rows, int G, const float *g, const float 
     return check("anderson_residual");
 }
 
+constexpr int GRAM_MAX_BLOCKS = 1024;
+
+static int gram_blocks(int64_t n) {
+    int blocks = (int)min((int64_t)GRAM_MAX_BLOCKS, (n + 4095) / 4096);
+    return blocks < 1 ? 1 : blocks;
+}
+
 int gram_launch(int64_t n, int G, int depth, int slot, unsigned filled, const float *ring, double *work, double *gram,
                 hipStream_t s) {
-    int blocks = (int)min((int64_t)1024, (n + 4095) / 4096);
-    if (blocks < 1) blocks = 1;
+    const int blocks = gram_blocks(n);
     hipLaunchKernelGGL(gram_partial, dim3(blocks), dim3(256), 0, s, n, G, depth, slot, filled, ring, work);
     hipLaunchKernelGGL(gram_finish, dim3(1), dim3(256), 0, s, blocks, G, depth, slot, filled, work, gram);
     return check("anderson_gram");

@@ -120,7 +120,7 @@ class Context:
             sun_col=self.sun.col.astype(F),
             profile=dict(air_H=st.air_height_km, aer_H=st.aerosol_height_km, o3_c=st.ozone_center_km,
                          o3_s=st.ozone_sigma_km, no2_c=st.no2_center_km, no2_s=st.no2_sigma_km,
-                         st_c=st.sulfate_center_km, st_s=st.sulfate_sigma_km),
+                         st_c=st.sulfate_center_km, st_s=st.sulfate_sigma_km, spec=st.profiles),
             combine=True,
         )
 

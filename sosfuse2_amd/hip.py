@@ -9,6 +9,7 @@ c_i32, c_i64, c_u32, c_f32 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32, ct
 P = ctypes.c_void_p
 
 LANE_CONSTANTS = 12
+PROFILE_SHAPES, PROFILE_EXP, PROFILE_GAUSS, PROFILE_TRIANGLE, PROFILE_TABLE, PROFILE_AIR_SQUARED, PROFILE_ZERO = range(7)
 FIRST, BOUNDARY, RECORDS, UNIFORM_PHASE, GROUND_RECORDS, NODE_GEOMETRY = 1, 2, 8, 16, 32, 64
 
 
@@ -30,6 +31,8 @@ class Grid(ctypes.Structure):
         ('sun_nh', c_i32), ('sun_nx', c_i32), ('sun_h_dev', P), ('sun_col_dev', P), ('goff_dev', P),
         ('gdir_dev', P), ('gval_dev', P), ('gm_dev', P), ('blocks_per_cu', c_i32), ('sh_tile', c_i32),
         ('pool_max', c_i32),
+        ('profile_kind', c_i32 * 6), ('profile_a', c_f32 * 6), ('profile_b', c_f32 * 6), ('profile_table_dev', P),
+        ('profile_table_n', c_i32),
     ]
 
 

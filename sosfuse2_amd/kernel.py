@@ -205,13 +205,25 @@ class DeviceConstants:
         self.h, self.chi, self.hr = t(c['h'], 'f4'), t(c['chi'], 'f4'), t(c['hr'], 'f4')
         self.ha, self.sd = t(c['ha'], 'f4'), t(c['strato_density'], 'f4')
         self.sun_h, self.sun_col = t(c['sun_h'], 'f4'), t(c['sun_col'], 'f4')
-        pools = ctx.pools
-        pd = pool_device(pools, device)
-        self.goff, self.gdir = pd['off'], pd['loc32']
-        P = len(pools.loc)
-        # the state-dependent pool values and tensor moments are computed on the device (sos_ground_pools, below)
-        self.gval = torch.zeros((max(P, 1), G), dtype=torch.float32, device=device)
-        self.gm = torch.zeros((ctx.geo.ns, 4, G), dtype=torch.float32, device=device)
+        given = getattr(ctx, 'given_pools', None)
+        if given is not None:
+            # pool offsets, directions, values [P, W] and per-cell tensor moments [cells, 4, W] supplied by the caller
+            # (the reference's s.goff, s.gdir, s.gval, s.gm)
+            goff, gdir, gval, gm = given
+            P = len(gdir)
+            self.goff, self.gdir = t(goff, 'i8'), t(gdir if P else np.zeros((1, 3)), 'f4')
+            self.gval = t(_pad_lanes(gval, G) if P else np.zeros((1, G)), 'f4')
+            self.gm = t(_pad_lanes(gm, G), 'f4')
+            pool_max = int(np.diff(goff).max(initial=0))
+        else:
+            pools = ctx.pools
+            pd = pool_device(pools, device)
+            self.goff, self.gdir = pd['off'], pd['loc32']
+            P = len(pools.loc)
+            pool_max = int(np.diff(pools.off).max(initial=0))
+            # the state-dependent pool values and tensor moments are computed on the device (sos_ground_pools, below)
+            self.gval = torch.zeros((max(P, 1), G), dtype=torch.float32, device=device)
+            self.gm = torch.zeros((ctx.geo.ns, 4, G), dtype=torch.float32, device=device)
         self.factor = t(_pad_lanes(ctx.eigen(), G), 'f4')
         # one asymmetry per launch (a single-state chunk): first light may share phase evaluations across lanes
         self.uniform_phase = bool(np.all(lanes[6] == lanes[6][0]) and np.all(lanes[11] == lanes[11][0]))
@@ -232,12 +244,27 @@ class DeviceConstants:
             p['no2_c'], p['no2_s'], p['st_c'], p['st_s'],
             *(hip.ptr(x) for x in (self.gl_u, self.gl_w, self.h, self.chi, self.hr, self.lane)),
             len(c['sun_h']), c['sun_col'].shape[1], hip.ptr(self.sun_h), hip.ptr(self.sun_col), hip.ptr(self.goff),
-            hip.ptr(self.gdir), hip.ptr(self.gval), hip.ptr(self.gm), 0, 0,
-            int(np.diff(pools.off).max(initial=0)))
-        if P:
+            hip.ptr(self.gdir), hip.ptr(self.gval), hip.ptr(self.gm), 0, 0, pool_max)
+        set_profiles(self.struct, p['spec'], self, device)
+        if P and given is None:
             hip.call('sos_ground_pools', self.struct, hip.ptr(pd['off']), hip.ptr(pd['column']), hip.ptr(pd['factor']),
                      hip.ptr(pd["loc"]), hip.ptr(self.gval), hip.ptr(self.gm),
                      torch.cuda.current_stream(device).cuda_stream)
+
+
+def set_profiles(struct, spec, owner, device):
+    """Fill a grid struct's profile kinds from a ProfileSpec (atmosphere.ProfileSpec); a table goes to the device as
+    [6][n] FP32 rows, kept on `owner`."""
+    for k in range(6):
+        struct.profile_kind[k] = int(spec.kinds[k])
+        struct.profile_a[k] = float(spec.a[k])
+        struct.profile_b[k] = float(spec.b[k])
+    owner.profile_table = None
+    if spec.table is not None:
+        rows = spec.table_array()
+        owner.profile_table = torch.from_numpy(np.ascontiguousarray(rows, dtype='f4')).to(device)
+        struct.profile_table_dev = hip.ptr(owner.profile_table)
+        struct.profile_table_n = rows.shape[1]
 
 
 _POOLS = {}
