@@ -231,6 +231,15 @@ int sos_anderson_residual(int64_t cells, int32_t rows, int32_t lanes, const floa
 int sos_anderson_gram(int64_t n, int32_t lanes, int32_t depth, int32_t slot, uint32_t filled, const float *ring_dev,
                       double *work_dev, double *gram_dev, void *stream);
 
+/* sos_anderson_gram without the work buffer; no counterpart in the reference. For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops): it calls no allocator. The partial sums go to
+ * a file-scope device array of 1024 x 32 x 8 doubles (2 MiB per device, held until the process ends), enough for
+ * every argument the limits allow (lanes <= 32, depth <= 8); a larger need is rejected with an error. Every call on a
+ * device shares that device's array, so calls that overlap on one device race. Results equal sos_anderson_gram's
+ * bit for bit (the same kernels and summation order). */
+int sos_anderson_gram_static(int64_t n, int32_t lanes, int32_t depth, int32_t slot, uint32_t filled,
+                             const float *ring_dev, double *gram_dev, void *stream);
+
 /* Per lane, FP64 on the device: the regularised Gram system of the m slots order_dev[0..m-1] (oldest first),
  * G += max(tr G 1e-10, 1e-30) I, w = G^-1 1 / sum, accepted when finite and max |w| < 8 (source_iteration.py:25-33).
  * w_dev [lanes][depth] float; w[lane][0] = NaN marks a lane that keeps g. Lanes with bit set in frozen are skipped. */

@@ -49,6 +49,8 @@ int residual_launch(int64_t cells, int rows, int G, const float *g, const float 
                     float *work, float *f, hipStream_t s);
 int gram_launch(int64_t n, int G, int depth, int slot, unsigned filled, const float *ring, double *work, double *gram,
                 hipStream_t s);
+int gram_static_launch(int64_t n, int G, int depth, int slot, unsigned filled, const float *ring, double *gram,
+                       hipStream_t s);
 int solve_launch(int G, int depth, int m, const int *order, const double *gram, unsigned frozen, float *w,
                  hipStream_t s);
 int mix_launch(int64_t cells, int rows, int G, int depth, int m, const float *ring, const int *order, const float *w,
@@ -344,6 +346,13 @@ int sos_anderson_gram(int64_t n, int32_t lanes, int32_t depth, int32_t slot, uin
         lanes > 32)
         return arg_error("sos_anderson_gram: bad argument");
     return gram_launch(n, lanes, depth, slot, filled, ring_dev, work_dev, gram_dev, (hipStream_t)stream);
+}
+
+int sos_anderson_gram_static(int64_t n, int32_t lanes, int32_t depth, int32_t slot, uint32_t filled,
+                             const float *ring_dev, double *gram_dev, void *stream) {
+    if (!ring_dev || !gram_dev || depth < 1 || depth > 8 || slot < 0 || slot >= depth || lanes < 1 || lanes > 32)
+        return arg_error("sos_anderson_gram_static: bad argument");
+    return gram_static_launch(n, lanes, depth, slot, filled, ring_dev, gram_dev, (hipStream_t)stream);
 }
 
 int sos_anderson_solve(int32_t lanes, int32_t depth, int32_t m, const int32_t *order_dev, const double *gram_dev,

@@ -7,6 +7,8 @@
 // anderson_*   source_iteration.py:432-443 (scaled residual), :618-635 (Gram products and mixing), :25-33 (FP64
 //              regularised solve and guard). All floating sums have a fixed order.
 
+#include <mutex>
+
 #include "common.h"
 
 namespace sos {
@@ -391,6 +393,39 @@ int gram_launch(int64_t n, int G, int depth, int slot, unsigned filled, const fl
     hipLaunchKernelGGL(gram_partial, dim3(blocks), dim3(256), 0, s, n, G, depth, slot, filled, ring, work);
     hipLaunchKernelGGL(gram_finish, dim3(1), dim3(256), 0, s, blocks, G, depth, slot, filled, work, gram);
     return check("anderson_gram");
+}
+
+// Allocation-free Gram products: the per-block partial sums live in a file-scope device array instead of a caller
+// buffer. Its host-side name is only a stand-in for the symbol, so the launcher passes the device address that
+// hipGetSymbolAddress reports for the current device, looked up once per device and kept.
+constexpr int64_t GRAM_SCRATCH = (int64_t)GRAM_MAX_BLOCKS * 32 * 8;  // blocks x lanes x depth at their limits (2 MiB)
+static __device__ __align__(256) double scratch[GRAM_SCRATCH];
+
+static double *scratch_address() {
+    static std::mutex mu;
+    static double *addr[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        addr[dev] = static_cast<double *>(p);
+    }
+    return addr[dev];
+}
+
+int gram_static_launch(int64_t n, int G, int depth, int slot, unsigned filled, const float *ring, double *gram,
+                       hipStream_t s) {
+    const int blocks = gram_blocks(n);
+    if ((int64_t)blocks * G * depth > GRAM_SCRATCH)
+        return set_error("anderson_gram_static: %d blocks x %d lanes x depth %d exceed the %lld-double scratch", blocks,
+                         G, depth, (long long)GRAM_SCRATCH);
+    double *work = scratch_address();
+    if (!work) return set_error("anderson_gram_static: hipGetSymbolAddress of the scratch array failed");
+    hipLaunchKernelGGL(gram_partial, dim3(blocks), dim3(256), 0, s, n, G, depth, slot, filled, ring, work);
+    hipLaunchKernelGGL(gram_finish, dim3(1), dim3(256), 0, s, blocks, G, depth, slot, filled, work, gram);
+    return check("anderson_gram_static");
 }
 
 int solve_launch(int G, int depth, int m, const int *order, const double *gram, unsigned frozen, float *w,

@@ -53,6 +53,7 @@ _SIGNATURES = {
     'sos_source_metrics': [c_i64, c_i32, P, P, c_u32, P, P],
     'sos_anderson_residual': [c_i64, c_i32, c_i32, P, P, P, c_i32, P, P, P],
     'sos_anderson_gram': [c_i64, c_i32, c_i32, c_i32, c_u32, P, P, P, P],
+    'sos_anderson_gram_static': [c_i64, c_i32, c_i32, c_i32, c_u32, P, P, P],
     'sos_anderson_solve': [c_i32, c_i32, c_i32, P, P, c_u32, P, P],
     'sos_anderson_mix': [c_i64, c_i32, c_i32, c_i32, c_i32, P, P, P, P, P],
     'sos_mlp_gemm': [c_i64, c_i64, c_i64, P, c_i64, c_i64, P, c_i64, c_i64, P, c_i32, P, c_i64, P, c_i64, c_i32, P, P],
