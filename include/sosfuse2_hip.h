@@ -273,6 +273,67 @@ int sos_mlp_bias_grad(int64_t rows, int64_t cols, const float *dz_dev, int64_t l
 int sos_mlp_adam(int64_t n, float *p_dev, const float *g_dev, float *m_dev, float *v_dev, int32_t step, float lr,
                  float beta1, float beta2, float eps, void *stream);
 
+/* Column integrals on the device (FP64): the slant columns of the five component profiles and the sulfate profile
+ * along rays, Gauss-Legendre nodes on every segment between the ray's boundaries (shell exit or ground hit, tangent
+ * point, shadow-cylinder crossings, interior shells at 10, 25, 40 km and 80 km when the top is higher), graded
+ * quadratically toward each segment's lower end (rays/shell_segments.py:43-66,115-147). Profile kinds as in sos_grid
+ * (SOS_PROFILE_SHAPES is not accepted: give every kind), parameters in FP64, the table in FP64. */
+typedef struct sos_columns {
+    double top_km;
+    int32_t nq;
+    const double *gl_u_dev;         /* [nq] nodes on [0, 1] */
+    const double *gl_w_dev;         /* [nq] weights */
+    int32_t profile_kind[6];
+    double profile_a[6], profile_b[6];
+    const double *profile_table_dev; /* [6][profile_table_n] */
+    int32_t profile_table_n;
+} sos_columns;
+
+/* Sun-column table (replaces rays/shell_segments.py:267-307, SunGeometry's table): out_dev [nh][nx][6] = columns from
+ * (0, 0, R + h[i]) toward mu = mg + (1 - mg) x[j]^2 (mg the horizon's cosine there; mg + 1e-9 at j = 0). */
+int sos_sun_table(const sos_columns *c, int32_t nh, int32_t nx, const double *h_dev, const double *x_dev,
+                  double *out_dev, void *stream);
+
+/* Columns [n][6] from each ray's start (starts_dev[group_dev[i]], or starts_dev[i] without groups) along dirs_dev[i]
+ * to its end, for the rays with hit_dev[i] set (all rays without hit_dev; the others 0): the end columns of the
+ * ground-hitting rays (replaces solve/ray_geometry.py:53-65 and rays/ray_columns.py) and the path columns of the
+ * ground pools (solve/engine_core.py:103-147). */
+int sos_ray_columns(const sos_columns *c, int64_t n, const double *starts_dev, const int32_t *group_dev,
+                    const double *dirs_dev, const uint8_t *hit_dev, double *out_dev, void *stream);
+
+/* Incident-direction quadrature of every cell of a grid (replaces rays/incoming_rings.py:63-126): per azimuth j of
+ * nphi + 1 over [0, pi] (trapezoid, the y-mirror folded into the weights), nmu Gauss-Legendre nodes on every
+ * sinh-graded mu interval between the horizon, shadow, tangent-pair and ring edges; weights sum to 4 pi per cell. */
+typedef struct sos_incident {
+    int32_t nh, nc, nmu, nphi, nlevels;
+    const double *h_dev;            /* [nh] heights (km) */
+    const double *chi_dev;          /* [nc] solar zenith nodes (rad) */
+    double levels[8];               /* ring levels -sqrt((R + h)^2 - R^2) of the ring heights */
+    const double *gl_u_dev;         /* [nmu] nodes on [0, 1] */
+    const double *gl_w_dev;         /* [nmu] weights */
+} sos_incident;
+
+/* counts_dev [nh nc][nphi + 1]: the node count of every (cell, azimuth). */
+int sos_incident_count(const sos_incident *a, int64_t *counts_dev, void *stream);
+/* From the exclusive prefix sums off_dev of those counts: every node's direction (global frame, FP64 [rays][3]),
+ * solid-angle weight (FP64) and cell, in cell, azimuth, mu order. */
+int sos_incident_fill(const sos_incident *a, const int64_t *off_dev, double *dirs_dev, double *weights_dev,
+                      int32_t *ray_cell_dev, void *stream);
+
+/* Ray packs of a bundle (replaces solve/ray_geometry.py:28-82 pack and :148-180 _csr_vector): per ray from its cell's
+ * start along dirs_dev, the sorted boundaries (sos_columns' top), kept intervals of strictly positive width.
+ * sos_pack_count writes the compacted edge count of every ray (kept intervals + 1, or 1 with a single 0) and its
+ * intervals wider than 1e-8 km (the sampled ones); sos_pack_fill writes, from the prefix sums edge_off_dev, the
+ * compacted edges, the lower-end flag bits, the end's chi index and fraction, the ground hit and the direction in the
+ * cell's target frame (targets_dev [cells][3][3], frame vectors as columns). */
+int sos_pack_count(const sos_columns *c, int32_t nc, const double *chi_dev, const double *starts_dev, int64_t rays,
+                   const int32_t *ray_cell_dev, const double *dirs_dev, int64_t *edge_count_dev, int64_t *sampled_dev,
+                   void *stream);
+int sos_pack_fill(const sos_columns *c, int32_t nc, const double *chi_dev, const double *starts_dev,
+                  const double *targets_dev, int64_t rays, const int32_t *ray_cell_dev, const double *dirs_dev,
+                  const int64_t *edge_off_dev, double *edges_dev, int16_t *lower_dev, int32_t *end_j_dev,
+                  float *end_f_dev, uint8_t *hits_dev, float *dloc_dev, void *stream);
+
 const char *sos_last_error(void);
 /* Compute units, XCDs (hipDeviceAttributeNumberOfXccs) and the gfx architecture name of the current device. */
 int sos_device_info(int32_t *cus, int32_t *xcds, char *arch, int32_t arch_len);

@@ -9,7 +9,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / 'csrc'
 LIB = HERE / 'libsosfuse2_hip.so'
-SOURCES = ('transport.hip', 'sh.hip', 'ground.hip', 'project.hip', 'iterate.hip', 'capi.cpp', 'mlp.hip')
+SOURCES = ('transport.hip', 'sh.hip', 'ground.hip', 'project.hip', 'iterate.hip', 'geometry.hip', 'capi.cpp', 'mlp.hip')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = [
     '--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17', '-fgpu-flush-denormals-to-zero',

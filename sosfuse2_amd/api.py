@@ -12,6 +12,7 @@ Settings (`cfg`) name the grid and quadrature the way the reference's settings d
 top_km, the chi grid kind, query_n and the Anderson depth of this engine.
 """
 
+import os
 import time
 from pathlib import Path
 
@@ -53,18 +54,57 @@ class Scope:
     """State-independent preparation shared by every chunk and state on one grid: the grid, the sun-column table,
     the ground pools, the cell packs and their device bundles."""
 
-    def __init__(self, cfg, profile_state, kernel):
+    def __init__(self, cfg, profile_state, kernel, device_columns=False, device_geometry=False):
+        """device_columns: leave the column integrals (sun table, ground-pool and end columns) to the device;
+        device_geometry: also build the incident quadrature, ray packs and bundles there (finish_on_device, after the
+        kernel exists). The ground pools' directions are chosen here either way; host packs are built on first use
+        of `packs` when the bundles come from the device."""
         t = time.perf_counter()
         self.cfg = cfg
         self.grid = make_grid(cfg)
         self.kernel = kernel
         self.state = profile_state
-        self.sun = geometry.sun_table(profile_state, self.grid.domain, int(cfg.get('sun_scale', 1)))
-        workers = int(cfg.get('workers', geometry.default_workers()))
-        self.pools = geometry.ground_pools(profile_state, self.grid, self.sun, workers=workers)
-        self.packs = geometry.cell_packs(profile_state, self.grid, workers=workers)
+        self.device_columns = device_columns or device_geometry
+        self.device_geometry = device_geometry
+        self.workers = int(cfg.get('workers', geometry.default_workers()))
+        self.bundles = None
+        self._packs = None
+        if self.device_columns:
+            self.sun = None
+            self.pools = geometry.ground_pools(profile_state, self.grid, None, workers=self.workers, columns_too=False)
+            if not device_geometry:
+                self._packs = geometry.cell_packs(profile_state, self.grid, workers=self.workers, end_columns=False)
+        else:
+            self.sun = geometry.sun_table(profile_state, self.grid.domain, int(cfg.get('sun_scale', 1)))
+            self.pools = geometry.ground_pools(profile_state, self.grid, self.sun, workers=self.workers)
+            self._packs = geometry.cell_packs(profile_state, self.grid, workers=self.workers)
         self.geometry = None
         self.prepare_seconds = time.perf_counter() - t
+
+    @property
+    def packs(self):
+        """Host ray packs of every cell with their end columns (built on first use when the device built the
+        bundles)."""
+        if self._packs is None:
+            self._packs = geometry.cell_packs(self.state, self.grid, workers=self.workers)
+        return self._packs
+
+    def finish_on_device(self):
+        """The column integrals on the device (device_geometry): the sun-column table, the ground pools' path columns
+        and the packs' end columns, or with device_geometry the whole cell bundles."""
+        from . import device_geometry as dg
+
+        if self.sun is None:
+            t = time.perf_counter()
+            st, dom = self.state, self.grid.domain
+            self.sun = dg.sun_table(st, dom, int(self.cfg.get('sun_scale', 1)))
+            self.pools = dg.pool_columns(st, dom, self.pools, self.sun)
+            if self.device_geometry:
+                self.bundles = dg.cell_bundles(st, self.grid)
+            else:
+                dg.end_columns(st, dom, self._packs)
+            self.prepare_seconds += time.perf_counter() - t
+        return self
 
     def context(self, optics):
         return engine.Context(self.grid, optics, self.kernel, self.sun, self.pools, self.cfg.get('observer_km', 0.1),
@@ -72,7 +112,10 @@ class Scope:
 
     def solver(self, s, log=print, graph=False):
         if self.geometry is None:
-            self.geometry = Geometry(s, self.packs, self.kernel)
+            if self.bundles is not None:
+                self.geometry = Geometry(s, kernel=self.kernel, bundles=self.bundles)
+            else:
+                self.geometry = Geometry(s, self.packs, self.kernel)
         return Solver(s, self.geometry, depth=int(self.cfg.get('anderson_depth', 4)), log=log, graph=graph)
 
 
@@ -86,9 +129,12 @@ def create_configured(output, cfg, wavelengths, backend, state, block_size=256, 
         cfg = dict(cfg, sun_scale=sun_scale)
         if backend != 'hip':
             raise ValueError(f"E_BACKEND: backend {backend!r}; this engine runs 'hip' only")
-        # the host preparation (process pools) runs before the kernel brings up the GPU runtime
-        scope = Scope(cfg, state, None)
+        # the host preparation (process pools) runs before the kernel brings up the GPU runtime; the column
+        # integrals follow on the device
+        scope = Scope(cfg, state, None, device_columns=bool(cfg.get('device_columns', True)),
+                      device_geometry=bool(cfg.get('device_geometry', True)))
         scope.kernel = Kernel(backend=backend, L=int(cfg['L']), block_size=block_size, mixed=mixed, records=records)
+        scope.finish_on_device()
     else:
         atmosphere.check_profile_shapes(state, scope.state)
     wl = np.asarray(wavelengths, float)
@@ -169,3 +215,46 @@ def make_table(s, source, hs, el, az, group=64, batch=25, log=print, stokes=True
         if log:
             log(f'TABLE sun={hs[part[0]]:+g}..{hs[part[-1]]:+g} rays={db.rays} seconds={seconds[-1]:.3f}')
     return table, seconds
+
+
+UNITS = 'W m^-2 sr^-1 nm^-1'
+TABLE_AXES = ('solar_elevation_deg', 'view_elevation_deg', 'relative_azimuth_deg', 'wavelength_nm')
+
+
+def save_table(path, table, wl, hs, el, az):
+    """Write a table as the reference's table.npz (`src/sosfuse2/outputs/radiance_table.py:319-340`): `radiance`
+    f32 [solar elevation, view elevation, relative azimuth, wavelength] of total radiance in W m^-2 sr^-1 nm^-1, the
+    four axes as f32 (`wavelength_nm`, `solar_elevation_deg`, `view_elevation_deg`, `relative_azimuth_deg`) and
+    `units`. A table with Stokes components ([..., W, 3]) also stores `stokes_q` and `stokes_u` of the same shape,
+    which readers of the reference's format ignore. Written to a temporary name and renamed into place."""
+    table = np.asarray(table, 'f4')
+    wl, hs, el, az = (np.asarray(a, 'f8') for a in (wl, hs, el, az))
+    shape = (len(hs), len(el), len(az), len(wl))
+    extra = {}
+    if table.shape == shape + (3,):
+        extra = dict(stokes_q=np.ascontiguousarray(table[..., 1]), stokes_u=np.ascontiguousarray(table[..., 2]))
+        table = table[..., 0]
+    if table.shape != shape:
+        raise ValueError(f'E_TABLE_SHAPE: {table.shape} for axes {shape}')
+    path = Path(path)
+    tmp = path.with_name(path.name + '.tmp')
+    with tmp.open('wb') as f:
+        np.savez_compressed(f, radiance=np.ascontiguousarray(table), wavelength_nm=wl.astype('f4'),
+                            solar_elevation_deg=hs.astype('f4'), view_elevation_deg=el.astype('f4'),
+                            relative_azimuth_deg=az.astype('f4'), units=np.array(UNITS), **extra)
+    os.replace(tmp, path)
+    return path
+
+
+def read_table(path):
+    """((solar elevation, view elevation, relative azimuth, wavelength) axes, radiance) of a table.npz, with the
+    reference's checks (`radiance_table.py:291-300`): finite one-dimensional axes, radiance f32 of their shape,
+    finite, and the units."""
+    with np.load(path, allow_pickle=False) as archive:
+        axes = tuple(archive[k] for k in TABLE_AXES)
+        radiance = archive['radiance']
+        if (any(a.ndim != 1 or not a.size or a.dtype.kind not in 'fiu' or not np.isfinite(a).all() for a in axes)
+                or radiance.shape != tuple(len(a) for a in axes) or radiance.dtype != np.dtype('f4')
+                or not np.isfinite(radiance).all() or archive['units'].item() != UNITS):
+            raise ValueError('E_TABLE_ARRAY: invalid axes, radiance or units')
+        return axes, radiance

@@ -59,6 +59,24 @@ def pack(items, lanes=32, max_states=4):
     return chunks
 
 
+def solve_chunk(scope, items, el, az, s, tolerance=1e-5, max_eval=60, x0=None, graph=True):
+    """Solve one packed chunk (its context `s` holds the members' lanes in order) and query its table over the union
+    of the members' solar elevations: per member (table [h_s, el, az, W_i, 3] I/Q/U, info)."""
+    solver = scope.solver(s, log=None, graph=graph)
+    start = x0(s, solver) if x0 is not None else None
+    x, info = solver.solve(max_eval, tolerance, x0=start)
+    hs = sorted({h for it in items for h in it.elevations}, reverse=True)
+    table, _ = api.make_table(s, x, hs, el, az, log=None)
+    out, lane = [], 0
+    lanes = sum(len(it.wavelengths) for it in items)
+    for it in items:
+        w = len(it.wavelengths)
+        rows = [hs.index(h) for h in it.elevations]
+        out.append((table[rows][..., lane : lane + w, :], dict(info, lanes=lanes)))
+        lane += w
+    return out
+
+
 def solve(scope, items, el, az, tolerance=1e-5, max_eval=60, lanes=32, log=None, x0=None, graph=True):
     """Solve every item and return per item (table [h_s, el, az, W_i, 3] I/Q/U, info). `x0(s)` may give a warm
     start for a chunk context (the learned warm start); the iteration still runs to `tolerance`. graph: each chunk's
@@ -71,18 +89,10 @@ def solve(scope, items, el, az, tolerance=1e-5, max_eval=60, lanes=32, log=None,
         optics = atmosphere.concat_optics([atmosphere.lane_optics(items[i].state, items[i].wavelengths)
                                            for i in ch.members])
         s = scope.context(optics)
-        solver = scope.solver(s, log=None, graph=graph)
-        start = x0(s, solver) if x0 is not None else None
-        x, info = solver.solve(max_eval, tolerance, x0=start)
-        hs = sorted({h for i in ch.members for h in items[i].elevations}, reverse=True)
-        table, _ = api.make_table(s, x, hs, el, az, log=None)
-        lane = 0
-        for i in ch.members:
-            w = len(items[i].wavelengths)
-            rows = [hs.index(h) for h in items[i].elevations]
-            out[i] = (table[rows][..., lane : lane + w, :], dict(info, chunk=ch.members, lanes=ch.lanes))
-            lane += w
+        res = solve_chunk(scope, [items[i] for i in ch.members], el, az, s, tolerance, max_eval, x0, graph)
+        for i, r in zip(ch.members, res):
+            out[i] = (r[0], dict(r[1], chunk=ch.members))
         if log:
-            log(f'BATCH chunk states={ch.members} lanes={ch.lanes} evaluations={info["evaluations"]} '
-                f'elevations={len(hs)} seconds={time.perf_counter() - t0:.2f}')
+            log(f'BATCH chunk states={ch.members} lanes={ch.lanes} evaluations={res[0][1]["evaluations"]} '
+                f'seconds={time.perf_counter() - t0:.2f}')
     return out

@@ -57,7 +57,7 @@ int mix_launch(int64_t cells, int rows, int G, int depth, int m, const float *ri
                float *x, hipStream_t s);
 
 static bool lanes_ok(int G) { return G == 1 || G == 16 || G == 32; }
-static bool order_ok(int L) { return L == 8 || L == 16 || L == 32; }
+static bool order_ok(int L) { return L == 8 || L == 16 || L == 24 || L == 32; }
 
 int mlp_gemm_launch(const GemmArgs &in, float *work, hipStream_t s);
 int mlp_gelu_backward_launch(int64_t rows, int64_t cols, const float *dh, int64_t ldd, const float *z, int64_t ldz,
@@ -67,13 +67,121 @@ int mlp_mse_launch(int64_t rows, int64_t cols, const float *y, int64_t ldy, cons
 int mlp_bias_grad_launch(int64_t rows, int64_t cols, const float *dz, int64_t ld, float *db, hipStream_t s);
 int mlp_adam_launch(int64_t n, float *p, const float *g, float *m, float *v, int step, float lr, float b1, float b2,
                     float eps, hipStream_t s);
+int sun_table_launch(int nh, int nx, const double *h, const double *x, const ColumnArgs &c, double *out, hipStream_t s);
+int incident_launch(const IncidentArgs &a, int64_t *counts, const int64_t *off, double *dirs, double *weights,
+                    int32_t *ray_cell, hipStream_t s);
+int pack_launch(const PackArgs &a, int64_t rays, const int32_t *ray_cell, const double *dirs, int64_t *edge_count,
+                int64_t *sampled, const int64_t *edge_off, double *edges, int16_t *lower, int32_t *end_j, float *end_f,
+                uint8_t *hits, float *dloc, hipStream_t s);
+int ray_columns_launch(int64_t n, const double *starts, const int32_t *group, const double *dirs, const uint8_t *hit,
+                       const ColumnArgs &c, double *out, hipStream_t s);
+
 }  // namespace sos
 
 using namespace sos;
 
+// Column arguments from the C struct: interior shells by the top, profile kinds checked (every kind explicit).
+static int column_args(const sos_columns *c, ColumnArgs &a, const char *who) {
+    if (!c || !c->gl_u_dev || !c->gl_w_dev || c->nq < 1 || c->nq > 64 || !(c->top_km > 0))
+        return arg_error("%s: bad column arguments", who);
+    a.top = c->top_km;
+    a.shells = c->top_km > 80.0 ? 4 : 3;
+    const double sh[4] = {10.0, 25.0, 40.0, 80.0};
+    for (int i = 0; i < 4; ++i) a.shell_h[i] = sh[i];
+    a.nq = c->nq; a.gl_u = c->gl_u_dev; a.gl_w = c->gl_w_dev;
+    for (int k = 0; k < 6; ++k) {
+        const int kind = c->profile_kind[k];
+        if (kind < SOS_PROFILE_EXP || kind > SOS_PROFILE_ZERO) return arg_error("%s: profile kind %d of %d", who, kind, k);
+        if (kind == SOS_PROFILE_TABLE && (!c->profile_table_dev || c->profile_table_n < 2))
+            return arg_error("%s: profile table missing", who);
+        if (kind != SOS_PROFILE_ZERO && kind != SOS_PROFILE_AIR_SQUARED && !(c->profile_a[k] > 0))
+            return arg_error("%s: profile %d scale %g", who, k, c->profile_a[k]);
+        a.shape.kind[k] = kind;  // the device codes of geometry.hip follow the SOS_PROFILE_* values
+        a.shape.a[k] = c->profile_a[k];
+        a.shape.b[k] = c->profile_b[k];
+    }
+    a.shape.table = c->profile_table_dev;
+    a.shape.table_n = c->profile_table_n;
+    return SOS_OK;
+}
+
+
 extern "C" {
 
 const char *sos_last_error(void) { return g_error; }
+
+int sos_sun_table(const sos_columns *c, int32_t nh, int32_t nx, const double *h_dev, const double *x_dev,
+                  double *out_dev, void *stream) {
+    ColumnArgs a;
+    if (int rc = column_args(c, a, "sos_sun_table")) return rc;
+    if (nh < 2 || nx < 2 || !h_dev || !x_dev || !out_dev) return arg_error("sos_sun_table: bad table arguments");
+    return sun_table_launch(nh, nx, h_dev, x_dev, a, out_dev, (hipStream_t)stream);
+}
+
+static int incident_args(const sos_incident *a, IncidentArgs &o, const char *who) {
+    if (!a || a->nh < 1 || a->nc < 2 || a->nmu < 1 || a->nphi < 1 || a->nlevels < 0 || a->nlevels > 8 || !a->h_dev ||
+        !a->chi_dev || !a->gl_u_dev || !a->gl_w_dev)
+        return arg_error("%s: bad quadrature arguments", who);
+    o.nh = a->nh; o.nc = a->nc; o.nmu = a->nmu; o.nphi = a->nphi; o.nlevels = a->nlevels;
+    o.h = a->h_dev; o.chi = a->chi_dev; o.gl_u = a->gl_u_dev; o.gl_w = a->gl_w_dev;
+    for (int i = 0; i < 8; ++i) o.levels[i] = a->levels[i];
+    return SOS_OK;
+}
+
+int sos_incident_count(const sos_incident *a, int64_t *counts_dev, void *stream) {
+    IncidentArgs o;
+    if (int rc = incident_args(a, o, "sos_incident_count")) return rc;
+    if (!counts_dev) return arg_error("sos_incident_count: null argument");
+    return incident_launch(o, counts_dev, nullptr, nullptr, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int sos_incident_fill(const sos_incident *a, const int64_t *off_dev, double *dirs_dev, double *weights_dev,
+                      int32_t *ray_cell_dev, void *stream) {
+    IncidentArgs o;
+    if (int rc = incident_args(a, o, "sos_incident_fill")) return rc;
+    if (!off_dev || !dirs_dev || !weights_dev || !ray_cell_dev) return arg_error("sos_incident_fill: null argument");
+    return incident_launch(o, nullptr, off_dev, dirs_dev, weights_dev, ray_cell_dev, (hipStream_t)stream);
+}
+
+static int pack_args(const sos_columns *c, int32_t nc, const double *chi_dev, const double *starts_dev,
+                     const double *targets_dev, PackArgs &o, const char *who) {
+    if (int rc = column_args(c, o.c, who)) return rc;
+    if (nc < 2 || !chi_dev || !starts_dev) return arg_error("%s: bad grid arguments", who);
+    o.nc = nc; o.chi = chi_dev; o.starts = starts_dev; o.targets = targets_dev;
+    return SOS_OK;
+}
+
+int sos_pack_count(const sos_columns *c, int32_t nc, const double *chi_dev, const double *starts_dev, int64_t rays,
+                   const int32_t *ray_cell_dev, const double *dirs_dev, int64_t *edge_count_dev, int64_t *sampled_dev,
+                   void *stream) {
+    PackArgs o;
+    if (int rc = pack_args(c, nc, chi_dev, starts_dev, nullptr, o, "sos_pack_count")) return rc;
+    if (rays < 0 || (rays > 0 && (!ray_cell_dev || !dirs_dev || !edge_count_dev || !sampled_dev)))
+        return arg_error("sos_pack_count: null argument");
+    return pack_launch(o, rays, ray_cell_dev, dirs_dev, edge_count_dev, sampled_dev, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int sos_pack_fill(const sos_columns *c, int32_t nc, const double *chi_dev, const double *starts_dev,
+                  const double *targets_dev, int64_t rays, const int32_t *ray_cell_dev, const double *dirs_dev,
+                  const int64_t *edge_off_dev, double *edges_dev, int16_t *lower_dev, int32_t *end_j_dev,
+                  float *end_f_dev, uint8_t *hits_dev, float *dloc_dev, void *stream) {
+    PackArgs o;
+    if (int rc = pack_args(c, nc, chi_dev, starts_dev, targets_dev, o, "sos_pack_fill")) return rc;
+    if (rays < 0 || (rays > 0 && (!targets_dev || !ray_cell_dev || !dirs_dev || !edge_off_dev || !edges_dev ||
+                                  !lower_dev || !end_j_dev || !end_f_dev || !hits_dev || !dloc_dev)))
+        return arg_error("sos_pack_fill: null argument");
+    return pack_launch(o, rays, ray_cell_dev, dirs_dev, nullptr, nullptr, edge_off_dev, edges_dev, lower_dev, end_j_dev,
+                       end_f_dev, hits_dev, dloc_dev, (hipStream_t)stream);
+}
+
+int sos_ray_columns(const sos_columns *c, int64_t n, const double *starts_dev, const int32_t *group_dev,
+                    const double *dirs_dev, const uint8_t *hit_dev, double *out_dev, void *stream) {
+    ColumnArgs a;
+    if (int rc = column_args(c, a, "sos_ray_columns")) return rc;
+    if (n < 0 || (n > 0 && (!starts_dev || !dirs_dev || !out_dev))) return arg_error("sos_ray_columns: null argument");
+    return ray_columns_launch(n, starts_dev, group_dev, dirs_dev, hit_dev, a, out_dev, (hipStream_t)stream);
+}
 
 int sos_device_info(int32_t *cus, int32_t *xcds, char *arch, int32_t arch_len) {
     int dev = 0;

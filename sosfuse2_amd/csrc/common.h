@@ -103,6 +103,41 @@ __device__ inline void profiles(const ProfileShape &s, float h, float *p, float 
     st = profile_value(s, 5, h, air);
 }
 
+// FP64 profile kinds of the column integrals (geometry.hip): codes as SOS_PROFILE_* (1 exponential, 2 Gaussian,
+// 3 triangle, 4 table, 5 air squared, 6 zero), scales in km.
+struct ProfileShape64 {
+    int kind[6];
+    double a[6], b[6];      // scale (km) and centre, as the host's ProfileSpec
+    const double *table;    // [6][table_n] rows of the table kinds (FP64)
+    int table_n;
+};
+
+struct ColumnArgs {
+    double top;
+    int shells;              // interior shells: 3, or 4 with 80 km when top > 80
+    double shell_h[4];
+    int nq;
+    const double *gl_u, *gl_w;
+    ProfileShape64 shape;
+};
+
+// Incident quadrature of a grid
+struct IncidentArgs {
+    int nh, nc, nmu, nphi, nlevels;
+    const double *h, *chi;      // grid heights (km) and solar zenith nodes (rad)
+    double levels[8];           // ring levels: -sqrt((R + h_ring)^2 - R^2)
+    const double *gl_u, *gl_w;  // [nmu] Gauss-Legendre nodes and weights on [0, 1]
+};
+
+// Ray packs of a bundle
+struct PackArgs {
+    int nc;
+    const double *chi;       // [nc]
+    const double *starts;    // [cells][3]
+    const double *targets;   // [cells][3][3] frame vectors as columns
+    ColumnArgs c;            // top and shells of the boundaries
+};
+
 struct TransportArgs {
     // grid
     int nh, nc, nq, strato, flags, pairs, rows, sun_nh, sun_nx;

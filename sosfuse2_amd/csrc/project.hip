@@ -180,6 +180,7 @@ static int project_L(const ProjectArgs &a, int L, hipStream_t s) {
     switch (L) {
         case 8: return project_launch<G, 8>(a, s);
         case 16: return project_launch<G, 16>(a, s);
+        case 24: return project_launch<G, 24>(a, s);
         case 32: return project_launch<G, 32>(a, s);
         default: return set_error("unsupported L %d", L);
     }

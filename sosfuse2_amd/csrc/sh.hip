@@ -172,6 +172,7 @@ int plan_basis_launch(int L, int64_t rays, const double *dirs, float *basis, hip
     switch (L) {
         case 8: hipLaunchKernelGGL(plan_basis_kernel<8>, grid, dim3(256), 0, s, rays, dirs, basis); break;
         case 16: hipLaunchKernelGGL(plan_basis_kernel<16>, grid, dim3(256), 0, s, rays, dirs, basis); break;
+        case 24: hipLaunchKernelGGL(plan_basis_kernel<24>, grid, dim3(256), 0, s, rays, dirs, basis); break;
         case 32: hipLaunchKernelGGL(plan_basis_kernel<32>, grid, dim3(256), 0, s, rays, dirs, basis); break;
         default: return set_error("plan_basis: L %d", L);
     }
@@ -1046,6 +1047,7 @@ static int sh_L(const ShArgs &a, bool half, hipStream_t s) {
     switch (a.L) {
         case 8: return sh_launch<G, 8>(a, half, s);
         case 16: return sh_launch<G, 16>(a, half, s);
+        case 24: return sh_launch<G, 24>(a, half, s);
         case 32: return sh_launch<G, 32>(a, half, s);
         default: return set_error("sh_records: L %d", a.L);
     }

@@ -1041,8 +1041,9 @@ static int launch_flags(const TransportArgs &a, int L, hipStream_t s) {
     switch (L) {
         case 8: SOS_DIRECT(8)
         case 16: SOS_DIRECT(16)
+        case 24: SOS_DIRECT(24)
         case 32: SOS_DIRECT(32)
-        default: return set_error("unsupported L %d (8, 16, 32)", L);
+        default: return set_error("unsupported L %d (8, 16, 24, 32)", L);
     }
 #undef SOS_DIRECT
 }

@@ -292,6 +292,13 @@ def sun_table(state, domain, scale=1, chunk=4096):
     return SunTable(h, x, col)
 
 
+def lit_flag(q, r):
+    """Whether points q at radius r see the sun: their cosine to the sun axis reaches the horizon's (sun_lookup's lit
+    flag without the table)."""
+    mg = -np.sqrt(np.maximum(0, 1 - (R / np.maximum(r, R)) ** 2))
+    return q[..., 2] / r >= mg
+
+
 def sun_lookup(table, q, r, top_km):
     """Bilinear sun columns and the lit flag at points q (`shell_segments.py:309-325`)."""
     h = np.clip(r - R, 0, top_km)
@@ -448,6 +455,7 @@ class GroundPools:
     loc: np.ndarray  # [P, 3]
     factor: np.ndarray  # [P]
     column: np.ndarray  # [P, 6]
+    path: tuple = None  # (starts, dirs, ground points) [P, 3] while the columns are left to the device
 
     def values(self, optics):
         """gv[P, W] = albedo * factor * exp(-column . ext)."""
@@ -487,9 +495,11 @@ class GroundPools:
         return gd, gvp
 
 
-def _pools_of_cells(state, grid, sun, cells, qscale):
+def _pools_of_cells(state, grid, sun, cells, qscale, columns_too=True):
+    """Pools of some cells; columns_too=False leaves out the path columns (cell to ground point) and returns the rays
+    they follow instead, for device_geometry.pool_columns."""
     domain = grid.domain
-    locs, facs, cols, counts = [], [], [], []
+    locs, facs, cols, counts, paths = [], [], [], [], []
     for cell in cells:
         ih, ic = divmod(cell, grid.nc)
         r = R + grid.h[ih]
@@ -503,25 +513,33 @@ def _pools_of_cells(state, grid, sun, cells, qscale):
         q = pp + end[:, None] * d
         rq = np.linalg.norm(q, axis=-1)
         hit = rq < R + 1e-5
-        sc, lit = sun_lookup(sun, q, np.maximum(R, rq), domain.top_km)
+        if columns_too:
+            sc, lit = sun_lookup(sun, q, np.maximum(R, rq), domain.top_km)
+        else:
+            lit = lit_flag(q, np.maximum(R, rq))
         ch = np.arccos(np.clip(q[:, 2] / rq, -1, 1))
         factor = np.maximum(0, np.cos(ch)) / math.pi * lit * hit * w
         take = np.flatnonzero(factor > 0)
         col = np.zeros((len(take), 6))
         if len(take):
-            col = columns(state, pp[take], d[take], e[take], 24, domain, strato=True) + sc[take]
+            if columns_too:
+                col = columns(state, pp[take], d[take], e[take], 24, domain, strato=True) + sc[take]
+            else:
+                paths.append((pp[take], d[take], q[take]))
         locs.append(loc[take])
         facs.append(factor[take])
         cols.append(col)
         counts.append(len(take))
-    return locs, facs, cols, counts
+    return locs, facs, cols, counts, paths
 
 
 def _cells_job(args):
-    kind, state, grid, sun, cells, qscale = args
+    kind, state, grid, sun, cells, qscale, columns_too = args
     if kind == 'pools':
-        return _pools_of_cells(state, grid, sun, cells, qscale)
-    return [with_end_columns(state, cell_pack(grid, c), grid.domain) for c in cells]
+        return _pools_of_cells(state, grid, sun, cells, qscale, columns_too)
+    if columns_too:
+        return [with_end_columns(state, cell_pack(grid, c), grid.domain) for c in cells]
+    return [cell_pack(grid, c) for c in cells]
 
 
 def gpu_runtime_started():
@@ -537,17 +555,17 @@ def gpu_runtime_started():
     return bool(torch is not None and torch.cuda.is_initialized())
 
 
-def _parallel(kind, state, grid, sun, workers, qscale=1):
+def _parallel(kind, state, grid, sun, workers, qscale=1, columns_too=True):
     """Run a per-cell preparation over a spawn process pool (numpy only in the workers), in cell order; a thread
     pool once the process has started the GPU runtime (gpu_runtime_started)."""
     cells = list(range(grid.ns))
     if workers <= 1 or grid.ns < 64:
-        return [_cells_job((kind, state, grid, sun, cells, qscale))]
+        return [_cells_job((kind, state, grid, sun, cells, qscale, columns_too))]
     from concurrent.futures import ProcessPoolExecutor, ThreadPoolExecutor
     import multiprocessing as mp
 
     chunk = max(1, (grid.ns + 4 * workers - 1) // (4 * workers))
-    jobs = [(kind, state, grid, sun, cells[i : i + chunk], qscale) for i in range(0, grid.ns, chunk)]
+    jobs = [(kind, state, grid, sun, cells[i : i + chunk], qscale, columns_too) for i in range(0, grid.ns, chunk)]
     if gpu_runtime_started():
         pool = ThreadPoolExecutor(workers)
     else:
@@ -566,26 +584,33 @@ def default_workers():
     return max(1, min(16, n))
 
 
-def ground_pools(state, grid, sun, qscale=1, workers=1):
-    """Build the ground pools of every cell (`engine_core.py:103-147`)."""
-    parts = _parallel('pools', state, grid, sun, workers, qscale)
-    locs, facs, cols, counts = [], [], [], []
-    for a, b, c, d in parts:
+def ground_pools(state, grid, sun, qscale=1, workers=1, columns_too=True):
+    """Build the ground pools of every cell (`engine_core.py:103-147`). columns_too=False: the columns (path to the
+    ground and the sun table at the ground point) are left for device_geometry.pool_columns, and `sun` may be None."""
+    parts = _parallel('pools', state, grid, sun, workers, qscale, columns_too)
+    locs, facs, cols, counts, paths = [], [], [], [], []
+    for a, b, c, d, e in parts:
         locs += a
         facs += b
         cols += c
         counts += d
+        paths += e
+    path = None
+    if not columns_too:
+        path = tuple(np.concatenate([x[i] for x in paths]) if paths else np.zeros((0, 3)) for i in range(3))
     return GroundPools(
         off=np.r_[0, np.cumsum(counts)].astype('i8'),
         loc=np.concatenate(locs) if locs else np.zeros((0, 3)),
         factor=np.concatenate(facs) if facs else np.zeros(0),
         column=np.concatenate(cols) if cols else np.zeros((0, 6)),
+        path=path,
     )
 
 
-def cell_packs(state, grid, workers=1):
-    """Incident-ray packs with end columns of every cell, in cell order."""
-    return [p for part in _parallel('packs', state, grid, None, workers) for p in part]
+def cell_packs(state, grid, workers=1, end_columns=True):
+    """Incident-ray packs of every cell, in cell order, with their end columns (end_columns=False: left for the
+    device, device_geometry.end_columns)."""
+    return [p for part in _parallel('packs', state, grid, None, workers, 1, end_columns) for p in part]
 
 
 # ------------------------------------------------------------------------------------------------------- bundles

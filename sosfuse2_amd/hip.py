@@ -36,7 +36,28 @@ class Grid(ctypes.Structure):
     ]
 
 
+class Columns(ctypes.Structure):
+    _fields_ = [
+        ('top_km', ctypes.c_double), ('nq', c_i32), ('gl_u_dev', P), ('gl_w_dev', P), ('profile_kind', c_i32 * 6),
+        ('profile_a', ctypes.c_double * 6), ('profile_b', ctypes.c_double * 6), ('profile_table_dev', P),
+        ('profile_table_n', c_i32),
+    ]
+
+
+class Incident(ctypes.Structure):
+    _fields_ = [
+        ('nh', c_i32), ('nc', c_i32), ('nmu', c_i32), ('nphi', c_i32), ('nlevels', c_i32), ('h_dev', P),
+        ('chi_dev', P), ('levels', ctypes.c_double * 8), ('gl_u_dev', P), ('gl_w_dev', P),
+    ]
+
+
 _SIGNATURES = {
+    'sos_incident_count': [ctypes.POINTER(Incident), P, P],
+    'sos_incident_fill': [ctypes.POINTER(Incident), P, P, P, P, P],
+    'sos_pack_count': [ctypes.POINTER(Columns), c_i32, P, P, c_i64, P, P, P, P, P],
+    'sos_pack_fill': [ctypes.POINTER(Columns), c_i32, P, P, P, c_i64, P, P, P, P, P, P, P, P, P, P],
+    'sos_sun_table': [ctypes.POINTER(Columns), c_i32, c_i32, P, P, P, P],
+    'sos_ray_columns': [ctypes.POINTER(Columns), c_i64, P, P, P, P, P, P],
     'sos_transport': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), c_i32, c_i32, c_i32, P, P, P, P, P, P, P],
     'sos_plan_count': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, P],
     'sos_plan_nodes': [ctypes.POINTER(Grid), ctypes.POINTER(Bundle), P, P, P, P, P, P],

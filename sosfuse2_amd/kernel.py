@@ -64,6 +64,25 @@ class DeviceBundle:
                 self.edge_off, self.edges, self.lower, self.end_j, self.end_f, self.end_col, self.hits)), None, None,
             None)
 
+    @classmethod
+    def from_tensors(cls, groups, rays, nq, samples, t):
+        """A device bundle from arrays already in HBM (device_geometry builds them there): t holds group_off i8,
+        starts f8 [groups, 3], targets f4 [groups, 9], ray_group i4, dirs f8 [rays, 3], dloc f4 [rays, 3], weights
+        f4, edge_off i8 [rays + 1], edges f8, lower i2, end_j i4, end_f f4, end_col f4 [rays, 6], hits u1."""
+        self = cls.__new__(cls)
+        self.host = None
+        self.groups, self.rays, self.nq, self.samples = int(groups), int(rays), int(nq), int(samples)
+        for k in ('group_off', 'starts', 'targets', 'ray_group', 'dirs', 'dloc', 'weights', 'edge_off', 'edges', 'lower',
+                  'end_j', 'end_f', 'end_col', 'hits'):
+            setattr(self, k, t[k].contiguous())
+        self.plan = None
+        self.struct = hip.Bundle(
+            self.groups, self.rays, *(hip.ptr(x) for x in (
+                self.group_off, self.starts, self.targets, self.ray_group, self.dirs, self.dloc, self.weights,
+                self.edge_off, self.edges, self.lower, self.end_j, self.end_f, self.end_col, self.hits)), None, None,
+            None)
+        return self
+
     def nbytes(self):
         return sum(x.numel() * x.element_size() for x in (
             self.group_off, self.starts, self.targets, self.ray_group, self.dirs, self.dloc, self.weights,
@@ -299,8 +318,6 @@ class Kernel:
         if backend != 'hip':
             raise ValueError(f"E_BACKEND: backend {backend!r}; this engine runs 'hip' only")
         layout.count(L)
-        if L == 24:
-            raise ValueError('E_SH_L: the HIP kernels are built for L = 8, 16 and 32')
         hip.mark_started()
         if not torch.cuda.is_available():
             raise RuntimeError('E_HIP_DEVICE: no GPU visible to torch')

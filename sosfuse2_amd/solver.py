@@ -17,12 +17,19 @@ from .kernel import lane_count
 
 
 class Geometry:
-    """Cell packs of one grid and their device bundles (the reference's TransientGeometry)."""
+    """Cell packs of one grid and their device bundles (the reference's TransientGeometry). bundles: the (warm,
+    ground) device bundles when they were built on the device (device_geometry.cell_bundles); no host packs then."""
 
-    def __init__(self, s, packs=None, kernel=None):
+    def __init__(self, s, packs=None, kernel=None, bundles=None):
         self.s = s
         k = kernel or s.kernel
         t = time.perf_counter()
+        if bundles is not None:
+            self.packs = None
+            self.bundle, self.ground_bundle = bundles
+            self.stats = dict(cells=s.geo.ns, rays=self.bundle.rays, samples=self.bundle.samples,
+                              seconds=time.perf_counter() - t, device_bytes=self.bundle.nbytes())
+            return
         if packs is None:
             packs = [geometry.with_end_columns(s.state, geometry.cell_pack(s.geo, i), s.geo.domain)
                      for i in range(s.geo.ns)]

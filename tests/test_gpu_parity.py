@@ -48,10 +48,10 @@ def physical_field(s, seed):
     return f
 
 
-def setup_case(W, L=16, nr=5, state=None, nmu=1, nphi=2, path_n=4, records=True, mixed=False):
+def setup_case(W, L=16, nr=5, state=None, nmu=1, nphi=2, path_n=4, records=True, mixed=False, top_km=100.0):
     from oracle.solve import OracleSolver
 
-    cfg = dict(nr=nr, chi_grid='tiny', nmu=nmu, nphi=nphi, path_n=path_n, L=L, top_km=100.0, observer_km=0.1,
+    cfg = dict(nr=nr, chi_grid='tiny', nmu=nmu, nphi=nphi, path_n=path_n, L=L, top_km=top_km, observer_km=0.1,
                query_n=6, anderson_depth=4)
     wl = atmosphere.log_spaced_wavelengths(W, 420, 680) if W > 1 else [550.0]
     st = state or atmosphere.config0_state()
@@ -60,8 +60,9 @@ def setup_case(W, L=16, nr=5, state=None, nmu=1, nphi=2, path_n=4, records=True,
     return s, scope, orc
 
 
-@pytest.fixture(scope='module', params=[(1, 16, True), (3, 16, True), (20, 8, True), (3, 16, False), (20, 32, True)],
-                ids=['W1-G1', 'W3-G16', 'W20-G32', 'W3-G16-direct', 'W20-G32-L32'])
+@pytest.fixture(scope='module', params=[(1, 16, True), (3, 16, True), (20, 8, True), (3, 16, False), (20, 32, True),
+                                        (5, 24, True), (3, 24, False)],
+                ids=['W1-G1', 'W3-G16', 'W20-G32', 'W3-G16-direct', 'W20-G32-L32', 'W5-G16-L24', 'W3-G16-L24-direct'])
 def case(request, gpu):
     W, L, records = request.param
     print(f'case W={W} L={L} records={records}', flush=True)
@@ -174,7 +175,7 @@ def test_ground_pools_on_device_match_host(gpu):
     assert np.all(np.abs(gm_dev - gm_host) <= 1e-6 * np.abs(gm_host).max((0, 1)) + 1e-30)
 
 
-@pytest.mark.parametrize('L', [8, 32])
+@pytest.mark.parametrize('L', [8, 24, 32])
 def test_basis_table_matches_oracle_basis(gpu, L):
     """sos_plan_basis against the oracle's FP64 basis (oracle/arith.py basis, the recurrence of arithmetic.py:86-109)
     at each ray's sun-axis direction cosine, reordered to the m-major K order: every entry within two FP32 roundings
@@ -200,7 +201,7 @@ def test_basis_table_matches_oracle_basis(gpu, L):
     assert np.all(err <= 2.5e-7 * np.abs(ref) + 1e-30), err.max()
 
 
-@pytest.mark.parametrize('W,L', [(3, 16), (20, 32), (1, 16)])
+@pytest.mark.parametrize('W,L', [(3, 16), (20, 32), (1, 16), (5, 24)])
 def test_mixed_records_match_fp32(gpu, W, L):
     """Split-FP16 SH records (three FP16 MFMA products, FP32 accumulation) against the FP32 records of the same
     field. Operands carry 22 bits (hi + lo 2^-11) and both are scaled into the top of FP16's range (small entries stay
@@ -464,10 +465,13 @@ def test_batched_solve_matches_per_state_solves(gpu):
     alone may run on another lane count, G = 1 or 16, whose kernels sum in another order)."""
     from sosfuse2_amd import batch
 
-    s0, scope, _ = setup_case(3, 16)
-    # the 5-level test grid resolves the 1.2 km aerosol layer with one 2.3 km cell: states of large optical depth
-    # stall there (the FP32 oracle stalls the same way), so the test keeps AOD < 0.2
-    items = [it for it in batch.items_from_seed(16, seed=4, top_km=100.0, all_elevations=[-2.0, -4.0, -6.0],
+    from dataclasses import replace
+
+    # the reference's default 80 km top (five stop monitors): on this coarse 5-level grid the deep-twilight monitor a
+    # higher top adds (-15 degrees) makes some states stall; the 5 levels also resolve the 1.2 km aerosol layer with
+    # one 2.3 km cell, where states of large optical depth stall (the FP32 oracle the same way): AOD < 0.2
+    s0, scope, _ = setup_case(3, 16, state=replace(atmosphere.config0_state(), top_km=80.0), top_km=80.0)
+    items = [it for it in batch.items_from_seed(16, seed=4, top_km=80.0, all_elevations=[-2.0, -4.0, -6.0],
                                                 max_wavelengths=9, max_elevations=3) if it.state.aod550 < 0.2][:5]
     assert len(items) == 5
     el = 90 - (np.arange(3) + 0.5) * 30
@@ -480,6 +484,41 @@ def test_batched_solve_matches_per_state_solves(gpu):
         alone, _ = api.make_table(s, x, it.elevations, el, az, log=None)
         assert table.shape == alone.shape
         assert np.max(np.abs(table[..., 0] - alone[..., 0]) / np.abs(alone[..., 0])) < 2e-5
+
+
+def test_state_queue_matches_per_state_solves(gpu, tmp_path):
+    """The state queue fed from two threads: every state's table equals the state solved alone (to the iteration
+    tolerance) and is written as table.npz."""
+    import threading
+    from dataclasses import replace
+
+    from sosfuse2_amd import batch, state_queue
+
+    s0, scope, _ = setup_case(3, 16, state=replace(atmosphere.config0_state(), top_km=80.0), top_km=80.0)
+    items = [it for it in batch.items_from_seed(16, seed=5, top_km=80.0, all_elevations=[-2.0, -4.0, -6.0],
+                                                max_wavelengths=9, max_elevations=3) if it.state.aod550 < 0.2][:6]
+    el = 90 - (np.arange(3) + 0.5) * 30
+    az = (np.arange(4) + 0.5) * 45
+    jobs = [None] * len(items)
+    with state_queue.StateQueue(scope, el, az, tolerance=1e-5, max_eval=40, lanes=16, out_dir=tmp_path) as q:
+        def client(part):
+            for i in part:
+                jobs[i] = q.submit(items[i].state, items[i].wavelengths, items[i].elevations, name=f'state{i}')
+
+        threads = [threading.Thread(target=client, args=(range(k, len(items), 2),)) for k in (0, 1)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+    assert q.stats['chunks'] < len(items) and q.stats['failed'] == 0
+    for it, job in zip(items, jobs):
+        table, info = job.wait(0)
+        s = scope.context(atmosphere.lane_optics(it.state, it.wavelengths))
+        x, _ = scope.solver(s, log=None).solve(40, 1e-5)
+        alone, _ = api.make_table(s, x, it.elevations, el, az, log=None)
+        assert np.max(np.abs(table[..., 0] - alone[..., 0]) / np.abs(alone[..., 0])) < 2e-5
+        _, radiance = api.read_table(job.path)
+        np.testing.assert_array_equal(radiance, table[..., 0])
 
 
 def test_graph_replay_matches_eager(gpu):
@@ -520,7 +559,7 @@ def test_autotuner_keeps_results(gpu):
     assert again == best and t2 == {}
 
 
-@pytest.mark.parametrize('W,L', [(20, 32), (3, 16), (5, 8)])
+@pytest.mark.parametrize('W,L', [(20, 32), (3, 16), (5, 8), (20, 24)])
 def test_sh_record_forms_are_bitwise_equal(gpu, W, L):
     """The three FP32 SH-records workgroup forms (flow, stepwise and quad) and three work-item sizes give the same
     records bit for bit (same basis arithmetic, same MFMA accumulation order): L = 32 on 32 lanes, L = 16 and L = 8

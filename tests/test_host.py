@@ -74,3 +74,97 @@ def test_host_preparation_uses_threads_once_the_gpu_runtime_is_up(monkeypatch):
     for a, b in zip(threaded, serial):
         np.testing.assert_array_equal(a['d'], b['d'])
         np.testing.assert_array_equal(a['end_columns'], b['end_columns'])
+
+
+def test_table_npz_has_the_reference_layout(tmp_path):
+    """api.save_table writes the reference's table.npz keys, dtypes and axis order; read_table (the reference's
+    checks) reads it back; Stokes Q and U ride along as extra arrays."""
+    wl, hs, el, az = api.axes(api.CONFIG0)
+    rng = np.random.default_rng(0)
+    table = rng.uniform(1e-6, 1e-3, size=(len(hs), len(el), len(az), len(wl), 3)).astype('f4')
+    path = api.save_table(tmp_path / 'table.npz', table, wl, hs, el, az)
+    with np.load(path, allow_pickle=False) as f:
+        assert set(f.files) == {'radiance', 'wavelength_nm', 'solar_elevation_deg', 'view_elevation_deg',
+                                'relative_azimuth_deg', 'units', 'stokes_q', 'stokes_u'}
+        assert f['radiance'].dtype == np.float32 and f['radiance'].shape == (2, 4, 6, 1)
+        assert all(f[k].dtype == np.float32 for k in api.TABLE_AXES)
+        assert f['units'].item() == 'W m^-2 sr^-1 nm^-1'
+        np.testing.assert_array_equal(f['stokes_q'], table[..., 1])
+    axes, radiance = api.read_table(path)
+    np.testing.assert_array_equal(radiance, table[..., 0])
+    np.testing.assert_array_equal(axes[1], np.float32(el))
+    np.testing.assert_array_equal(axes[0], [-3, -6])
+    np.testing.assert_array_equal(axes[1], [78.75, 56.25, 33.75, 11.25])  # the reference's reduced view rows
+    np.testing.assert_array_equal(axes[2], [15, 45, 75, 105, 135, 165])
+    with pytest.raises(ValueError, match='E_TABLE_SHAPE'):
+        api.save_table(tmp_path / 'bad.npz', table[:, :2], wl, hs, el, az)
+    assert not (tmp_path / 'table.npz.tmp').exists()
+
+
+def test_state_queue_packs_prepares_and_fails_per_chunk(tmp_path):
+    """The queue's host logic with a stand-in GPU step: states submitted from several threads are packed into
+    chunks of at most 32 lanes, every job gets its own lanes' table back, tables are written as table.npz, a chunk
+    whose solve fails fails its own jobs only, and a state of other profile shapes is refused at submit."""
+    import threading
+    from dataclasses import replace
+    from types import SimpleNamespace
+
+    from sosfuse2_amd import state_queue
+
+    base = atmosphere.config1_state()
+    scope = SimpleNamespace(state=base, context=lambda optics: SimpleNamespace(optics=optics))
+    el, az = np.array([45.0, 15.0]), np.array([30.0, 90.0, 150.0])
+    seen = []
+
+    def fake_solve(scope_, items, el_, az_, ctx):
+        seen.append(ctx.optics.W)
+        if any(it.state.albedo == 0.99 for it in items):
+            raise RuntimeError('E_NOT_CONVERGED: stand-in failure')
+        out = []
+        lane = 0
+        for it in items:
+            w = len(it.wavelengths)
+            t = np.zeros((len(it.elevations), len(el_), len(az_), w, 3), 'f4')
+            t[..., 0] = np.arange(lane, lane + w)[None, None, None, :] + 1
+            out.append((t, dict(evaluations=7)))
+            lane += w
+        return out
+
+    q = state_queue.StateQueue(scope, el, az, out_dir=tmp_path, linger=0.05, solve_chunk=fake_solve)
+    jobs = []
+    lock = threading.Lock()
+
+    def client(k):
+        for i in range(4):
+            st = replace(base, name=f'c{k}_{i}', albedo=0.99 if (k, i) == (1, 2) else 0.1)
+            j = q.submit(st, atmosphere.log_spaced_wavelengths(3 + 5 * i), [2.0, -3.0, -6.0][: 1 + i % 3])
+            with lock:
+                jobs.append(j)
+
+    threads = [threading.Thread(target=client, args=(k,)) for k in range(3)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    with pytest.raises(ValueError, match='E_BATCH_PROFILE'):
+        q.submit(replace(base, aerosol_height_km=2.0), [550.0], [-3.0])
+    q.close()
+    assert all(w <= 32 for w in seen) and len(seen) < len(jobs)
+    failed = [j for j in jobs if j.error is not None]
+    assert failed and all('E_NOT_CONVERGED' in str(j.error) for j in failed)
+    assert any(j.item.state.albedo == 0.99 for j in failed)
+    for j in jobs:
+        if j.error is not None:
+            with pytest.raises(RuntimeError):
+                j.wait(1)
+            continue
+        table, info = j.wait(1)
+        w = len(j.item.wavelengths)
+        assert table.shape == (len(j.item.elevations), 2, 3, w, 3) and info['evaluations'] == 7
+        lanes = table[0, 0, 0, :, 0]
+        assert np.all(np.diff(lanes) == 1)  # its own contiguous lanes of the chunk
+        axes, radiance = api.read_table(j.path)
+        np.testing.assert_array_equal(radiance, table[..., 0])
+    assert q.stats['states'] + q.stats['failed'] == len(jobs)
+    with pytest.raises(RuntimeError, match='E_QUEUE_CLOSED'):
+        q.submit(base, [550.0], [-3.0])
