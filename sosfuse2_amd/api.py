@@ -75,7 +75,7 @@ class Scope:
             if not device_geometry:
                 self._packs = geometry.cell_packs(profile_state, self.grid, workers=self.workers, end_columns=False)
         else:
-            self.sun = geometry.sun_table(profile_state, self.grid.domain, int(cfg.get('sun_scale', 1)))
+            self.sun = geometry.sun_table(profile_state, self.grid.domain, int(cfg.get('sun_scale', 1)), self.workers)
             self.pools = geometry.ground_pools(profile_state, self.grid, self.sun, workers=self.workers)
             self._packs = geometry.cell_packs(profile_state, self.grid, workers=self.workers)
         self.geometry = None
@@ -120,9 +120,10 @@ class Scope:
 
 
 def create_configured(output, cfg, wavelengths, backend, state, block_size=256, geometry_cache=None, *,
-                      share_sun_geometry=True, sun_scale=1, scope=None, mixed=False, records=True):
+                      share_sun_geometry=True, sun_scale=1, scope=None, mixed=False, records=True, optics=None):
     """Solver context `s` over `wavelengths` (nm) for one state, and its scope (`solve/build.py:14-118`).
-    `scope` reuses another state's preparation on the same grid and profile shapes."""
+    `scope` reuses another state's preparation on the same grid and profile shapes. `optics` (an atmosphere.Optics
+    over the same wavelengths, e.g. atmosphere.optics_from_coefficients) replaces the state's own lane optics."""
     if output is not None:
         Path(output).mkdir(parents=True, exist_ok=True)
     if scope is None:
@@ -140,7 +141,10 @@ def create_configured(output, cfg, wavelengths, backend, state, block_size=256, 
     wl = np.asarray(wavelengths, float)
     if wl.ndim != 1 or len(wl) < 1:
         raise ValueError('E_WAVELENGTHS: a one-dimensional wavelength list is required')
-    optics = atmosphere.lane_optics(state, wl)
+    if optics is None:
+        optics = atmosphere.lane_optics(state, wl)
+    elif not np.array_equal(np.asarray(optics.wavelengths, float), wl):
+        raise ValueError('E_WAVELENGTHS: the optics are over other wavelengths')
     lane_count(optics.W)
     s = scope.context(optics)
     return s, scope

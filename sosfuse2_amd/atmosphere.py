@@ -361,3 +361,44 @@ def log_spaced_wavelengths(n, lo=400.0, hi=700.0):
 
 def with_albedo(state, albedo):
     return replace(state, albedo=albedo)
+
+
+def legacy_state(parameters, no2_profile, step_km=0.5, depolarisation=0.03, name='legacy'):
+    """A state of the reference's legacy density (`src/sosfuse2/atmosphere/density.py:92,497-513`): air exp(-h/8),
+    aerosol exp(-h/H) with H = parameters['scale_height_km'], the triangle ozone of half width 15 km about 25 km, NO2
+    tabulated on a uniform grid of step_km from height 0 (no2_profile, maximum 1), O4 as the squared air profile, no
+    sulfate layer, top 80 km. `parameters` are the reference state's (aod550, angstrom, ssa, scale_height_km, g,
+    ozone_du, albedo); the coefficients per km come from the caller (optics_from_coefficients)."""
+    no2 = np.asarray(no2_profile, 'f8')
+    if no2.ndim != 1 or len(no2) < 2 or not np.all(np.isfinite(no2)) or np.any(no2 < 0):
+        raise ValueError('E_PROFILE_TABLE: a finite non-negative NO2 profile of at least two nodes is required')
+    if (len(no2) - 1) * step_km < 80.0:
+        raise ValueError('E_PROFILE_TABLE: the NO2 profile must reach 80 km')
+    rows = np.zeros((6, len(no2)))
+    rows[3] = no2
+    p = parameters
+    H = float(p['scale_height_km'])
+    spec = ProfileSpec(kinds=(EXP, EXP, TRIANGLE, TABLE, AIR_SQUARED, ZERO), a=(8.0, H, 15.0, float(step_km), 1.0, 1.0),
+                       b=(0.0, 0.0, 25.0, 0.0, 0.0, 0.0), table=tuple(tuple(r) for r in rows))
+    return State(name=name, top_km=80.0, depolarisation=depolarisation, aod550=float(p['aod550']),
+                 aerosol_height_km=H, aerosol_g=float(p['g']), aerosol_ssa=float(p['ssa']),
+                 angstrom=float(p['angstrom']), ozone_du=float(p['ozone_du']), albedo=float(p['albedo']),
+                 profile_spec=spec)
+
+
+def optics_from_coefficients(state, wl, ext, asca, solar):
+    """Optics of one state from coefficients the caller computed: ext[5, W] per km at unit profile value (Rayleigh,
+    aerosol, O3, NO2, O4 in the order of the reference's `extinction.coefficients`,
+    `src/sosfuse2/atmosphere/extinction.py:25-33`), asca[W] the aerosol scattering per km and solar[W] the solar
+    flux (`src/sosfuse2/solve/engine_core.py:86-92`). g, albedo and the depolarisation come from the state."""
+    wl = np.atleast_1d(np.asarray(wl, 'f8'))
+    W = len(wl)
+    ext = np.asarray(ext, 'f8').reshape(NCOMP, W)
+    asca = np.broadcast_to(np.asarray(asca, 'f8'), (W,)).copy()
+    solar = np.broadcast_to(np.asarray(solar, 'f8'), (W,)).copy()
+    if not (np.all(np.isfinite(ext)) and np.all(ext >= 0) and np.all(asca >= 0) and np.all(asca <= ext[1] * (1 + 1e-12))):
+        raise ValueError('E_OPTICS: finite non-negative coefficients with asca <= aerosol extinction required')
+    zero = np.zeros(W)
+    return Optics(ext=ext, asca=asca, g=np.full(W, state.aerosol_g), albedo=np.full(W, state.albedo),
+                  mix=np.full(W, state.mix), solar=solar, wavelengths=wl, strato_ext=zero, strato_sca=zero.copy(),
+                  strato_g=np.full(W, state.sulfate_g), lane_state=np.zeros(W, 'i4'), states=[state])

@@ -275,11 +275,10 @@ class SunTable:
     col: np.ndarray  # [nh, nx, 6]: five components and the sulfate layer
 
 
-def sun_table(state, domain, scale=1, chunk=4096):
-    nh = 80 * scale + 1
-    h = np.linspace(0, math.sqrt(domain.top_km), nh) ** 2
-    x = np.linspace(0, 1, 400 * scale + 1)
-    col = np.empty((nh, len(x), 6))
+def _sun_rows(args):
+    """Sun-table rows of the heights h (one job of sun_table)."""
+    state, domain, h, x, scale = args
+    out = np.empty((len(h), len(x), 6))
     for i, hh in enumerate(h):
         r = R + hh
         mg = -math.sqrt(max(0.0, 1 - (R / r) ** 2))
@@ -288,8 +287,27 @@ def sun_table(state, domain, scale=1, chunk=4096):
         p = np.column_stack([np.zeros(len(mu)), np.zeros(len(mu)), np.full(len(mu), r)])
         d = np.column_stack([np.sqrt(np.maximum(1 - mu * mu, 0)), np.zeros(len(mu)), mu])
         e = boundaries(p, d, domain)
-        col[i] = columns(state, p, d, e, 32 * scale, domain, strato=True)
-    return SunTable(h, x, col)
+        out[i] = columns(state, p, d, e, 32 * scale, domain, strato=True)
+    return out
+
+
+def sun_table(state, domain, scale=1, workers=1):
+    """Sun-column table over heights (square-root spaced) and the horizon-relative cosine x; rows split over a
+    process pool (a thread pool once the GPU runtime has started) when workers > 1."""
+    nh = 80 * scale + 1
+    h = np.linspace(0, math.sqrt(domain.top_km), nh) ** 2
+    x = np.linspace(0, 1, 400 * scale + 1)
+    if workers <= 1 or nh < 2 * workers:
+        return SunTable(h, x, _sun_rows((state, domain, h, x, scale)))
+    from concurrent.futures import ProcessPoolExecutor, ThreadPoolExecutor
+    import multiprocessing as mp
+
+    parts = np.array_split(np.arange(nh), 4 * workers)
+    jobs = [(state, domain, h[idx], x, scale) for idx in parts if len(idx)]
+    pool = (ThreadPoolExecutor(workers) if gpu_runtime_started()
+            else ProcessPoolExecutor(workers, mp_context=mp.get_context('spawn')))
+    with pool:
+        return SunTable(h, x, np.concatenate(list(pool.map(_sun_rows, jobs))))
 
 
 def lit_flag(q, r):

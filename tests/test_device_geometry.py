@@ -63,11 +63,26 @@ def test_end_and_pool_columns_on_the_device(gpu, which):
     assert len(hp.column) > 100 and close(dp.column, hp.column) < 1e-12
 
 
+def sampled_intervals(edge_off, edges, lower):
+    """(ray, start, end, lower flag) of every compacted interval wider than 1e-8 km."""
+    edge_off = np.asarray(edge_off, 'i8')
+    bits = np.asarray(lower).astype('u2').astype('i8')
+    n = np.diff(edge_off) - 1  # intervals per ray
+    ray = np.repeat(np.arange(len(n)), np.maximum(n, 0))
+    first = np.repeat(edge_off[:-1], np.maximum(n, 0))
+    rank = np.arange(len(ray)) - np.repeat(np.cumsum(np.maximum(n, 0)) - np.maximum(n, 0), np.maximum(n, 0))
+    a, e = edges[first + rank], edges[first + rank + 1]
+    flag = (bits[ray] >> rank) & 1
+    keep = e - a > 1e-8
+    return ray[keep], a[keep], e[keep], flag[keep]
+
+
 @pytest.mark.parametrize('which', [0, 1])
 def test_cell_bundles_on_the_device(gpu, which):
     """device_geometry.cell_bundles (incident quadrature, packs, CSR, end columns by HIP kernels) against
     geometry.Bundle of the host packs: the same rays in the same order, equal integer arrays, directions and edges
-    to FP64 rounding (the device and host transcendental functions may differ in the last bit)."""
+    to 1e-12 (the device and host transcendental functions may differ in the last bit, which the sinh grading
+    near the horizon amplifies)."""
     from sosfuse2_amd import device_geometry as dg
 
     st, top = _states()[which]
@@ -78,14 +93,25 @@ def test_cell_bundles_on_the_device(gpu, which):
         host = geometry.Bundle(packs)
         assert (b.groups, b.rays, b.samples) == (host.groups, host.rays, host.samples)
         get = lambda name: getattr(b, name).cpu().numpy()
-        for name in ('group_off', 'edge_off', 'lower', 'end_j', 'hits', 'ray_group'):
+        for name in ('group_off', 'hits', 'ray_group'):
             np.testing.assert_array_equal(get(name), getattr(host, name).astype(get(name).dtype), err_msg=name)
-        np.testing.assert_allclose(get('dirs'), host.dirs, rtol=0, atol=1e-14)
-        np.testing.assert_allclose(get('edges'), host.edges, rtol=1e-13, atol=1e-9)
+        np.testing.assert_allclose(get('dirs'), host.dirs, rtol=0, atol=1e-12)
+        # the sampled intervals (wider than 1e-8 km) with their lower-end flags; intervals narrower than that carry no
+        # nodes and may come and go with the last bit of a boundary at an interior shell where a ray starts
+        sd = sampled_intervals(get('edge_off'), get('edges'), get('lower'))
+        sh = sampled_intervals(host.edge_off, host.edges, host.lower)
+        np.testing.assert_array_equal(sd[0], sh[0])
+        np.testing.assert_array_equal(sd[3], sh[3])
+        np.testing.assert_allclose(sd[1], sh[1], rtol=1e-12, atol=1e-9)
+        np.testing.assert_allclose(sd[2], sh[2], rtol=1e-12, atol=1e-9)
         np.testing.assert_allclose(get('starts'), host.starts, rtol=1e-15, atol=0)
-        np.testing.assert_allclose(get('weights'), host.weights, rtol=2e-7, atol=0)
+        np.testing.assert_allclose(get('weights'), host.weights, rtol=1e-6, atol=0)
         np.testing.assert_allclose(get('dloc'), host.dloc, rtol=0, atol=2e-7)
-        np.testing.assert_allclose(get('end_f'), host.end_f, rtol=0, atol=2e-6)
+        # a ray ending on a chi node may take the cell on either side (index j with fraction 1, or j + 1 with 0): the
+        # interpolation position j + f is what must agree
+        np.testing.assert_allclose(get('end_j') + get('end_f').astype('f8'), host.end_j + host.end_f.astype('f8'),
+                                   rtol=0, atol=2e-6)
+        assert np.mean(get('end_j') == host.end_j) > 0.99
         np.testing.assert_allclose(get('end_col'), host.end_col, rtol=2e-6, atol=1e-30)
 
 
