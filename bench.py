@@ -306,6 +306,49 @@ def config0_variant(tol_default=None):
                 cells=s.geo.ns)
 
 
+PRODUCTION = dict(nmu=12, nphi=16, path_n=12, sun_scale=4)  # the reference's full-grid quadrature
+
+
+def production_variant(pscope, kernel, state, tol, headline_table, release):
+    """config1 at the reference's production quadrature (nmu 12, nphi 16, path_n 12, sun table scale 4): one table
+    of the state (device geometry, first light, the solve to the tolerance, the 25-elevation I/Q/U table), the warm
+    and first-light passes run in parts of at most Kernel.part_nodes quadrature nodes with one visit plan resident
+    at a time. Reported beside the headline with the table's difference from the headline's (coarser) quadrature."""
+    import numpy as np
+    import torch
+
+    from sosfuse2_amd import api, atmosphere
+
+    release()  # the headline's resident plans
+    torch.cuda.empty_cache()
+    torch.cuda.reset_peak_memory_stats()
+    cfg = pscope.cfg
+    wl, hs, el, az = api.axes(cfg)
+    pscope.kernel = kernel
+    t = time.perf_counter()
+    pscope.finish_on_device()
+    s = pscope.context(atmosphere.lane_optics(state, wl))
+    sv = pscope.solver(s, log=log)
+    db = sv.geometry.bundle
+    parts = kernel.parts(db, kernel.constants(s, db.nq))
+    torch.cuda.synchronize()
+    geo_s = time.perf_counter() - t
+    log(f'production quadrature: {db.rays} rays, {db.nodes_total} nodes in {len(parts)} parts, geometry {geo_s:.1f} s')
+    t = time.perf_counter()
+    x, info = sv.solve(60, tol)
+    table, _ = api.make_table(s, x, hs, el, az, log=log)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t
+    I, I0 = table[..., 0], headline_table[..., 0]
+    return dict(workload=f'config1 at nmu={cfg["nmu"]} nphi={cfg["nphi"]} path_n={cfg["path_n"]} '
+                         f'sun_scale={cfg["sun_scale"]} (the reference full-grid quadrature), plain FP32, one table',
+                seconds=round(sec, 2), tables_per_hour=round(3600.0 / sec, 2), evaluations=info['evaluations'],
+                rays=db.rays, nodes=int(db.nodes_total), parts=len(parts), part_nodes=kernel.part_nodes,
+                geometry_seconds=round(geo_s, 1), peak_device_gb=round(torch.cuda.max_memory_allocated() / 1e9, 1),
+                max_I_difference_vs_headline_quadrature=float(np.max(np.abs(I - I0) / np.abs(I0))),
+                rms_I_difference_vs_headline_quadrature=float(np.sqrt(np.mean(((I - I0) / I0) ** 2))))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -315,9 +358,10 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-tune', action='store_true', help='keep the default launch shapes (no autotuning)')
     ap.add_argument('--mixed', action='store_true', help='headline in the mixed-precision form (split-FP16 SH)')
-    ap.add_argument('--variants', default='mixed,batched,learned,config0',
+    ap.add_argument('--variants', default='mixed,batched,learned,config0,production',
                     help='forms measured beside the headline: mixed, batched, learned (learned + mixed + batched, '
-                         'needs batched), config0 (comma list, or "none")')
+                         'needs batched), config0, production (config1 at the reference full-grid quadrature: '
+                         'one table, about 30 s) (comma list, or "none")')
     ap.add_argument('--batched-states', type=int, default=64, help='states of the batched sample')
     ap.add_argument('--learned-train', type=int, default=16, help='training states of the warm-start network')
     args = ap.parse_args()
@@ -341,6 +385,12 @@ def main():
     # host preparation (grid, sun table, ground pools, cell packs) and the CPU baseline's worker processes come
     # before anything starts the GPU runtime
     scope = api.Scope(dict(cfg, sun_scale=1), state, SimpleNamespace(L=int(cfg['L'])))
+    variants = [v for v in args.variants.split(',') if v and v != 'none']
+    pscope = None
+    if 'production' in variants and args.config == 'config1':
+        # host part of the production geometry (ground-pool directions); the columns, quadrature, packs and
+        # bundles follow on the device
+        pscope = api.Scope(dict(cfg, **PRODUCTION), state, None, device_columns=True, device_geometry=True)
     cpu = None
     if not args.no_cpu and args.config == 'config1':
         cpu = CpuBaseline(scope, scope.context(atmosphere.lane_optics(state, wl)), dict(api.CONFIG0),
@@ -453,7 +503,6 @@ def main():
         transport_roofline=transport_roofline,
         launch_seconds_per_step=launch_s,
     )
-    variants = [v for v in args.variants.split(',') if v and v != 'none']
     if args.mixed or 'mixed' in variants:
         # the same state's table in the other form (same kernel, plans and state): per-entry error of I, Q, U of the
         # mixed table against the FP32 one
@@ -488,6 +537,14 @@ def main():
             log(f'learned + mixed + batched: {lm}')
     if 'config0' in variants:
         out['config0'] = config0_variant()
+    if pscope is not None:
+        def release():
+            solver.geometry.bundle.plan = None
+            solver.geometry.ground_bundle.plan = None
+
+        out['production'] = production_variant(pscope, k, state, tol, table, release)
+        out['production']['vs_headline'] = round(out['production']['tables_per_hour'] / out['value'], 4)
+        log(f'production quadrature: {out["production"]}')
     if cpu is not None:
         out['cpu_baseline'] = cpu.run(int(round(np.mean(evals))))
     print(json.dumps(out), flush=True)

@@ -82,7 +82,7 @@ typedef struct sos_grid {
     int32_t nh, nc;            /* heights, solar zenith nodes; cells = nh * nc */
     int32_t L;                 /* SH order: 8, 16 or 32 */
     int32_t lanes;             /* lanes per ray: 1, 16 or 32 (>= active wavelengths) */
-    int32_t nq;                /* Gauss-Legendre nodes per segment */
+    int32_t nq;                /* Gauss-Legendre nodes per segment, 1..256 (the sun table takes 32 x its scale) */
     int32_t strato;            /* sulfate layer present */
     float top_km;
     float air_H, aer_H, o3_c, o3_s, no2_c, no2_s, st_c, st_s; /* profile shapes (km) */

@@ -82,7 +82,7 @@ using namespace sos;
 
 // Column arguments from the C struct: interior shells by the top, profile kinds checked (every kind explicit).
 static int column_args(const sos_columns *c, ColumnArgs &a, const char *who) {
-    if (!c || !c->gl_u_dev || !c->gl_w_dev || c->nq < 1 || c->nq > 64 || !(c->top_km > 0))
+    if (!c || !c->gl_u_dev || !c->gl_w_dev || c->nq < 1 || c->nq > 256 || !(c->top_km > 0))
         return arg_error("%s: bad column arguments", who);
     a.top = c->top_km;
     a.shells = c->top_km > 80.0 ? 4 : 3;

@@ -88,8 +88,61 @@ class DeviceBundle:
             self.group_off, self.starts, self.targets, self.ray_group, self.dirs, self.dloc, self.weights,
             self.edge_off, self.edges, self.lower, self.end_j, self.end_f, self.end_col, self.hits))
 
+    def part(self, g0, g1):
+        """Sub-bundle of the groups [g0, g1): views of this bundle's per-group, per-ray and per-edge arrays, with
+        the group, edge and ray-group offsets rebased to the part (new small tensors). Every ray of a group stays in
+        its group's part, so transport and projection give the same values per ray and group as the whole bundle."""
+        if not 0 <= g0 < g1 <= self.groups:
+            raise ValueError(f'E_BUNDLE_PART: groups [{g0}, {g1}) outside [0, {self.groups})')
+        go = self.group_off
+        r0, r1 = int(go[g0].item()), int(go[g1].item())
+        e0, e1 = int(self.edge_off[r0].item()), int(self.edge_off[r1].item())
+        sub = DeviceBundle.__new__(DeviceBundle)
+        sub.host = None
+        sub.groups, sub.rays, sub.nq = g1 - g0, r1 - r0, self.nq
+        sub.group_off = (go[g0 : g1 + 1] - r0).contiguous()
+        sub.starts, sub.targets = self.starts[g0:g1], self.targets[g0:g1]
+        sub.ray_group = (self.ray_group[r0:r1] - g0).contiguous()
+        for k in ('dirs', 'dloc', 'weights', 'lower', 'end_j', 'end_f', 'end_col', 'hits'):  # per ray
+            setattr(sub, k, getattr(self, k)[r0:r1])
+        sub.edge_off = (self.edge_off[r0 : r1 + 1] - e0).contiguous()
+        sub.edges = self.edges[e0:e1]
+        # sampled intervals (longer than 1e-8 km) of the part's rays, times the node count
+        sampled = (sub.edges[1:] - sub.edges[:-1]) > 1e-8
+        if sub.rays > 1 and len(sampled):
+            cross = sub.edge_off[1:-1] - 1  # pairs that join the last edge of a ray to the next ray's first
+            sampled[cross[(cross >= 0) & (cross < len(sampled))]] = False
+        sub.samples = int(sampled.sum().item()) * self.nq
+        sub.plan = None
+        sub.struct = hip.Bundle(
+            sub.groups, sub.rays, *(hip.ptr(x) for x in (
+                sub.group_off, sub.starts, sub.targets, sub.ray_group, sub.dirs, sub.dloc, sub.weights,
+                sub.edge_off, sub.edges, sub.lower, sub.end_j, sub.end_f, sub.end_col, sub.hits)), None, None, None)
+        sub.span = (g0, g1, r0, r1)
+        return sub
+
 
 INDEX_LIMIT = 2**31 - 1  # node and (node, corner) visit ids are int32 on the device
+PART_NODES = 2**27  # quadrature nodes per plan part (4 x 2^27 visit ids stay far inside int32)
+
+
+def split_groups(per_group, limit):
+    """Split consecutive groups of per_group[i] nodes into ranges [g0, g1) of at most `limit` nodes each (a group
+    larger than the limit alone in its range, within the int32 visit-id bound)."""
+    per_group = np.asarray(per_group, np.int64)
+    if limit < 1:
+        raise ValueError('E_PART_NODES: the part size must be positive')
+    out, g0, acc = [], 0, 0
+    for g, n in enumerate(per_group):
+        n = int(n)
+        check_plan_size(n)
+        if acc and n and acc + n > limit:
+            out.append((g0, g))
+            g0, acc = g, 0
+        acc += n
+    if len(per_group):
+        out.append((g0, len(per_group)))
+    return out
 
 
 def check_plan_size(nodes):
@@ -339,6 +392,10 @@ class Kernel:
         # launch shapes chosen by the autotuner (sosfuse2_amd.tune); results do not depend on them
         self.tuning = dict(blocks_per_cu=0, node_chunk=1024, sh_tile=2)
         self.counters = None  # device uint64 [2] while counting is on
+        # bundles of more quadrature nodes than this run in parts of whole groups, one visit plan at a time (each
+        # part's plan is built for its launch and freed after it): about 2^27 nodes x (232 + 16 G) bytes of plan,
+        # node geometry and records per part, and int32 visit ids within a part
+        self.part_nodes = PART_NODES
 
     def sync(self):
         torch.cuda.synchronize(self.device)
@@ -391,6 +448,29 @@ class Kernel:
             db.plan.set_node_chunk(self.tuning['node_chunk'])
         return db.plan
 
+    def parts(self, db, dc):
+        """[db] when its plan fits part_nodes, else sub-bundles of consecutive whole groups of at most part_nodes
+        quadrature nodes each (cached on the bundle per part size)."""
+        key = ('parts', self.part_nodes)
+        cached = getattr(db, '_parts', None)
+        if cached is not None and cached[0] == key:
+            return cached[1]
+        if db.plan is not None or db.rays == 0:
+            db.nodes_total = db.plan.nodes if db.plan is not None else 0
+            out = [db]
+        else:
+            counts = torch.empty(max(db.rays, 1), dtype=torch.int64, device=self.device)
+            hip.call('sos_plan_count', dc.struct, db.struct, hip.ptr(counts), self.stream())
+            cum = torch.zeros(db.rays + 1, dtype=torch.int64, device=self.device)
+            cum[1:] = torch.cumsum(counts[: db.rays], 0)
+            per_group = cum[db.group_off[1:]] - cum[db.group_off[:-1]]
+            db.nodes_total = int(cum[-1].item())
+            out = [db.part(g0, g1) for g0, g1 in split_groups(per_group.cpu().numpy(), self.part_nodes)]
+            if len(out) == 1:
+                out = [db]
+        db._parts = (key, out)
+        return out
+
     def _event(self):
         if self.timed is None:
             return None
@@ -399,11 +479,26 @@ class Kernel:
         return e
 
     def radiance(self, db, ctx, field, first=False, boundary=None, pairs=4, plane=None):
-        """Per-ray radiance tensor [rays, G, pairs]."""
+        """Per-ray radiance tensor [rays, G, pairs] (part by part for a bundle above part_nodes)."""
+        dc = self.constants(ctx, db.nq)
+        parts = self.parts(db, dc)
+        if len(parts) == 1:
+            return self._radiance(db, ctx, field, first, boundary, pairs, plane)
+        if not first and plane is None:
+            plane = self.plane(ctx, field, dc)
+        out = torch.empty((db.rays, dc.G, pairs), dtype=torch.float32, device=self.device)
+        for part in parts:
+            _, _, r0, r1 = part.span
+            self._radiance(part, ctx, field, first, boundary, pairs, plane, out[r0:r1])
+            part.plan = None
+        return out
+
+    def _radiance(self, db, ctx, field, first=False, boundary=None, pairs=4, plane=None, out=None):
         dc = self.constants(ctx, db.nq)
         dc.struct.blocks_per_cu = self.tuning['blocks_per_cu']
         dc.struct.sh_tile = self.tuning['sh_tile']
-        out = torch.empty((db.rays, dc.G, pairs), dtype=torch.float32, device=self.device)
+        if out is None:
+            out = torch.empty((db.rays, dc.G, pairs), dtype=torch.float32, device=self.device)
         flags = (hip.FIRST if first else 0) | (hip.BOUNDARY if boundary is not None else 0)
         records = None
         if first and dc.uniform_phase and self.uniform_phase:
@@ -458,15 +553,23 @@ class Kernel:
     def launch(self, db, ctx, field, first=False, phase=0, boundary=None, plane=None):
         """(moments [groups, 4+K, G], flux [groups, G], query) of one bundle (`source_solver.py:2428-2456`).
         phase 0: moments and flux; 1: flux only; 2: the per-ray query [rays, G, 4]."""
-        rad = self.radiance(db, ctx, field, first, boundary, 4, plane)
         if phase == 2:
-            return None, None, rad
+            return None, None, self.radiance(db, ctx, field, first, boundary, 4, plane)
         dc = self.constants(ctx, db.nq)
         flux = torch.empty((db.groups, dc.G), dtype=torch.float32, device=self.device)
         moments = None
         if phase == 0:
             moments = torch.empty((db.groups, 4 + self.K, dc.G), dtype=torch.float32, device=self.device)
-        basis = self.plan(db, dc).basis_table(self, dc, db) if phase == 0 else None
-        hip.call('sos_project', dc.struct, db.struct, hip.ptr(rad), int(phase == 1), hip.ptr(basis), hip.ptr(moments),
-                 hip.ptr(flux), self.stream())
+        parts = self.parts(db, dc)
+        if len(parts) > 1 and not first and plane is None:
+            plane = self.plane(ctx, field, dc)
+        for part in parts:
+            rad = self._radiance(part, ctx, field, first, boundary, 4, plane)
+            g0, g1 = part.span[:2] if part is not db else (0, db.groups)
+            basis = self.plan(part, dc).basis_table(self, dc, part) if phase == 0 else None
+            hip.call('sos_project', dc.struct, part.struct, hip.ptr(rad), int(phase == 1), hip.ptr(basis),
+                     hip.ptr(None if moments is None else moments[g0:g1]), hip.ptr(flux[g0:g1]), self.stream())
+            if part is not db:
+                del rad, basis
+                part.plan = None
         return moments, flux, None

@@ -122,6 +122,11 @@ class Solver:
         if self._graph is None:
             self._apply(field)  # eager warm-up: first light, plans and constants exist before the capture
             self.kernel.sync()
+            bundle = self.geometry.bundle
+            if len(self.kernel.parts(bundle, self.kernel.constants(self.s, bundle.nq))) > 1:
+                # a bundle run in parts builds each part's plan at its launch (host synchronisation): no capture
+                self.graph = False
+                return self._apply(field)
             self._gx = torch.empty_like(field)
             self._gx.copy_(field)
             timed, self.kernel.timed = self.kernel.timed, None

@@ -40,6 +40,20 @@ def test_sun_table_on_the_device(gpu, which):
     assert close(dev.col, host.col) < 1e-12
 
 
+def test_sun_table_scale_4_on_the_device(gpu):
+    """The reference's production sun table scale (4: 321 heights x 1601 cosines, 128 nodes per segment) against
+    host rows at a few heights."""
+    from sosfuse2_amd import device_geometry as dg
+
+    st, top = _states()[1]
+    dom = geometry.Domain(top)
+    dev = dg.sun_table(st, dom, 4)
+    assert dev.col.shape == (321, 1601, 6)
+    rows = [0, 1, 97, 200, 320]
+    host = geometry._sun_rows((st, dom, dev.h[rows], dev.x, 4))
+    assert close(dev.col[rows], host) < 1e-12
+
+
 @pytest.mark.parametrize('which', [0, 1])
 def test_end_and_pool_columns_on_the_device(gpu, which):
     from sosfuse2_amd import device_geometry as dg
