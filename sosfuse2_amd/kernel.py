@@ -186,18 +186,26 @@ class Plan:
         self.sorted_cp = self.node_cp[self.node.long()]
         self.basis = None  # basis table [rays][KZ] (sos_plan_basis), built on first use (basis_table())
         self._cells = cells
+        self._nc = nc
         self.set_node_chunk(node_chunk)
-        # visits by visited cell (ground records)
-        offs = torch.tensor([0, 1, nc, nc + 1], dtype=torch.int32, device=dev)
-        vcell = (self.node_cell[:N, None] + offs[None, :]).reshape(-1)
-        self.vis = torch.argsort(vcell, stable=True).to(torch.int32)
-        self.item_cell, self.item_v0, self.item_v1 = _items(vcell, cells, chunk, dev)
-        del vcell
-        self.items = len(self.item_cell)
+        self._chunk = chunk
+        self.vis = None  # (node, corner) visits by visited cell, built on first use (visit_items())
         self.visits = 4 * N
         self.records = torch.empty((max(4 * N, 1), dc.G), dtype=torch.float32, device=dev)
         self.geom = None  # static node geometry of the warm consumer, built on first use (geometry())
         self.sun = None  # sun geometry of first light, built on first use (sun_geometry())
+
+    def visit_items(self):
+        """The (node, corner) visits sorted by visited cell and their work items of at most `chunk` visits (the
+        ground records of first light read them), built once."""
+        if self.vis is None:
+            N, nc, dev = self.nodes, self._nc, self.node_off.device
+            offs = torch.tensor([0, 1, nc, nc + 1], dtype=torch.int32, device=dev)
+            vcell = (self.node_cell[:N, None] + offs[None, :]).reshape(-1)
+            self.vis = torch.argsort(vcell, stable=True).to(torch.int32)
+            self.item_cell, self.item_v0, self.item_v1 = _items(vcell, self._cells, self._chunk, dev)
+            self.items = len(self.item_cell)
+        return self
 
     def basis_table(self, kernel, dc, db):
         """[rays, KZ] Legendre part of every ray's SH basis row (sos_plan_basis), built once."""
@@ -508,6 +516,7 @@ class Kernel:
             ng, gg, sgg = dc.phase_groups
             e0 = self._event()
             records = pl.records_for(dc.G)
+            pl.visit_items()
             hip.call('sos_ground_records', dc.struct, ng, gg, sgg, pl.items, hip.ptr(pl.item_cell),
                      hip.ptr(pl.item_v0), hip.ptr(pl.item_v1), hip.ptr(pl.vis), hip.ptr(pl.node_dir),
                      hip.ptr(records), self.stream())

@@ -20,9 +20,13 @@ width (the table's ozone column when mask bit 2), the table's NO2 column, O4 = (
 here, each raising an error: the sulfate layer's tabulated Mie phase (STRATO), the tropospheric Mie and two-layer
 aerosol options, and tops above 80 km (the reference's extended-domain support switch).
 
-Outputs are torch tensors on the device (the reference returns CuPy arrays): launch gives moments f32[cells, 4+K, W]
-and flux f32[cells, W] (phase 0, 1) or the query f32[rays, W, 4] (phase 2); transport gives host f8[n, W, 2, 4] with
-lane 0 filled, as the reference's adapter does.
+Array module: with array_module='numpy' (the default) the kernel's `xp` is NumPy and launch takes and returns host
+arrays, as the reference's numpy backend does, so the reference's own Solver (its NumPy branch of
+`src/sosfuse2/solve/source_iteration.py:309-663`), TransientGeometry and make_table run over the HIP transport
+unchanged, the moments crossing PCIe once per launch. With array_module='torch' the arrays stay on the device as
+torch tensors (the role CuPy arrays play for the reference's cupy backend). launch gives moments f32[cells, 4+K, W]
+and flux f32[cells, W] (phase 0, 1) or the query f32[rays, W, 4] (phase 2); transport gives host f8[n, W, 2, 4]
+with lane 0 filled, as the reference's adapter does.
 """
 
 import math
@@ -229,9 +233,11 @@ class Kernel:
     host, memory, basis, constants_device, ksplit, ksplit_memory, ksplit_preflight)."""
 
     def __init__(self, threads=4, height=1.2, g=0.8, backend='hip', block_size=128, flags=None, L=32, *,
-                 profile_step_km=0.5, domain=None, device='cuda'):
+                 profile_step_km=0.5, domain=None, device='cuda', array_module='numpy'):
         if backend != 'hip':
             raise ValueError(f"E_BACKEND: this kernel runs backend 'hip' (got {backend!r})")
+        if array_module not in ('numpy', 'torch'):
+            raise ValueError(f"E_ARRAY_MODULE: numpy or torch (got {array_module!r})")
         if block_size not in (32, 64, 128, 256):
             raise ValueError('E_BLOCK_SIZE: select 32,64,128,256')
         self.core = _Core(backend='hip', L=L, device=device)
@@ -246,7 +252,7 @@ class Kernel:
         self.deterministic = True  # fixed-order sums, no floating-point atomics: repeat runs are bitwise equal
         self.ksplit = False
         self.ksplit_mode = 0
-        self.xp = torch
+        self.xp = np if array_module == 'numpy' else torch
         self.constants = {}  # setup() results, keyed (id(s), nq)
         self.device_constants = {}
         self.metrics = dict(samples=0, angular_pairs=0, launches=0, query_launches=0)
@@ -372,13 +378,14 @@ class Kernel:
         if timed:
             start = torch.cuda.Event(enable_timing=True)
             start.record()
+        give = (lambda t: t.cpu().numpy()) if self.xp is np else (lambda t: t.contiguous())
         if phase == 2:
             rad = self.core.radiance(db, ctx, fd, first, bd, 4)
-            out = (None, None, rad[:, :W].contiguous())
+            out = (None, None, give(rad[:, :W]))
             self.metrics['query_launches'] += 1
         else:
             moments, flux, _ = self.core.launch(db, ctx, fd, first, phase, bd)
-            out = (None if moments is None else moments[..., :W].contiguous(), flux[..., :W].contiguous(), None)
+            out = (None if moments is None else give(moments[..., :W]), give(flux[..., :W]), None)
         if timed:
             end = torch.cuda.Event(enable_timing=True)
             end.record()

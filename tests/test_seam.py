@@ -172,17 +172,17 @@ def test_seam_launch_against_the_oracle(seam_case):
     m, f, q = k.launch(sc.bundle, s, None, True, 0)
     assert q is None and tuple(m.shape) == (ctx.geo.ns, 4 + k.K, ctx.W)
     mo, fo = orc.evaluate(zero, first=True, phase=0)
-    assert rel(m.cpu().numpy(), mo) < 1e-5
-    assert rel(f.cpu().numpy(), fo) < 1e-5
+    assert rel(k.host(m), mo) < 1e-5
+    assert rel(k.host(f), fo) < 1e-5
     field = physical_field(ctx, 4)
     bnd = np.abs(np.random.default_rng(2).normal(size=(ctx.geo.nc, ctx.W))).astype('f4') * 1e-2
     m, f, _ = k.launch(sc.bundle, s, torch.from_numpy(field).cuda(), False, 0, torch.from_numpy(bnd).cuda())
     mo, fo = orc.evaluate(field, phase=0, boundary=bnd.astype('f8'))
-    assert rel(m.cpu().numpy(), mo) < 1e-5
+    assert rel(k.host(m), mo) < 1e-5
     m, f, _ = k.launch(sc.ground, s, field, False, 1)
     assert m is None
     _, fo = orc.evaluate(field, phase=1, ground_only=True)
-    assert rel(f.cpu().numpy(), fo) < 1e-5
+    assert rel(k.host(f), fo) < 1e-5
     assert k.metrics['launches'] == 3
 
 
@@ -203,3 +203,26 @@ def test_seam_transport_adapter_against_the_oracle(seam_case):
         assert out.shape == (len(g['d']), ctx.W, 2, 4) and np.all(out[:, :, 1] == 0)
         ref = transport(g, orc.cq, field, first, None, PAIRS4)
         assert rel(out[:, :, 0], ref) < 2e-5, first
+
+
+@pytest.mark.gpu
+def test_seam_drives_the_reference_apply_loop_on_host_arrays(seam_case):
+    """The reference Solver's NumPy branch (`src/sosfuse2/solve/source_iteration.py:123-216`: first light once, then
+    per apply the ground pass, the boundary, the warm pass, moment + first-light moments, composition) driven through
+    the seam with host arrays only (xp = numpy), two applies from a zero source, against the oracle's two applies."""
+    from oracle.solve import compose
+
+    sc = seam_case
+    k, s, ctx, orc = sc.k, sc.s, sc.ctx, sc.orc
+    assert k.xp is np
+    nc, F = ctx.geo.nc, np.float32
+    x = k.xp.zeros((ctx.geo.ns, ctx.rows, ctx.W), 'f4')
+    m0, f0, _ = k.launch(sc.bundle, s, None, True, 0)
+    assert isinstance(m0, np.ndarray) and isinstance(f0, np.ndarray)
+    xo = x.copy()
+    for _ in range(2):
+        _, flux, _ = k.launch(sc.ground, s, x, False, 1)
+        m, _, _ = k.launch(sc.bundle, s, x, False, 0, flux + f0[:nc])
+        x = compose(m.astype('f8') + m0, orc.c, F)
+        xo = orc.apply(xo)
+    assert rel(x, xo) < 2e-5
