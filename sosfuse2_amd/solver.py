@@ -120,8 +120,7 @@ class Solver:
 
     def _apply_graph(self, field):
         if self._graph is None:
-            self._apply(field)  # eager warm-up: first light, plans and constants exist before the capture
-            self.kernel.sync()
+            self.initialize()  # first light before the capture
             bundle = self.geometry.bundle
             if len(self.kernel.parts(bundle, self.kernel.constants(self.s, bundle.nq))) > 1:
                 # a bundle run in parts builds each part's plan at its launch (host synchronisation): no capture
@@ -130,16 +129,20 @@ class Solver:
             self._gx = torch.empty_like(field)
             self._gx.copy_(field)
             timed, self.kernel.timed = self.kernel.timed, None
-            g = torch.cuda.CUDAGraph()
+            main = torch.cuda.current_stream(self.device)
             side = torch.cuda.Stream(self.device)
-            side.wait_stream(torch.cuda.current_stream(self.device))
+            side.wait_stream(main)
             with torch.cuda.stream(side):
-                self._apply(self._gx)  # a second eager pass on the capture stream settles its allocator state
-            torch.cuda.current_stream(self.device).wait_stream(side)
+                # the first evaluation runs eagerly on the capture stream: it builds the plans and constants the
+                # capture needs, settles the stream's allocator state, and is this call's result
+                first = self._apply(self._gx)
+            main.wait_stream(side)
+            g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
                 self._gy = self._apply(self._gx)
             self.kernel.timed = timed
             self._graph = g
+            return first
         self._gx.copy_(field)
         self._graph.replay()
         return self._gy.clone()
