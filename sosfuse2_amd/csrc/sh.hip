@@ -194,11 +194,13 @@ __device__ inline void stage_km(unsigned char *km, int nt) {
 }
 
 // cos(m phi), m = 0 .. L, by the FP32 Chebyshev recurrence (arithmetic.py:112-119).
+// With scale a power of two the values are cos(m phi) scale: the recurrence is linear, so running it from scale and
+// cp scale gives the scaled values of the unscaled recurrence exactly.
 template <int L>
-__device__ inline void cos_chain(float cp, float *c) {
-    float c0 = 1.0f, c1 = cp;
-    c[0] = 1.0f;
-    c[1] = cp;
+__device__ inline void cos_chain(float cp, float *c, float scale = 1.0f) {
+    float c0 = scale, c1 = cp * scale;
+    c[0] = c0;
+    c[1] = c1;
     for (int m = 2; m <= L; ++m) {
         const float cm = (2.0f * cp) * c1 - c0;
         c0 = c1;
@@ -218,10 +220,11 @@ struct RoundMeta {
 
 // Thread v < V: node v of the round starting at sorted position r0 into slot q (id -1 and ray -1 past v1).
 template <int L, int V, int S>
-__device__ inline void meta_fill(RoundMeta<L, V, S> &mt, int q, int v, bool in, int id, int ray, float cp) {
+__device__ inline void meta_fill(RoundMeta<L, V, S> &mt, int q, int v, bool in, int id, int ray, float cp,
+                                 float scale = 1.0f) {
     mt.id[q][v] = in ? id : -1;
     mt.ray[q][v] = in ? ray : -1;
-    cos_chain<L>(in ? cp : 1.0f, mt.cm[q][v]);
+    cos_chain<L>(in ? cp : 1.0f, mt.cm[q][v], scale);
 }
 
 // Basis rows of the round in slot q, PT float4 chunks per thread (chunk c = tid + j NT: row c / (KZ/4)), read into
@@ -465,6 +468,34 @@ __device__ inline void quad_fetch_rows(float4 *raw, const float *basis, const in
     }
 }
 
+// Row-block layout of a round's basis rows (the split form): wavefront w takes rows 4 (w mod 4) .. + 3 and, in pass
+// j, the group of 16 chunk positions KG j + w / 4 (KG = waves / 4); lane l the row 4 (w mod 4) + l / 16 and the chunk
+// 16 group + l mod 16. A wavefront's 64 chunks land contiguously at raw[(group 4 + w mod 4) 64 ..], so raw holds
+// RAW_GROUPS(KZ) x 256 chunks; groups past the row are skipped (uniform per wavefront) and chunk positions past it
+// read the row's last chunk into their own slot, which the tile never reads. Row, chunk and LDS offsets are then
+// constants per thread and pass: no division per chunk.
+template <int KZ>
+constexpr int raw_groups() { return (KZ / 4 + 15) / 16; }
+
+template <int V, int KZ, int NT>
+__device__ inline void quad_fetch_rows_rb(float4 *raw, const float *basis, const int *rays, int tid) {
+    constexpr int KG = NT / 256, NGR = raw_groups<KZ>(), PASSES = (NGR + KG - 1) / KG;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63, wave = tid >> 6, rb = wave & 3;
+    const int ray = max(rays[4 * rb + (lane >> 4)], 0);
+    const float *row = basis + (int64_t)ray * KZ;
+#pragma unroll
+    for (int j = 0; j < PASSES; ++j) {
+        const int grp = KG * j + (wave >> 2);
+        if (grp < NGR) {
+            const int k4 = min(16 * grp + (lane & 15), KZ / 4 - 1);
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(row + 4 * k4),
+                                             (__attribute__((address_space(3))) void *)&raw[(grp * 4 + rb) * 64], 16,
+                                             0, 0);
+        }
+    }
+}
+
 // The rounds of the quad form (below). Every LDS array comes in as a restrict pointer, so that each LDS access
 // carries alias-scope information: the compiler's wait insertion can then tell the reads of Z, the node data and the
 // partials from the LDS-direct loads into raw and ndata still in flight, instead of waiting for those loads before
@@ -669,6 +700,18 @@ __device__ inline half4_t split_hi(float4 z) {
     return half4_t{(_Float16)z.x, (_Float16)z.y, (_Float16)z.z, (_Float16)z.w};
 }
 
+// Split by truncation: hi keeps the leading 11 significant bits of z (exact in FP16 for |z| in FP16's normal
+// range), lo = (z - hi) 2^11 holds the next 13 bits rounded to 11; no conversion back from FP16 is needed.
+__device__ inline void split_trunc(float4 z, half4_t &h, half4_t &l) {
+    const float hx = __uint_as_float(__float_as_uint(z.x) & 0xffffe000u);
+    const float hy = __uint_as_float(__float_as_uint(z.y) & 0xffffe000u);
+    const float hz = __uint_as_float(__float_as_uint(z.z) & 0xffffe000u);
+    const float hw = __uint_as_float(__float_as_uint(z.w) & 0xffffe000u);
+    h = half4_t{(_Float16)hx, (_Float16)hy, (_Float16)hz, (_Float16)hw};
+    l = half4_t{(_Float16)((z.x - hx) * 2048.0f), (_Float16)((z.y - hy) * 2048.0f), (_Float16)((z.z - hz) * 2048.0f),
+                (_Float16)((z.w - hw) * 2048.0f)};
+}
+
 __device__ inline half4_t split_lo(float4 z, half4_t h) {
     return half4_t{(_Float16)((z.x - (float)h.x) * 2048.0f), (_Float16)((z.y - (float)h.y) * 2048.0f),
                    (_Float16)((z.z - (float)h.z) * 2048.0f), (_Float16)((z.w - (float)h.w) * 2048.0f)};
@@ -755,7 +798,6 @@ __device__ __attribute__((always_inline)) inline void quad_rounds_split(
     constexpr int ZS = KZ + 8;  // halves
     constexpr float ZSCALE = 8192.0f;
     constexpr int SW = NT - 64;
-    constexpr int PT = (V * KZ / 4 + NT - 1) / NT;
     RoundMeta<L, V, 8> &mt = *mtp;
     const int32_t *node = a.node, *node_ray = a.node_ray, *node_cp = reinterpret_cast<const int32_t *>(a.node_cp);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -763,14 +805,14 @@ __device__ __attribute__((always_inline)) inline void quad_rounds_split(
     const bool meta = tid >= SW && tid < SW + V;
     if (tid < 2 * V) {
         const NodeRef n = node_ref(a, v0 + tid, v1);
-        meta_fill<L, V, 8>(mt, tid / V, tid % V, n.id >= 0, n.id, n.ray, n.cp);
+        meta_fill<L, V, 8>(mt, tid / V, tid % V, n.id >= 0, n.id, n.ray, n.cp, ZSCALE);
     }
     const bool nlane = tid >= SW && lane < 3 * V;  // the node-data loaders
     const int64_t ndel = lane < V ? 0 : lane < 2 * V ? (int64_t)(node_ray - node) : (int64_t)(node_cp - node);
     if (rounds > 2) quad_fetch_nodes<V>(ndata + 2 * 3 * V, node + ndel, v0 + 2 * V, v1, nlane, lane);
     __syncthreads();
     const float *basis = a.basis;
-    if (rounds > 0) quad_fetch_rows<V, KZ, NT, PT>(raw, basis, mt.ray[0], tid);
+    if (rounds > 0) quad_fetch_rows_rb<V, KZ, NT>(raw, basis, mt.ray[0], tid);
     floatx4 dprev = floatx4{0.f, 0.f, 0.f, 0.f};
     for (int r = 0; r <= rounds; ++r) {
         const int cur = r & 1;
@@ -778,32 +820,35 @@ __device__ __attribute__((always_inline)) inline void quad_rounds_split(
         if (r < rounds) {
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this thread's rows of round r are in raw
             const int zt = cur * V * ZS;
+            constexpr int KG = NT / 256, NGR = raw_groups<KZ>(), PASSES = (NGR + KG - 1) / KG;
             int t = tid;
-            asm volatile("" : "+v"(t));  // chunk positions recomputed per round (registers)
+            asm volatile("" : "+v"(t));  // positions recomputed per round (registers)
+            const int rb = (t >> 6) & 3, v = 4 * rb + ((t & 63) >> 4);
+            const float *cm = mt.cm[r & 7][v];  // cos(m phi) ZSCALE
+            _Float16 *zh = Zh + zt + v * ZS, *zl = Zl + zt + v * ZS;
 #pragma unroll
-            for (int j = 0; j < PT; ++j) {
-                const int c = t + j * NT;
-                if (c < V * (KZ / 4)) {
-                    const int v = c / (KZ / 4), k4 = c % (KZ / 4);
-                    const float4 p = raw[c];
+            for (int j = 0; j < PASSES; ++j) {
+                const int grp = KG * j + (t >> 8);
+                const int k4 = 16 * grp + (t & 15);
+                if (grp < NGR && k4 < KZ / 4) {
+                    const float4 p = raw[(grp * 4 + rb) * 64 + (t & 63)];
                     const uchar4 m = reinterpret_cast<const uchar4 *>(km)[k4];
-                    const float *cm = mt.cm[r & 7][v];
-                    const float4 z = float4{(p.x * cm[m.x]) * ZSCALE, (p.y * cm[m.y]) * ZSCALE,
-                                            (p.z * cm[m.z]) * ZSCALE, (p.w * cm[m.w]) * ZSCALE};
-                    const half4_t h = split_hi(z);
-                    reinterpret_cast<half4_t *>(Zh + zt + v * ZS)[k4] = h;
-                    reinterpret_cast<half4_t *>(Zl + zt + v * ZS)[k4] = split_lo(z, h);
+                    const float4 z = float4{p.x * cm[m.x], p.y * cm[m.y], p.z * cm[m.z], p.w * cm[m.w]};
+                    half4_t h, l;
+                    split_trunc(z, h, l);
+                    reinterpret_cast<half4_t *>(zh)[k4] = h;
+                    reinterpret_cast<half4_t *>(zl)[k4] = l;
                 }
             }
             if (meta && r + 2 < rounds) {  // from ndata, fetched during round r - 1 (or before the loop)
                 const int v = lane;
                 const int *nd = ndata + ((r + 2) & 3) * 3 * V;
-                meta_fill<L, V, 8>(mt, (r + 2) & 7, v, r0 + 2 * V + v < v1, nd[v], nd[V + v], __int_as_float(nd[2 * V + v]));
+                meta_fill<L, V, 8>(mt, (r + 2) & 7, v, r0 + 2 * V + v < v1, nd[v], nd[V + v], __int_as_float(nd[2 * V + v]), ZSCALE);
             }
             if (r + 3 < rounds)  // slot (r + 3) & 3 was last read in round r - 3
                 quad_fetch_nodes<V>(ndata + ((r + 3) & 3) * 3 * V, node + ndel, r0 + 3 * V, v1, nlane, lane);
             // the reads of raw above have returned (their values are in Z): the rows of round r + 1 may land
-            if (r + 1 < rounds) quad_fetch_rows<V, KZ, NT, PT>(raw, basis, mt.ray[(r + 1) & 7], tid);
+            if (r + 1 < rounds) quad_fetch_rows_rb<V, KZ, NT>(raw, basis, mt.ray[(r + 1) & 7], tid);
         }
         // Workgroup barrier over the LDS only: the wavefronts share Z, the node data and the partials, all in LDS,
         // and nothing in global memory, so the barrier waits for this wavefront's LDS operations and leaves its
@@ -931,7 +976,7 @@ void sh_records_split_quad_kernel(ShArgs a) {
     constexpr int ZS = KZ + 8;
     __shared__ __attribute__((aligned(16))) _Float16 Zh[2][V][ZS];
     __shared__ __attribute__((aligned(16))) _Float16 Zl[2][V][ZS];
-    __shared__ float4 raw[V * KZ / 4];
+    __shared__ float4 raw[raw_groups<KZ>() * 256];  // row-block layout (quad_fetch_rows_rb)
     __shared__ RoundMeta<L, V, 8> mt;
     __shared__ int ndata[4][3 * V];
     __shared__ __attribute__((aligned(4))) unsigned char km[KZ];
