@@ -449,31 +449,14 @@ __device__ inline void quad_fetch_nodes(int *slot, const int32_t *src, int64_t f
     }
 }
 
-// Basis rows of one round (rays[v] of its 16 nodes) straight into raw with LDS-direct 16-byte loads: lane l of a
-// wavefront writes 16 bytes at the wavefront's base + 16 l, so thread tid's chunk j (c = tid + j NT: row c / (KZ / 4))
-// lands at raw[c]. Each thread later reads back only its own chunks, so its own wait for the loads is all the
-// ordering needed. Absent nodes (ray -1) read row 0: their products are never stored.
-template <int V, int KZ, int NT, int PT>
-__device__ inline void quad_fetch_rows(float4 *raw, const float *basis, const int *rays, int tid) {
-    asm volatile("" : "+v"(tid));  // the chunk positions are recomputed per call, not held across the rounds
-#pragma unroll
-    for (int j = 0; j < PT; ++j) {
-        const int c = tid + j * NT;
-        if (c < V * (KZ / 4)) {  // the same for the whole wavefront: V KZ / 4 is a multiple of 64
-            const int ray = max(rays[c / (KZ / 4)], 0);
-            const float *src = basis + (int64_t)ray * KZ + 4 * (c % (KZ / 4));
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
-                                             (__attribute__((address_space(3))) void *)&raw[j * NT + (tid & ~63)], 16, 0, 0);
-        }
-    }
-}
-
-// Row-block layout of a round's basis rows (the split form): wavefront w takes rows 4 (w mod 4) .. + 3 and, in pass
-// j, the group of 16 chunk positions KG j + w / 4 (KG = waves / 4); lane l the row 4 (w mod 4) + l / 16 and the chunk
-// 16 group + l mod 16. A wavefront's 64 chunks land contiguously at raw[(group 4 + w mod 4) 64 ..], so raw holds
-// RAW_GROUPS(KZ) x 256 chunks; groups past the row are skipped (uniform per wavefront) and chunk positions past it
-// read the row's last chunk into their own slot, which the tile never reads. Row, chunk and LDS offsets are then
-// constants per thread and pass: no division per chunk.
+// Basis rows of one round (rays[v] of its 16 nodes) straight into raw with LDS-direct 16-byte loads, in a row-block
+// layout: wavefront w takes rows 4 (w mod 4) .. + 3 and, in pass j, the group of 16 chunk positions KG j + w / 4
+// (KG = waves / 4); lane l the row 4 (w mod 4) + l / 16 and the chunk 16 group + l mod 16. The hardware writes lane
+// l's 16 bytes at the wavefront's base + 16 l, so a wavefront's 64 chunks land at raw[(group 4 + w mod 4) 64 ..] and
+// raw holds raw_groups(KZ) x 256 chunks; each thread later reads back only its own slot, so its own wait for the
+// loads is all the ordering needed. Groups past the row are skipped (uniform per wavefront) and chunk positions past
+// it read the row's last chunk into their own slot, which the tile never reads. Absent nodes (ray -1) read row 0:
+// their products are never stored. Row, chunk and LDS offsets are constants per thread and pass (no division).
 template <int KZ>
 constexpr int raw_groups() { return (KZ / 4 + 15) / 16; }
 
@@ -515,7 +498,6 @@ __device__ __attribute__((always_inline)) inline void quad_rounds(
     constexpr int V = 16;
     constexpr int ZS = KZ + 4;
     constexpr int SW = NT - 64;
-    constexpr int PT = (V * KZ / 4 + NT - 1) / NT;
     constexpr int NG = KQH;
     RoundMeta<L, V, 8> &mt = *mtp;
     const int32_t *node = a.node, *node_ray = a.node_ray, *node_cp = reinterpret_cast<const int32_t *>(a.node_cp);
@@ -531,26 +513,27 @@ __device__ __attribute__((always_inline)) inline void quad_rounds(
     if (rounds > 2) quad_fetch_nodes<V>(ndata + 2 * 3 * V, node + ndel, v0 + 2 * V, v1, nlane, lane);
     __syncthreads();
     const float *basis = a.basis;
-    if (rounds > 0) quad_fetch_rows<V, KZ, NT, PT>(raw, basis, mt.ray[0], tid);
+    if (rounds > 0) quad_fetch_rows_rb<V, KZ, NT>(raw, basis, mt.ray[0], tid);
     floatx4 dprev = floatx4{0.f, 0.f, 0.f, 0.f};
     for (int r = 0; r <= rounds; ++r) {
         const int cur = r & 1;
         const int64_t r0 = v0 + (int64_t)r * V;
         if (r < rounds) {
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this thread's rows of round r are in raw
-            float *Zc = Z + cur * V * ZS;
+            constexpr int KG = NT / 256, NGR = raw_groups<KZ>(), PASSES = (NGR + KG - 1) / KG;
             int t = tid;
-            asm volatile("" : "+v"(t));  // chunk positions recomputed per round (registers)
+            asm volatile("" : "+v"(t));  // positions recomputed per round (registers)
+            const int rb = (t >> 6) & 3, v = 4 * rb + ((t & 63) >> 4);
+            const float *cm = mt.cm[r & 7][v];
+            float *zr = Z + cur * V * ZS + v * ZS;
 #pragma unroll
-            for (int j = 0; j < PT; ++j) {
-                const int c = t + j * NT;
-                if (c < V * (KZ / 4)) {
-                    const int v = c / (KZ / 4), k4 = c % (KZ / 4);
-                    const float4 p = raw[c];
+            for (int j = 0; j < PASSES; ++j) {
+                const int grp = KG * j + (t >> 8);
+                const int k4 = 16 * grp + (t & 15);
+                if (grp < NGR && k4 < KZ / 4) {
+                    const float4 p = raw[(grp * 4 + rb) * 64 + (t & 63)];
                     const uchar4 m = reinterpret_cast<const uchar4 *>(km)[k4];
-                    const float *cm = mt.cm[r & 7][v];
-                    reinterpret_cast<float4 *>(Zc + v * ZS)[k4] =
-                        float4{p.x * cm[m.x], p.y * cm[m.y], p.z * cm[m.z], p.w * cm[m.w]};
+                    reinterpret_cast<float4 *>(zr)[k4] = float4{p.x * cm[m.x], p.y * cm[m.y], p.z * cm[m.z], p.w * cm[m.w]};
                 }
             }
             if (meta && r + 2 < rounds) {  // from ndata, fetched during round r - 1 (or before the loop)
@@ -561,7 +544,7 @@ __device__ __attribute__((always_inline)) inline void quad_rounds(
             if (r + 3 < rounds)  // slot (r + 3) & 3 was last read in round r - 3
                 quad_fetch_nodes<V>(ndata + ((r + 3) & 3) * 3 * V, node + ndel, r0 + 3 * V, v1, nlane, lane);
             // the reads of raw above have returned (their values are in Z): the rows of round r + 1 may land
-            if (r + 1 < rounds) quad_fetch_rows<V, KZ, NT, PT>(raw, basis, mt.ray[(r + 1) & 7], tid);
+            if (r + 1 < rounds) quad_fetch_rows_rb<V, KZ, NT>(raw, basis, mt.ray[(r + 1) & 7], tid);
         }
         // Workgroup barrier over the LDS only: the wavefronts share Z, the node data and the partials, all in LDS,
         // and nothing in global memory, so the barrier waits for this wavefront's LDS operations and leaves its
@@ -609,7 +592,7 @@ void sh_records_quad_kernel(ShArgs a) {
     constexpr int V = 16;
     constexpr int ZS = KZ + 4;
     __shared__ __attribute__((aligned(16))) float Z[2][V][ZS];
-    __shared__ float4 raw[V * KZ / 4];  // the round's basis rows as fetched: chunk c = tid + j NT of thread tid
+    __shared__ float4 raw[raw_groups<KZ>() * 256];  // the round's basis rows, row-block layout (quad_fetch_rows_rb)
     __shared__ RoundMeta<L, V, 8> mt;  // ids of round r are read after the barrier of round r + 1: eight slots
     __shared__ int ndata[4][3 * V];    // node id, ray and cos phi (bits) of round r in slot r & 3, as fetched
     __shared__ __attribute__((aligned(4))) unsigned char km[KZ];
