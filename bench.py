@@ -44,20 +44,32 @@ def measured_traffic(kernel, config, mixed):
     return None
 
 
-def angular_pairs(plan):
+def angular_pairs(plan, nc):
     """P of the transport's SH work: the (ray, corner cell) refills of the reference's per-ray SH cache
-    (numpy_reference.py:579-600, counted as `angular_pairs` at :597). The four corners of a node are its base cell
-    plus fixed offsets, so a ray refills all four exactly when its base cell differs from its previous node's:
-    P = 4 x (runs of equal base cell along the rays)."""
+    (numpy_reference.py:579-600, counted as `angular_pairs` at :597) after its adjacent-stencil aliases
+    (reuse_corners, :106-129: when the base cell moves by one row and/or column the cache entries of the corners the
+    old and new stencils share are copied, not refilled). So a corner cell of a node is refilled exactly when it is
+    not a corner of the ray's previous node (or the node is the ray's first): P = the number of maximal runs of
+    consecutive nodes of one ray that share a corner cell. The four corners of base cell b are b, b + 1, b + nc,
+    b + nc + 1 (nc chi nodes per row; base cells never sit in the last column or row)."""
     import torch
 
     n = plan.nodes
-    cell = plan.node_cell[:n].long()
-    first = plan.node_off[:-1][plan.node_off[:-1] < plan.node_off[1:]]
-    new = torch.ones(n, dtype=torch.bool, device=cell.device)
-    new[1:] = cell[1:] != cell[:-1]
-    new[first] = True
-    return 4 * int(new.sum().item())
+    cell = plan.node_cell[:n]
+    starts = plan.node_off[:-1][plan.node_off[:-1] < plan.node_off[1:]]
+    first = torch.zeros(n, dtype=torch.bool, device=cell.device)
+    first[starts] = True
+    first[0] = True
+    prev = torch.empty_like(cell)
+    prev[0] = cell[0]
+    prev[1:] = cell[:-1]
+    total = 0
+    for off in (0, 1, nc, nc + 1):
+        d = cell + off - prev
+        shared = (d == 0) | (d == 1) | (d == nc) | (d == nc + 1)
+        total += int((first | ~shared).sum().item())
+        del d, shared
+    return total
 
 
 def consumer_bytes(plan, rays, cells, G):
@@ -449,12 +461,12 @@ def main():
                                                   for n, v in kinds.items()))
     launch_s = {n: round(sum(v) / args.steps, 4) for n, v in kinds.items()}
     # the dominant kernel is the SH contraction of the warm pass; its algorithmic work is the reference's:
-    # 2 K W flops per (ray, corner cell) refill of the per-ray SH cache (angular_pairs)
+    # 2 K W flops per (ray, corner cell) refill of the per-ray SH cache after its corner aliases (angular_pairs)
     sh = kinds.get('sh_warm', [])
     t_sh = sum(sh) / max(1, len(sh))
     db = solver.geometry.bundle
     plan = k.plan(db, k.constants(s, db.nq))
-    pairs = angular_pairs(plan)
+    pairs = angular_pairs(plan, s.geo.nc)
     visits = int(plan.visits)
     flops = 2.0 * pairs * k.K * s.W
     peak = PEAK_FP16 if args.mixed else PEAK_FP32
