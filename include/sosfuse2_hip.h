@@ -105,8 +105,10 @@ typedef struct sos_grid {
                                   barrier per round of 16 nodes), 1 stepwise (seeds, basis and products separated by
                                   barriers), 2 quad (the flow's rounds on twice the wavefronts, each holding half of
                                   the K range); a tuning knob, the records are bitwise the same. The split-FP16
-                                  records (mixed precision) take the quad form whatever its value (the flow form at
-                                  L = 8) */
+                                  records (mixed precision) at 32 lanes and L = 16, 24, 32 take the staggered form
+                                  (8 whole-K wavefronts, tile formation of one under the products of the other on
+                                  every SIMD) unless sh_tile is 1, which takes the quad form; other lane counts the
+                                  quad form (the flow form at L = 8) */
     int32_t pool_max;          /* largest pool count of a cell (goff_dev differences); sos_ground_records holds a
                                   cell's pools in registers and takes at most 640 */
     /* Profile kinds of air, aerosol, O3, NO2, O4 and the sulfate layer (numpy_reference.py:187-223 evaluates the

@@ -23,116 +23,90 @@ __device__ inline float plane_at(const void *p, int64_t i) {
     else return reinterpret_cast<const float *>(p)[i];
 }
 
-// Recurrence constants (arithmetic.py:69-83) in FP64, rounded to T, staged once per workgroup.
-template <int L, typename T>
-__device__ inline void stage_constants(T *aa, T *bb, int nt) {
-    constexpr int K = (L + 1) * (L + 2) / 2;
-    for (int k = threadIdx.x; k < K; k += nt) {
-        int l = 0;
-        while ((l + 1) * (l + 2) / 2 <= k) ++l;
-        const int m = k - l * (l + 1) / 2;
-        const double fl = l, fm = m;
-        aa[k] = l > m ? (T)sqrt((4.0 * fl * fl - 1.0) / (fl * fl - fm * fm)) : (T)0;
-        bb[k] = l > m + 1 ? (T)sqrt((2.0 * fl + 1.0) * ((fl - 1.0) * (fl - 1.0) - fm * fm) /
-                                    ((2.0 * fl - 3.0) * (fl * fl - fm * fm)))
-                          : (T)0;
-    }
-}
-
 // dd[m] = -sqrt((2m + 1) / 2m), the factor of the diagonal seed chain (arithmetic.py:95-97), rounded to T.
 template <int L, typename T>
 __device__ inline void stage_dd(T *dd, int nt) {
     for (int m = threadIdx.x; m <= L; m += nt) dd[m] = m ? (T)-sqrt((2.0 * m + 1) / (2.0 * m)) : (T)0;
 }
 
-// Per node: the diagonal seeds 1/sqrt(4 pi) prod dd_j st (FP64, arithmetic.py:95-97) and cos(m phi) by the FP32
-// Chebyshev recurrence (arithmetic.py:112-119).
-template <int L, int V>
-__device__ inline void visit_seeds(int v, double mu, float cp, const double *dd, double (*seed)[V], float (*cmv)[V]) {
-    const double st = sqrt(fmax(0.0, 1.0 - mu * mu));
-    double diag = 0.28209479177387814;
-    float c0 = 1.0f, c1 = cp;
-    for (int m = 0; m <= L; ++m) {
-        if (m) diag = (diag * dd[m]) * st;
-        seed[m][v] = diag;
-        float cm;
-        if (m == 0) cm = 1.0f;
-        else if (m == 1) cm = cp;
-        else { cm = (2.0f * cp) * c1 - c0; c0 = c1; c1 = cm; }
-        cmv[m][v] = cm;
+// Recurrence constants at the m-major positions: ab[moff(m) + l - m] = (aa, bb) of (l, m) (arithmetic.py:69-83, FP64;
+// zero for l < m + 2), so that a chain of one order walks ab with its output position.
+template <int L>
+__device__ inline void stage_ab(double2 *ab, int nt) {
+    constexpr int K = (L + 1) * (L + 2) / 2;
+    for (int k = threadIdx.x; k < K; k += nt) {
+        int m = 0;
+        while (moff<L>(m + 1) <= k) ++m;
+        const int l = m + (k - moff<L>(m));
+        const double fl = l, fm = m;
+        const double x = l > m ? sqrt((4.0 * fl * fl - 1.0) / (fl * fl - fm * fm)) : 0.0;
+        const double y = l > m + 1 ? sqrt((2.0 * fl + 1.0) * ((fl - 1.0) * (fl - 1.0) - fm * fm) /
+                                          ((2.0 * fl - 3.0) * (fl * fl - fm * fm)))
+                                   : 0.0;
+        ab[k] = double2{x, y};
     }
 }
 
-// Basis rows of V nodes into LDS, tasks first, first + stride, ... below last. Task (node, p) runs the Legendre
-// recurrence over l for the two orders m = p and m = L - p side by side (their lengths add up to L + 2, so tasks
-// are balanced and each thread has two independent dependency chains): a loop over both chains while the short one
-// lasts (unrolled UB times), then the long one alone (unrolled UT times). Each value goes to sink(v, k, z). With
-// T = double (the reference's FP64 basis) z = fp32(x P) cos(m phi), x = sqrt 2 for m > 0, each basis value rounded
-// to FP32 before the product; with T = float (the split path) z = P (x cos(m phi)) in FP32.
-template <int L, int V, int UB, int UT, typename T, typename Sink>
-__device__ inline void build_rows(const T *mus, const T (*seed)[V], const float (*cmv)[V], const T *aa, const T *bb,
-                                  int nv, int first, int stride, int last, Sink sink) {
-    for (int task = first; task < last; task += stride) {
-        const int v = task % V, p = task / V;
-        const int ma = p, mb = L - p;
-        const bool two = mb != ma;
-        const int oa = moff<L>(ma), ob = moff<L>(mb);
-        if (v >= nv) {
-            for (int l = ma; l <= L; ++l) sink(v, oa + l - ma, 0.0f);
-            if (two)
-                for (int l = mb; l <= L; ++l) sink(v, ob + l - mb, 0.0f);
-            continue;
-        }
-        const T mu = mus[v];
-        const float ca = cmv[ma][v], cb = cmv[mb][v];
-        const float cxa = ma ? 1.41421356237309515f * ca : ca, cxb = 1.41421356237309515f * cb;
-        const double xa = ma ? 1.41421356237309515 : 1.0, xb = 1.41421356237309515;
-        auto za = [&](T pv) -> float {
-            if constexpr (sizeof(T) == 8) return (float)(xa * pv) * ca;
-            else return pv * cxa;
-        };
-        auto zb = [&](T pv) -> float {
-            if constexpr (sizeof(T) == 8) return (float)(xb * pv) * cb;
-            else return pv * cxb;
-        };
-        // chain a: m = p (long); chain b: m = L - p (short, length p + 1)
-        T a0 = seed[ma][v], b0 = seed[mb][v];
-        sink(v, oa, za(a0));
-        if (two) sink(v, ob, zb(b0));
-        T a1 = (sqrt((T)(2 * ma + 3)) * mu) * a0;  // ma < L always
-        sink(v, oa + 1, za(a1));
-        T b1 = 0;
-        if (two && mb < L) {
-            b1 = (sqrt((T)(2 * mb + 3)) * mu) * b0;
-            sink(v, ob + 1, zb(b1));
-        }
-        int la = ma + 2, lb = mb + 2;
-#pragma unroll UB
-        for (; two && lb <= L; ++la, ++lb) {
-            const int ka = kidx(la, ma), kb = kidx(lb, mb);
-            const T a2 = (aa[ka] * mu) * a1 - bb[ka] * a0;
-            const T b2 = (aa[kb] * mu) * b1 - bb[kb] * b0;
-            sink(v, oa + la - ma, za(a2));
-            sink(v, ob + lb - mb, zb(b2));
-            a0 = a1; a1 = a2;
-            b0 = b1; b1 = b2;
-        }
-#pragma unroll UT
-        for (; la <= L; ++la) {
-            const int ka = kidx(la, ma);
-            const T a2 = (aa[ka] * mu) * a1 - bb[ka] * a0;
-            sink(v, oa + la - ma, za(a2));
-            a0 = a1; a1 = a2;
-        }
+// The seeds x_m P_mm(mu) of every order (x_m = sqrt 2 for m > 0): the FP64 diagonal chain of arithmetic.py:95-97
+// (1 / sqrt(4 pi) times the products of dd[j] sin theta), times x_m; seed(m, value) for m = 0 .. L.
+template <int L, typename Out>
+__device__ inline void seeds_x(double mu, const double *dd, Out seed) {
+    const double st = sqrt(fmax(0.0, 1.0 - mu * mu));
+    double diag = 0.28209479177387814;
+    seed(0, diag);
+    for (int m = 1; m <= L; ++m) {
+        diag = (diag * dd[m]) * st;
+        seed(m, 1.41421356237309515 * diag);
+    }
+}
+
+// The basis values x_m P_lm(mu) of the orders ma = p and mb = L - p (p < L / 2; p = L / 2: the one order) from
+// their seeds sa, sb (seeds_x), by the FP64 recurrence P_l = (aa mu) P_(l-1) - bb P_(l-2) (arithmetic.py:98-109),
+// side by side while both last, then the longer alone; every value rounded to FP32 goes to sink_a(k, z) (order ma)
+// or sink_b(k, z) (order mb) at its m-major position k. The two lengths add up to L + 2 for every p, so tasks over
+// p are balanced, and positions advance by one per step (no index arithmetic per value).
+template <int L, typename SinkA, typename SinkB>
+__device__ inline void basis_pair(int p, double mu, double sa, double sb, const double2 *__restrict__ ab, SinkA sink_a,
+                                  SinkB sink_b) {
+    const int ma = p, mb = L - p;
+    const bool two = mb != ma;
+    int ka = moff<L>(ma), kb = moff<L>(mb);
+    double a0 = sa, b0 = sb;
+    sink_a(ka, (float)a0);
+    if (two) sink_b(kb, (float)b0);
+    double a1 = (sqrt((double)(2 * ma + 3)) * mu) * a0;  // ma < L always
+    sink_a(ka + 1, (float)a1);
+    double b1 = 0.0;
+    if (two && mb < L) {
+        b1 = (sqrt((double)(2 * mb + 3)) * mu) * b0;
+        sink_b(kb + 1, (float)b1);
+    }
+    ka += 2;
+    kb += 2;
+    const int na = L - ma - 1, nb = two ? L - mb - 1 : 0;  // values still to come in each chain (l = m + 2 .. L)
+    int i = 0;
+    for (; i < nb; ++i, ++ka, ++kb) {
+        const double2 ca = ab[ka], cb = ab[kb];
+        const double a2 = (ca.x * mu) * a1 - ca.y * a0;
+        const double b2 = (cb.x * mu) * b1 - cb.y * b0;
+        sink_a(ka, (float)a2);
+        sink_b(kb, (float)b2);
+        a0 = a1; a1 = a2;
+        b0 = b1; b1 = b2;
+    }
+    for (; i < na; ++i, ++ka) {
+        const double2 ca = ab[ka];
+        const double a2 = (ca.x * mu) * a1 - ca.y * a0;
+        sink_a(ka, (float)a2);
+        a0 = a1; a1 = a2;
     }
 }
 
 // Corner c of a node with base cell b (ir * nc + j): b, b + 1, b + nc, b + nc + 1, the transport's corner order.
 __device__ inline int corner_cell(int b, int c, int nc) { return b + (c & 1) + (c >> 1) * nc; }
 
-// Basis table of a bundle: one workgroup per 16 rays builds their rows fp32(x_m P_lm(mu)) with the FP64
-// recurrence (build_rows with cos(m phi) = 1, exactly: the Chebyshev chain of cp = 1 is 1 at every m) in LDS and
-// writes them out row by row, coalesced.
+// Basis table of a bundle: one workgroup per 16 rays builds their rows fp32(x_m P_lm(mu)) with the FP64 recurrence
+// (basis_pair over the tasks (ray, p)) in LDS and writes them out row by row, coalesced.
 template <int L>
 __global__ __launch_bounds__(256) void plan_basis_kernel(int64_t rays, const double *dirs, float *basis) {
     constexpr int K = (L + 1) * (L + 2) / 2;
@@ -140,26 +114,28 @@ __global__ __launch_bounds__(256) void plan_basis_kernel(int64_t rays, const dou
     constexpr int V = 16;
     constexpr int NT = 256;
     __shared__ __attribute__((aligned(16))) float Z[V][KZ];
-    __shared__ double aa[K], bb[K];
+    __shared__ double2 ab[K];
     __shared__ double seed[L + 1][V];
-    __shared__ float cmv[L + 1][V];
     __shared__ double smu[V];
     __shared__ double dd[L + 1];
     const int tid = threadIdx.x;
     const int64_t r0 = (int64_t)blockIdx.x * V;
     const int nv = (int)min((int64_t)V, rays - r0);
-    stage_constants<L>(aa, bb, NT);
+    stage_ab<L>(ab, NT);
     stage_dd<L>(dd, NT);
     for (int e = tid; e < V * (KZ - K); e += NT) Z[e / (KZ - K)][K + e % (KZ - K)] = 0.0f;
     __syncthreads();
     if (tid < V) {
         const double mu = tid < nv ? fmin(1.0, fmax(-1.0, dirs[3 * (r0 + tid) + 2])) : 0.0;
         smu[tid] = mu;
-        visit_seeds<L, V>(tid, mu, 1.0f, dd, seed, cmv);
+        seeds_x<L>(mu, dd, [&](int m, double x) { seed[m][tid] = x; });
     }
     __syncthreads();
-    build_rows<L, V, 2, 4, double>(smu, seed, cmv, aa, bb, nv, tid, NT, V * (L / 2 + 1),
-                                   [&](int v, int k, float z) { Z[v][k] = z; });
+    for (int task = tid; task < V * (L / 2 + 1); task += NT) {
+        const int v = task % V, p = task / V;
+        auto put = [&](int k, float z) { Z[v][k] = z; };
+        basis_pair<L>(p, smu[v], seed[p][v], seed[L - p][v], ab, put, put);
+    }
     __syncthreads();
     for (int e = tid; e < nv * (KZ / 4); e += NT) {
         const int v = e / (KZ / 4), q = e % (KZ / 4);
@@ -999,6 +975,231 @@ void sh_records_split_quad_kernel(ShArgs a) {
                             v0, v1, corner, col, kh, wc, g0, ng);
 }
 
+// Split FP16 of a tile chunk with the low part unscaled: hi = z truncated to 11 significant bits (exact in FP16 for
+// |z| in FP16's normal range), lo = z - hi (exact in FP32, rounded or flushed in FP16: at most 2^-14 absolute, below
+// 2^-28 of a tile scaled to a largest entry near 2^15).
+__device__ inline void split_trunc_unscaled(float4 z, half4_t &h, half4_t &l) {
+    const float hx = __uint_as_float(__float_as_uint(z.x) & 0xffffe000u);
+    const float hy = __uint_as_float(__float_as_uint(z.y) & 0xffffe000u);
+    const float hz = __uint_as_float(__float_as_uint(z.z) & 0xffffe000u);
+    const float hw = __uint_as_float(__float_as_uint(z.w) & 0xffffe000u);
+    h = half4_t{(_Float16)hx, (_Float16)hy, (_Float16)hz, (_Float16)hw};
+    l = half4_t{(_Float16)(z.x - hx), (_Float16)(z.y - hy), (_Float16)(z.z - hz), (_Float16)(z.w - hw)};
+}
+
+// Split-FP16 products over the whole K range (KS steps of 32) into three accumulators: hi hi, hi lo (the plane's lo
+// is scaled by 2^11) and lo hi (the tile's lo is unscaled); the A operands read D steps ahead, the schedule pinned
+// to (three products, two reads) per step.
+template <int KS, int ZS, int D>
+__device__ inline floatx4 split_products_full(const _Float16 *Zh, const _Float16 *Zl, int tile, const half8_t *bh,
+                                              const half8_t *bl, int lane) {
+    floatx4 am = floatx4{0.f, 0.f, 0.f, 0.f}, a1 = am, a2 = am;
+    int zrow = tile + (lane & 15) * ZS + 8 * (lane >> 4);
+    asm volatile("" : "+v"(zrow));  // one opaque offset: each read one constant offset away from Zh / Zl
+    half8_t ph[D], pl[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        ph[j] = *reinterpret_cast<const half8_t *>(Zh + (zrow + 32 * j));
+        pl[j] = *reinterpret_cast<const half8_t *>(Zl + (zrow + 32 * j));
+        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < KS; ++i) {
+        const half8_t ah = ph[i % D], al = pl[i % D];
+        if (i + D < KS) {
+            ph[i % D] = *reinterpret_cast<const half8_t *>(Zh + (zrow + 32 * (i + D)));
+            pl[i % D] = *reinterpret_cast<const half8_t *>(Zl + (zrow + 32 * (i + D)));
+        }
+        am = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[i], am, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[i], a1, 0, 0, 0);
+        a2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[i], a2, 0, 0, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
+        if (i + D < KS) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+    }
+    return (am + a2) + a1 * (1.0f / 2048.0f);
+}
+
+// Split-FP16 SH records, staggered form (32 lanes; L = 16, 24, 32): 8 wavefronts, wavefront (corner, 16-lane tile)
+// holding the hi / lo B operands of its whole K range (KS steps of 32, 8 KS registers), so no partial products
+// cross wavefronts. The wavefronts form two groups of four with one wavefront of each per SIMD (a workgroup's
+// wavefronts are placed on the SIMDs round robin): group A (corners 0 and 1) runs the products and records of round
+// r and then forms its chunks of tile r + 1; group B (corners 2 and 3) forms its chunks of tile r + 1 first and then
+// runs the products and records of round r. On every SIMD one wavefront's tile formation (VALU) thus runs under the
+// other's products (MFMA). Tiles and basis rows are double-buffered: a thread fetches its chunks of the rows of round
+// r + 2 (LDS-direct loads, the row-block layout of quad_fetch_rows_rb) right after forming its chunks of tile r + 1
+// from those of round r + 1, so every fetch has about a round to land and each thread waits only for its own loads.
+// Node data (record rows, rays, cos(m phi) 2^13) is prepared three rounds ahead in four slots by one wavefront of
+// group A, from node ids, rays and cosines it fetched a round earlier. Tile formation per chunk of four positions:
+// z = row x cos(m phi) 2^13 with at most two orders m per chunk (two cosine reads and a per-position select), hi and
+// lo by split_trunc_unscaled. Records = (hi hi + lo hi + hi lo 2^-11) x plane scale / 2^13.
+template <int L>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void sh_records_split_stagger_kernel(ShArgs a) {
+    constexpr int G = 32;
+    constexpr int K = (L + 1) * (L + 2) / 2;
+    constexpr int KZ = (K + 15) / 16 * 16;
+    constexpr int KS = (KZ + 31) / 32;      // K = 32 steps
+    constexpr int ZK = 32 * KS;             // tile positions; KZ .. ZK - 1 stay zero
+    constexpr int ZS = ZK + 8;              // halves per tile row: rows 4 banks apart
+    constexpr int V = 16;                   // nodes per round
+    constexpr int NT = 512;
+    constexpr int KG = NT / 256, NGR = raw_groups<KZ>(), PASSES = (NGR + KG - 1) / KG;
+    constexpr int MW = 3;                   // the node-data wavefront (group A)
+    constexpr float ZSCALE = 8192.0f;
+    __shared__ __attribute__((aligned(16))) _Float16 Zt[2][2][V][ZS];  // tile buffer, hi / lo, node, position
+    __shared__ float4 raw[2][NGR * 256];
+    __shared__ RoundMeta<L, V, 4> mt;
+    __shared__ int ndata[4][3 * V];
+    __shared__ __attribute__((aligned(4))) unsigned char km[KZ];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool group_b = wave >= 4;
+    const int corner = 2 * (wave >> 2) + ((wave >> 1) & 1), nt = wave & 1;
+    const int col = nt * 16 + (lane & 15);
+    const int64_t item = blockIdx.x;
+    const int cell = corner_cell(a.item_cell[item], corner, a.nc);
+    const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
+    const int rounds = (int)((v1 - v0 + V - 1) / V);
+    stage_km<L, KZ>(km, NT);
+    if constexpr (ZK > KZ) {
+        for (int e = tid; e < 2 * V * (ZK - KZ); e += NT) {
+            const int b = e / (V * (ZK - KZ)), rem = e % (V * (ZK - KZ));
+            Zt[b][0][rem / (ZK - KZ)][KZ + rem % (ZK - KZ)] = (_Float16)0.0f;
+            Zt[b][1][rem / (ZK - KZ)][KZ + rem % (ZK - KZ)] = (_Float16)0.0f;
+        }
+    }
+    const float scale = a.scale[(int64_t)cell * G + col];
+    // bh[i], bl[i]: K positions 32 i + 8 (lane / 16) + 0..7 of column col (quads 8 i + 2 (lane / 16) and the next of
+    // the layout [cell][KZ/4][G][4]); zero past KZ
+    half8_t bh[KS], bl[KS];
+    {
+        const half4_t *ph = reinterpret_cast<const half4_t *>(a.plane) + (int64_t)cell * (KZ / 4) * G + col;
+        const half4_t *pl = ph + a.plane_lo / 4;
+#pragma unroll
+        for (int i = 0; i < KS; ++i) {
+            const int q = 8 * i + 2 * (lane >> 4);
+            half4_t h0{}, h1{}, l0{}, l1{};
+            if (q < KZ / 4) {
+                h0 = ph[(int64_t)q * G];
+                l0 = pl[(int64_t)q * G];
+            }
+            if (q + 1 < KZ / 4) {
+                h1 = ph[(int64_t)(q + 1) * G];
+                l1 = pl[(int64_t)(q + 1) * G];
+            }
+            bh[i] = half8_t{h0[0], h0[1], h0[2], h0[3], h1[0], h1[1], h1[2], h1[3]};
+            bl[i] = half8_t{l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3]};
+        }
+    }
+    const int32_t *node = a.node, *node_ray = a.node_ray, *node_cp = reinterpret_cast<const int32_t *>(a.node_cp);
+    const bool meta = wave == MW && lane < V;
+    const bool nlane = wave == MW && lane < 3 * V;  // the node-data loaders
+    const int64_t ndel = lane < V ? 0 : lane < 2 * V ? (int64_t)(node_ray - node) : (int64_t)(node_cp - node);
+    if (tid < 3 * V) {  // node data of rounds 0, 1, 2
+        const NodeRef n = node_ref(a, v0 + tid, v1);
+        meta_fill<L, V, 4>(mt, tid / V, tid % V, n.id >= 0, n.id, n.ray, n.cp, ZSCALE);
+    }
+    if (rounds > 3) quad_fetch_nodes<V>(&ndata[3][0], node + ndel, v0 + 3 * V, v1, nlane, lane);
+    __syncthreads();
+    // per thread and pass: its chunk's row, the cosine offsets of the chunk's first order mA and other order mB
+    // (mB = mA for a chunk of one order) and the positions that take mB (bit 4 j + e); positions past K read mA
+    const int rb = wave & 3, vrow = 4 * rb + (lane >> 4);
+    int offa[PASSES], offb[PASSES];
+    unsigned sel = 0;
+#pragma unroll
+    for (int j = 0; j < PASSES; ++j) {
+        const int k4 = 16 * (KG * j + (wave >> 2)) + (lane & 15);
+        int ma = 0, mb = 0;
+        if (k4 < KZ / 4) {
+            ma = km[4 * k4];
+            mb = ma;
+#pragma unroll
+            for (int e = 1; e < 4; ++e)
+                if (4 * k4 + e < K && km[4 * k4 + e] != ma) {
+                    mb = km[4 * k4 + e];
+                    sel |= 1u << (4 * j + e);
+                }
+        }
+        offa[j] = vrow * (L + 1) + ma;
+        offb[j] = vrow * (L + 1) + mb;
+    }
+    // this thread's chunks of tile q from its chunks of the rows of round q
+    auto form = [&](int q) {
+        const float *cmr = &mt.cm[q & 3][0][0];
+        const float4 *rw = raw[q & 1];
+        _Float16 *zh = &Zt[q & 1][0][vrow][0], *zl = &Zt[q & 1][1][vrow][0];
+#pragma unroll
+        for (int j = 0; j < PASSES; ++j) {
+            const int grp = KG * j + (wave >> 2);
+            const int k4 = 16 * grp + (lane & 15);
+            if (grp < NGR && k4 < KZ / 4) {
+                const float4 p = rw[(grp * 4 + rb) * 64 + lane];
+                const float ca = cmr[offa[j]], cb = cmr[offb[j]];
+                const unsigned s = sel >> (4 * j);
+                const float4 z = float4{p.x * ca, p.y * ((s & 2u) ? cb : ca), p.z * ((s & 4u) ? cb : ca),
+                                        p.w * ((s & 8u) ? cb : ca)};
+                half4_t h, l;
+                split_trunc_unscaled(z, h, l);
+                reinterpret_cast<half4_t *>(zh)[k4] = h;
+                reinterpret_cast<half4_t *>(zl)[k4] = l;
+            }
+        }
+    };
+    auto store = [&](int q, floatx4 d) {
+        const int64_t p0 = v0 + (int64_t)q * V;
+        const int nvp = (int)min((int64_t)V, v1 - p0);
+        const int *ids = mt.id[q & 3];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = 4 * (lane >> 4) + v;
+            if (row < nvp) a.records[((int64_t)ids[row] * 4 + corner) * G + col] = (d[v] * scale) * (1.0f / ZSCALE);
+        }
+    };
+    const float *basis = a.basis;
+    if (rounds > 0) quad_fetch_rows_rb<V, KZ, NT>(raw[0], basis, mt.ray[0], tid);
+    if (rounds > 1) quad_fetch_rows_rb<V, KZ, NT>(raw[1], basis, mt.ray[1], tid);
+    if (rounds > 0) {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this thread's rows of round 0 (and 1) are in raw
+        form(0);
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    for (int r = 0; r < rounds; ++r) {
+        const bool next = r + 1 < rounds;
+        if (group_b) {
+            if (next) {
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this thread's rows of round r + 1 are in raw
+                form(r + 1);
+                // the reads of raw[(r + 1) & 1] have returned (their values are in the tile): raw[r & 1] was read
+                // in round r - 1, so the rows of round r + 2 may land there
+                if (r + 2 < rounds) quad_fetch_rows_rb<V, KZ, NT>(raw[r & 1], basis, mt.ray[(r + 2) & 3], tid);
+            }
+            store(r, split_products_full<KS, ZS, 2>(&Zt[r & 1][0][0][0], &Zt[r & 1][1][0][0], 0, bh, bl, lane));
+        } else {
+            store(r, split_products_full<KS, ZS, 2>(&Zt[r & 1][0][0][0], &Zt[r & 1][1][0][0], 0, bh, bl, lane));
+            if (next) {
+                // this thread's rows of round r + 1 and (node-data wavefront) the node data of round r + 3, both
+                // fetched in round r - 1, have landed
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                form(r + 1);
+                if (wave == MW && r + 3 < rounds) {
+                    // node data of round r + 3 into slot (r + 3) & 3, whose round r - 1 was last read by the records
+                    // of round r - 1; then the node data of round r + 4 into slot r & 3 (read in round r - 3)
+                    if (meta) {
+                        const int *nd = &ndata[(r + 3) & 3][0];
+                        meta_fill<L, V, 4>(mt, (r + 3) & 3, lane, v0 + (int64_t)(r + 3) * V + lane < v1, nd[lane],
+                                           nd[V + lane], __int_as_float(nd[2 * V + lane]), ZSCALE);
+                    }
+                    if (r + 4 < rounds)
+                        quad_fetch_nodes<V>(&ndata[r & 3][0], node + ndel, v0 + (int64_t)(r + 4) * V, v1, nlane, lane);
+                }
+                if (r + 2 < rounds) quad_fetch_rows_rb<V, KZ, NT>(raw[r & 1], basis, mt.ray[(r + 2) & 3], tid);
+            }
+        }
+        // LDS-only barrier: tile r + 1 and the node data are complete; the loads into raw stay in flight
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // no load into LDS outlives the workgroup
+}
+
 // One lane (W = 1): one thread per node, dot products of the node's basis row (table row times cos(m phi)) against
 // the four corner cells' coefficients staged in LDS (a split FP16 plane is recombined and scaled back to FP32 there).
 template <int L, bool HALF>
@@ -1054,6 +1255,15 @@ static int sh_launch(const ShArgs &a, bool half, hipStream_t s) {
         else hipLaunchKernelGGL((sh_records_scalar<L, false>), dim3((unsigned)a.items), dim3(256), 0, s, a);
     } else {
         if (half) {
+            if constexpr (G == 32 && L >= 16) {
+                if (a.tile != 1) {  // the staggered form unless the quad form (tile 1) is asked for
+                    hipLaunchKernelGGL((sh_records_split_stagger_kernel<L>), dim3((unsigned)a.items), dim3(512), 0, s,
+                                       a);
+                    hipError_t e = hipGetLastError();
+                    if (e != hipSuccess) return set_error("sh_records launch: %s", hipGetErrorString(e));
+                    return SOS_OK;
+                }
+            }
             if constexpr (((L + 1) * (L + 2) / 2 + 15) / 16 * 16 % 32 == 0)
                 hipLaunchKernelGGL((sh_records_split_quad_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 32), 0, s, a);
             else

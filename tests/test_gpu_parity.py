@@ -201,13 +201,16 @@ def test_basis_table_matches_oracle_basis(gpu, L):
     assert np.all(err <= 2.5e-7 * np.abs(ref) + 1e-30), err.max()
 
 
-@pytest.mark.parametrize('W,L', [(3, 16), (20, 32), (1, 16), (5, 24)])
-def test_mixed_records_match_fp32(gpu, W, L):
+@pytest.mark.parametrize('W,L,form', [(3, 16, 0), (20, 32, 0), (20, 32, 1), (20, 16, 0), (20, 24, 0), (1, 16, 0),
+                                       (5, 24, 0)])
+def test_mixed_records_match_fp32(gpu, W, L, form):
     """Split-FP16 SH records (three FP16 MFMA products, FP32 accumulation) against the FP32 records of the same
     field. Operands carry 22 bits (hi + lo 2^-11) and both are scaled into the top of FP16's range (small entries stay
     normal numbers), and the dropped lo lo product is 2^-22 relative, so the records agree with the FP32 contraction
     to a few 1e-7 of the largest record; an error in the lo scaling (2^-11), a dropped cross product or operands left
-    subnormal show up at 1e-5 or more."""
+    subnormal show up at 1e-5 or more. At 32 lanes (W = 20) the staggered form (form 0, the default: whole-K
+    wavefronts, its tile's lo part unscaled) and the quad form (form 1) are both checked; L = 24 pads the staggered
+    form's K range with zero steps."""
     s, scope, orc = setup_case(W, L)
     s16, _, _ = setup_case(W, L, mixed=True)
     k, k16 = s.kernel, s16.kernel
@@ -222,6 +225,7 @@ def test_mixed_records_match_fp32(gpu, W, L):
     dc16 = k16.constants(s16, db.nq)
     plane16 = k16.plane(s16, fd, dc16)
     assert plane16[0].shape == (2, s.geo.ns, (s.K + 15) // 16 * 4, dc.G, 4)
+    dc16.struct.sh_tile = form
     hip.call('sos_sh_records', dc16.struct, hip.ptr(plane16[0]), hip.ptr(plane16[1]), 1, pl.sh_items,
              hip.ptr(pl.sh_cell), hip.ptr(pl.sh_v0), hip.ptr(pl.sh_v1), hip.ptr(pl.node), hip.ptr(pl.sorted_ray),
              hip.ptr(pl.sorted_cp), hip.ptr(pl.basis_table(k, dc, db)), hip.ptr(pl.records), k16.stream())
@@ -230,7 +234,7 @@ def test_mixed_records_match_fp32(gpu, W, L):
     r32 = rec32[: 4 * pl.nodes].cpu().numpy().astype('f8')
     big = np.abs(r32).max()
     err = np.abs(r16 - r32)
-    print(f'W={W} L={L}: split-FP16 records vs FP32: max {err.max() / big:.3e}, RMS '
+    print(f'W={W} L={L} form {form}: split-FP16 records vs FP32: max {err.max() / big:.3e}, RMS '
           f'{np.sqrt(np.mean(err ** 2)) / big:.3e} of the largest record')
     assert err.max() / big < 2e-6
 
