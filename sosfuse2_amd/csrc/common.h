@@ -34,6 +34,14 @@ __device__ inline int interval(const float *__restrict__ a, int n, float x) {
 
 __device__ inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 
+// Base-2 logarithm and exponential on the hardware's instructions (v_log_f32, v_exp_f32, about 1 ulp; inputs below
+// FLT_MIN and results below it are flushed as everywhere here). The transport's log-space trace blend runs in base 2
+// (its scale (G / A)^W does not depend on the base) and its attenuations are exp2(-tau log2 e).
+constexpr float SOS_LOG2E = 1.44269504088896341f;
+__device__ inline float log2_hw(float x) { return __builtin_amdgcn_logf(x); }
+__device__ inline float exp2_hw(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ inline float exp_hw(float x) { return __builtin_amdgcn_exp2f(x * SOS_LOG2E); }
+
 // Henyey-Greenstein per steradian (numpy_reference.py:390-392).
 __device__ inline float hg(float u, float g) {
     float den = fmaxf(1e-20f, (1.0f + g * g) - (2.0f * g) * u);

@@ -325,7 +325,7 @@ __global__ __launch_bounds__(256) void transport_geom_kernel(TransportArgs a) {
             acc = acc + (((((g0.x * ext6[0] + g0.y * ext6[1]) + g0.z * ext6[2]) + g0.w * ext6[3]) + g1.x * ext6[4]) +
                          g1.y * ext6[5]);
             const float mid = (tau - tc) + acc;
-            const float att = g3.x * expf(-mid);
+            const float att = g3.x * exp_hw(-mid);
             const float ah = g3.y, ac = g3.z;
             const unsigned cb = __float_as_uint(g3.w);
             const int cell0 = (int)(cb & 0x7fffffffu);
@@ -362,14 +362,14 @@ __global__ __launch_bounds__(256) void transport_geom_kernel(TransportArgs a) {
                 }
                 const float tr = (tens[0] + tens[1]) + tens[2];
                 if (tr >= SOS_FLT_MIN) {
-                    logs = logs + sw[c] * logf(tr);
+                    logs = logs + sw[c] * log2_hw(tr);
                     wg = wg + sw[c];
                     am = am + sw[c] * tr;
                 }
             }
             const bool ok = (wg > 0.0f) && (am >= SOS_FLT_MIN);
-            const float mean = ok ? am / wg : 1.0f;
-            const float scale = expf(ok ? logs - wg * logf(mean) : 0.0f);
+            const float mean = ok ? am * __builtin_amdgcn_rcpf(wg) : 1.0f;
+            const float scale = exp2_hw(ok ? logs - wg * log2_hw(mean) : 0.0f);
 #pragma unroll
             for (int q = 0; q < P; ++q) {
                 const float y = (val[q] * scale) * att - corr[q];
@@ -396,7 +396,7 @@ __global__ __launch_bounds__(256) void transport_geom_kernel(TransportArgs a) {
 #pragma unroll
             for (int k = 0; k < 5; ++k) optical = optical + a.end_col[6 * ray + k] * lc[k];
             optical = optical + a.end_col[6 * ray + 5] * sext;
-            E = E * (expf(-optical) * (a.hits[ray] ? 1.0f : 0.0f));
+            E = E * (exp_hw(-optical) * (a.hits[ray] ? 1.0f : 0.0f));
 #pragma unroll
             for (int q = 0; q < P; ++q) total[q] = total[q] + (E * pu[q]) / 2.0f;
         }
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(256) void transport_first_geom_kernel(TransportArgs
             acc = acc + (((((g0.x * ext6[0] + g0.y * ext6[1]) + g0.z * ext6[2]) + g0.w * ext6[3]) + g1.x * ext6[4]) +
                          g1.y * ext6[5]);
             const float mid = (tau - tc) + acc;
-            const float att = g3.x * expf(-mid);
+            const float att = g3.x * exp_hw(-mid);
             const float ah = g3.y, ac = g3.z;
             const unsigned cb = __float_as_uint(g3.w);
             const int cell0 = (int)(cb & 0x7fffffffu);
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(256) void transport_first_geom_kernel(TransportArgs
             float optical = 0.0f;
 #pragma unroll
             for (int k = 0; k < 6; ++k) optical = optical + col[k] * (k < 5 ? lc[k] : sext);
-            const float sun = s1.z != 0.0f ? expf(-optical) : 0.0f;
+            const float sun = s1.z != 0.0f ? exp_hw(-optical) : 0.0f;
             const float prof0 = s1.w, prof1 = s2.x, stp = s2.y;
             const float *rec = a.ground_records + n * 4 * G + w;
             const float grec = ((sw[0] * rec[0] + sw[1] * rec[G]) + sw[2] * rec[2 * G]) + sw[3] * rec[3 * G];
@@ -699,7 +699,7 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
                     bprev = bnode;
                 }
                 const float mid = (tau - tc) + acc;
-                const float att = ds * expf(-mid);
+                const float att = ds * exp_hw(-mid);
                 const int nc = a.nc;
                 const int cells[4] = {ir * nc + j, ir * nc + j + 1, (ir + 1) * nc + j, (ir + 1) * nc + j + 1};
                 const float sw[4] = {(1.0f - ah) * (1.0f - ac), (1.0f - ah) * ac, ah * (1.0f - ac), ah * ac};
@@ -761,7 +761,7 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
                         col = col + w11 * c10[6 + k];
                         optical = optical + col * (k < 5 ? lc[k] : sext);
                     }
-                    const float sun = lit ? expf(-optical) : 0.0f;
+                    const float sun = lit ? exp_hw(-optical) : 0.0f;
                     // ground pools: exact paired HG phases (numpy_reference.py:526-557)
                     float mus = 0.0f, exd = 0.0f, eyd = 0.0f;
                     mus = (df0 * Q[2] + df1 * Q[5]) + df2 * Q[8];
@@ -922,14 +922,14 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
                         }
                         const float tr = (tens[0] + tens[1]) + tens[2];
                         if (tr >= SOS_FLT_MIN) {
-                            logs = logs + sw[c] * logf(tr);
+                            logs = logs + sw[c] * log2_hw(tr);
                             wg = wg + sw[c];
                             am = am + sw[c] * tr;
                         }
                     }
                     const bool ok = (wg > 0.0f) && (am >= SOS_FLT_MIN);
-                    const float mean = ok ? am / wg : 1.0f;
-                    const float scale = expf(ok ? logs - wg * logf(mean) : 0.0f);
+                    const float mean = ok ? am * __builtin_amdgcn_rcpf(wg) : 1.0f;
+                    const float scale = exp2_hw(ok ? logs - wg * log2_hw(mean) : 0.0f);
 #pragma unroll
                     for (int q = 0; q < P; ++q) val[q] = val[q] * scale;
                 }
@@ -965,7 +965,7 @@ __global__ __launch_bounds__(256) void transport_kernel(TransportArgs a) {
 #pragma unroll
             for (int k = 0; k < 5; ++k) optical = optical + a.end_col[6 * ray + k] * lc[k];
             optical = optical + a.end_col[6 * ray + 5] * sext;
-            E = E * (expf(-optical) * (a.hits[ray] ? 1.0f : 0.0f));
+            E = E * (exp_hw(-optical) * (a.hits[ray] ? 1.0f : 0.0f));
 #pragma unroll
             for (int q = 0; q < P; ++q) total[q] = total[q] + (E * pu[q]) / 2.0f;
         }
