@@ -292,7 +292,7 @@ __global__ __launch_bounds__(256) void transport_geom_kernel(TransportArgs a) {
     float lc[SOS_LANE_CONSTANTS];
 #pragma unroll
     for (int k = 0; k < SOS_LANE_CONSTANTS; ++k) lc[k] = a.lane[k * G + w];
-    const float mix = lc[SOS_LANE_MIX];
+    const float mix = lc[SOS_LANE_MIX], kmix = (1.0f - mix) / 3.0f;
     const float sext = a.strato ? lc[SOS_LANE_SEXT] : 0.0f;
     const float ext6[6] = {lc[0], lc[1], lc[2], lc[3], lc[4], sext};
     const int64_t nwg = gridDim.x;
@@ -314,6 +314,7 @@ __global__ __launch_bounds__(256) void transport_geom_kernel(TransportArgs a) {
             pu[q] = (i == k ? 1.0f : 0.0f) - dl[i] * dl[k];
         }
         (void)grp;
+        const float p3 = (pu[0] + pu[1]) + pu[2];
         float total[P], corr[P];
 #pragma unroll
         for (int q = 0; q < P; ++q) total[q] = corr[q] = 0.0f;
@@ -338,35 +339,41 @@ __global__ __launch_bounds__(256) void transport_geom_kernel(TransportArgs a) {
                 const float4 t4 = g[4 + q];
                 T[q][0] = t4.x; T[q][1] = t4.y; T[q][2] = t4.z; T[q][3] = t4.w;
             }
-            const float *rec = a.records + n * 4 * G + w;
-            float val[P];
+            // The node's rotation T is the same for its four corners, so the corners are blended before it is
+            // applied: with d the corner's air density, a its SH value and k = (1 - mix) / 3 trace(rm), a corner's
+            // tensor is tens[q] = d mix (T rm)[q] + pu[q] (d k + a / 2), so val[q] = sum_c sw tens[q] = mix (T R)[q]
+            // + pu[q] S with R = sum_c sw d rm and S = sum_c sw (d k + a / 2); each corner's trace for the blend is
+            // d (mix (t3 . rm) + k p3) + a p3 / 2 with t3 = T[0] + T[1] + T[2] and p3 = pu[0] + pu[1] + pu[2].
+            float t3[4];
 #pragma unroll
-            for (int q = 0; q < P; ++q) val[q] = 0.0f;
+            for (int uu = 0; uu < 4; ++uu) t3[uu] = (T[0][uu] + T[1][uu]) + T[2][uu];
+            const float *rec = a.records + n * 4 * G + w;
+            float R[4] = {0.0f, 0.0f, 0.0f, 0.0f}, S = 0.0f;
             float logs = 0.0f, wg = 0.0f, am = 0.0f;
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const float aero = rec[c * G];
                 const float *f = a.field + (int64_t)cells[c] * a.rows * G + w;
                 const float rm[4] = {f[0], f[G], f[2 * G], f[3 * G]};
-                const float trace = (rm[0] + rm[1]) + rm[2];
+                const float k = kmix * ((rm[0] + rm[1]) + rm[2]);
                 const float density = a.hr[ir + (c >> 1)];
-                float tens[P];
+                const float rot3 = ((t3[0] * rm[0] + t3[1] * rm[1]) + t3[2] * rm[2]) + t3[3] * rm[3];
+                const float half_a = aero * 0.5f;
+                const float tr = density * (mix * rot3 + k * p3) + half_a * p3;
+                const float sd = sw[c] * density;
 #pragma unroll
-                for (int q = 0; q < P; ++q) {
-                    float rot = 0.0f;
-#pragma unroll
-                    for (int uu = 0; uu < 4; ++uu) rot = rot + T[q][uu] * rm[uu];
-                    tens[q] = density * (mix * rot + ((1.0f - mix) / 3.0f * trace) * pu[q]);
-                    tens[q] = tens[q] + (aero * pu[q]) / 2.0f;
-                    val[q] = val[q] + sw[c] * tens[q];
-                }
-                const float tr = (tens[0] + tens[1]) + tens[2];
+                for (int uu = 0; uu < 4; ++uu) R[uu] = R[uu] + sd * rm[uu];
+                S = S + sw[c] * (density * k + half_a);
                 if (tr >= SOS_FLT_MIN) {
                     logs = logs + sw[c] * log2_hw(tr);
                     wg = wg + sw[c];
                     am = am + sw[c] * tr;
                 }
             }
+            float val[P];
+#pragma unroll
+            for (int q = 0; q < P; ++q)
+                val[q] = mix * (((T[q][0] * R[0] + T[q][1] * R[1]) + T[q][2] * R[2]) + T[q][3] * R[3]) + pu[q] * S;
             const bool ok = (wg > 0.0f) && (am >= SOS_FLT_MIN);
             const float mean = ok ? am * __builtin_amdgcn_rcpf(wg) : 1.0f;
             const float scale = exp2_hw(ok ? logs - wg * log2_hw(mean) : 0.0f);

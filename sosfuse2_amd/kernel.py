@@ -549,8 +549,14 @@ class Kernel:
                 if pairs == 4 and self.node_geometry:
                     pl.geometry(self, dc, db)
                     flags |= hip.NODE_GEOMETRY
+        rows = ctx.rows
+        if not first and (flags & hip.RECORDS) and self.tuning.get('tensor_rows', True) and field.shape[1] > 10:
+            # with records the warm pass reads only the tensor rows of the field: a compact copy [cells][10][lanes]
+            # keeps every corner's rows within 1.3 KB of each other instead of a full field row (4.5 KB at L = 32)
+            # apart, so the transport's corner reads stay in few pages and cache lines
+            field, rows = field[:, :10].contiguous(), 10
         e0 = self._event()
-        hip.call('sos_transport', dc.struct, db.struct, flags, pairs, ctx.rows,
+        hip.call('sos_transport', dc.struct, db.struct, flags, pairs, rows,
                  None if first else hip.ptr(field), None if first else hip.ptr(plane[0]), hip.ptr(boundary),
                  hip.ptr(records), hip.ptr(out), hip.ptr(self.counters), self.stream())
         if e0 is not None:
