@@ -104,7 +104,8 @@ typedef struct sos_grid {
     int32_t sh_tile;           /* sos_sh_records FP32 workgroup form: 0 flow (double-buffered basis tiles, one
                                   barrier per round of 16 nodes), 1 stepwise (seeds, basis and products separated by
                                   barriers), 2 quad (the flow's rounds on twice the wavefronts, each holding half of
-                                  the K range); a tuning knob, the records are bitwise the same. The split-FP16
+                                  the K range), 3 staggered quad (the two K halves form the next tile before and
+                                  after their products); a tuning knob, the records are bitwise the same. The split-FP16
                                   records (mixed precision) at 32 lanes and L = 16, 24, 32 take the staggered form
                                   (8 whole-K wavefronts, tile formation of one under the products of the other on
                                   every SIMD) unless sh_tile is 1, which takes the quad form; other lane counts the

@@ -48,16 +48,18 @@ __device__ inline float hg(float u, float g) {
     return (1.0f - g * g) / (12.566370614359172f * (den * sqrtf(den)));
 }
 
-// Henyey-Greenstein with the constants of one asymmetry hoisted: c = (1 + g^2, 2 g, (1 - g^2) / (4 pi)); den^-3/2
-// as rsq^3 (v_rsq_f32, about 1 ulp: the value agrees with hg to a few 1e-7 relative) instead of a correctly rounded
-// square root and division. For the inner loops that evaluate many phases of one asymmetry.
+// The paired Henyey-Greenstein phase (hg(up) + hg(dn)) / 2 of numpy_reference.py:526-558 with the constants of one
+// asymmetry hoisted, c = (1 + g^2, 2 g, (1 - g^2) / (4 pi)), and den^-3/2 as rsq^3 (v_rsq_f32, about 1 ulp: the value
+// agrees with hg to a few 1e-7 relative) instead of a correctly rounded square root and division; for |up|, |dn| <= 1
+// and |g| < 1 the denominator 1 + g^2 - 2 g u >= (1 - |g|)^2 > 0 needs no floor, and the half norm is applied once.
+// For the inner loops that evaluate many phases of one asymmetry.
 struct HgConst {
     float a, b, norm;
 };
 __device__ inline HgConst hg_const(float g) { return HgConst{1.0f + g * g, 2.0f * g, (1.0f - g * g) / 12.566370614359172f}; }
-__device__ inline float hg_fast(float u, const HgConst &c) {
-    const float r = rsqrtf(fmaxf(1e-20f, c.a - c.b * u));
-    return c.norm * ((r * r) * r);
+__device__ inline float hg_pair(float up, float dn, const HgConst &c) {
+    const float ru = rsqrtf(c.a - c.b * up), rd = rsqrtf(c.a - c.b * dn);
+    return (0.5f * c.norm) * ((ru * ru) * ru + (rd * rd) * rd);
 }
 
 // Height profiles of the five components (air, tropospheric aerosol, O3, NO2, O4) and the sulfate layer (index 5).

@@ -88,8 +88,8 @@ __global__ __launch_bounds__(64 * (G / 16) * 2) void ground_records_kernel(Groun
                         const float m0 = dv.x * g3[2] + dv.y * g3[0];
                         const float m1 = dv.z * g3[1];
                         const float up = clampf(m0 + m1, -1.0f, 1.0f), dn = clampf(m0 - m1, -1.0f, 1.0f);
-                        pt = (hg_fast(up, ht) + hg_fast(dn, ht)) * 0.5f;
-                        if (a.strato) ps = (hg_fast(up, hs) + hg_fast(dn, hs)) * 0.5f;
+                        pt = hg_pair(up, dn, ht);
+                        if (a.strato) ps = hg_pair(up, dn, hs);
                     }
                     PT[v][p] = pt;
                     PU[v][p] = ps;

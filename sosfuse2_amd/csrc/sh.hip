@@ -459,7 +459,15 @@ __device__ inline void quad_fetch_rows_rb(float4 *raw, const float *basis, const
 // carries alias-scope information: the compiler's wait insertion can then tell the reads of Z, the node data and the
 // partials from the LDS-direct loads into raw and ndata still in flight, instead of waiting for those loads before
 // every LDS read.
-template <int G, int L>
+// STAGGER = false: every round forms tile r from the rows fetched in round r - 1, prepares node data two rounds
+// ahead, fetches the rows of round r + 1, then (after the barrier) runs the products of round r.
+// STAGGER = true: the two K halves run their rounds in opposite orders, one wavefront of each half per SIMD (a
+// workgroup's wavefronts are placed on the SIMDs round robin): the second half (kh = 1) forms its chunks of tile r + 1
+// at the start of round r and then runs the products of round r, the first half runs the products first and then
+// forms its chunks, so on every SIMD tile formation (VALU) runs under products (MFMA). A thread refetches its own
+// chunks of the rows as soon as it has formed them (one raw buffer, about a round to land), and node data is
+// prepared three rounds ahead. The products, their order and the records are those of the unstaggered rounds.
+template <int G, int L, bool STAGGER = false>
 __device__ __attribute__((always_inline)) inline void quad_rounds(
     const ShArgs &a, float *__restrict__ Z, float4 *__restrict__ raw, int *__restrict__ ndata,
     RoundMeta<L, 16, 8> *__restrict__ mtp, const unsigned char *__restrict__ km, floatx4 *__restrict__ red,
@@ -475,57 +483,83 @@ __device__ __attribute__((always_inline)) inline void quad_rounds(
     constexpr int ZS = KZ + 4;
     constexpr int SW = NT - 64;
     constexpr int NG = KQH;
+    constexpr int AHEAD = STAGGER ? 3 : 2;  // rounds between a round's node data and its tile
     RoundMeta<L, V, 8> &mt = *mtp;
     const int32_t *node = a.node, *node_ray = a.node_ray, *node_cp = reinterpret_cast<const int32_t *>(a.node_cp);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int rounds = (int)((v1 - v0 + V - 1) / V);
-    const bool meta = tid >= SW && tid < SW + V;
-    if (tid < 2 * V) {
+    // the node-data wavefront: the last one (first K half in the staggered rounds, which forms late)
+    const int mwave = STAGGER ? NC - 1 : NT / 64 - 1;
+    const bool meta = wave == mwave && lane < V;
+    if (tid < AHEAD * V) {
         const NodeRef n = node_ref(a, v0 + tid, v1);
         meta_fill<L, V, 8>(mt, tid / V, tid % V, n.id >= 0, n.id, n.ray, n.cp);
     }
-    const bool nlane = tid >= SW && lane < 3 * V;  // the node-data loaders
+    const bool nlane = wave == mwave && lane < 3 * V;  // the node-data loaders
     const int64_t ndel = lane < V ? 0 : lane < 2 * V ? (int64_t)(node_ray - node) : (int64_t)(node_cp - node);
-    if (rounds > 2) quad_fetch_nodes<V>(ndata + 2 * 3 * V, node + ndel, v0 + 2 * V, v1, nlane, lane);
+    if (rounds > AHEAD) quad_fetch_nodes<V>(ndata + (AHEAD & 3) * 3 * V, node + ndel, v0 + AHEAD * V, v1, nlane, lane);
     __syncthreads();
     const float *basis = a.basis;
-    if (rounds > 0) quad_fetch_rows_rb<V, KZ, NT>(raw, basis, mt.ray[0], tid);
+    // this thread's chunks of tile q (into Z[q & 1]) from its chunks of the rows of round q in raw
+    auto form = [&](int q) {
+        constexpr int KG = NT / 256, NGR = raw_groups<KZ>(), PASSES = (NGR + KG - 1) / KG;
+        int t = tid;
+        asm volatile("" : "+v"(t));  // positions recomputed per round (registers)
+        const int rb = (t >> 6) & 3, v = 4 * rb + ((t & 63) >> 4);
+        const float *cm = mt.cm[q & 7][v];
+        float *zr = Z + (q & 1) * V * ZS + v * ZS;
+#pragma unroll
+        for (int j = 0; j < PASSES; ++j) {
+            const int grp = KG * j + (t >> 8);
+            const int k4 = 16 * grp + (t & 15);
+            if (grp < NGR && k4 < KZ / 4) {
+                const float4 p = raw[(grp * 4 + rb) * 64 + (t & 63)];
+                const uchar4 m = reinterpret_cast<const uchar4 *>(km)[k4];
+                reinterpret_cast<float4 *>(zr)[k4] = float4{p.x * cm[m.x], p.y * cm[m.y], p.z * cm[m.z], p.w * cm[m.w]};
+            }
+        }
+    };
+    // node data of round q from ndata (fetched a round earlier), then the node data of round q + 1 fetched
+    auto prepare = [&](int q) {
+        if (meta && q < rounds) {
+            const int v = lane;
+            const int *nd = ndata + (q & 3) * 3 * V;
+            meta_fill<L, V, 8>(mt, q & 7, v, v0 + (int64_t)q * V + v < v1, nd[v], nd[V + v], __int_as_float(nd[2 * V + v]));
+        }
+        if (q + 1 < rounds) quad_fetch_nodes<V>(ndata + ((q + 1) & 3) * 3 * V, node + ndel, v0 + (int64_t)(q + 1) * V, v1, nlane, lane);
+    };
     floatx4 dprev = floatx4{0.f, 0.f, 0.f, 0.f};
+    if constexpr (STAGGER) {
+        if (rounds > 0) {
+            quad_fetch_rows_rb<V, KZ, NT>(raw, basis, mt.ray[0], tid);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this thread's rows of round 0 (and the node data)
+            form(0);
+            if (rounds > 1) quad_fetch_rows_rb<V, KZ, NT>(raw, basis, mt.ray[1], tid);
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    } else {
+        if (rounds > 0) quad_fetch_rows_rb<V, KZ, NT>(raw, basis, mt.ray[0], tid);
+    }
     for (int r = 0; r <= rounds; ++r) {
         const int cur = r & 1;
-        const int64_t r0 = v0 + (int64_t)r * V;
-        if (r < rounds) {
+        if constexpr (STAGGER) {
+            if (kh == 1 && r + 1 < rounds) {
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this thread's rows of round r + 1 are in raw
+                form(r + 1);
+                // the reads of raw above have returned (their values are in Z): the rows of round r + 2 may land
+                if (r + 2 < rounds) quad_fetch_rows_rb<V, KZ, NT>(raw, basis, mt.ray[(r + 2) & 7], tid);
+            }
+        } else if (r < rounds) {
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this thread's rows of round r are in raw
-            constexpr int KG = NT / 256, NGR = raw_groups<KZ>(), PASSES = (NGR + KG - 1) / KG;
-            int t = tid;
-            asm volatile("" : "+v"(t));  // positions recomputed per round (registers)
-            const int rb = (t >> 6) & 3, v = 4 * rb + ((t & 63) >> 4);
-            const float *cm = mt.cm[r & 7][v];
-            float *zr = Z + cur * V * ZS + v * ZS;
-#pragma unroll
-            for (int j = 0; j < PASSES; ++j) {
-                const int grp = KG * j + (t >> 8);
-                const int k4 = 16 * grp + (t & 15);
-                if (grp < NGR && k4 < KZ / 4) {
-                    const float4 p = raw[(grp * 4 + rb) * 64 + (t & 63)];
-                    const uchar4 m = reinterpret_cast<const uchar4 *>(km)[k4];
-                    reinterpret_cast<float4 *>(zr)[k4] = float4{p.x * cm[m.x], p.y * cm[m.y], p.z * cm[m.z], p.w * cm[m.w]};
-                }
-            }
-            if (meta && r + 2 < rounds) {  // from ndata, fetched during round r - 1 (or before the loop)
-                const int v = lane;
-                const int *nd = ndata + ((r + 2) & 3) * 3 * V;
-                meta_fill<L, V, 8>(mt, (r + 2) & 7, v, r0 + 2 * V + v < v1, nd[v], nd[V + v], __int_as_float(nd[2 * V + v]));
-            }
-            if (r + 3 < rounds)  // slot (r + 3) & 3 was last read in round r - 3
-                quad_fetch_nodes<V>(ndata + ((r + 3) & 3) * 3 * V, node + ndel, r0 + 3 * V, v1, nlane, lane);
+            form(r);
+            if (r + 2 < rounds) prepare(r + 2);  // from ndata, fetched during round r - 1 (or before the loop)
             // the reads of raw above have returned (their values are in Z): the rows of round r + 1 may land
             if (r + 1 < rounds) quad_fetch_rows_rb<V, KZ, NT>(raw, basis, mt.ray[(r + 1) & 7], tid);
+            // Workgroup barrier over the LDS only: the wavefronts share Z, the node data and the partials, all in
+            // LDS, and nothing in global memory, so the barrier waits for this wavefront's LDS operations and
+            // leaves its loads into raw in flight (a __syncthreads would wait for them too, as LDS writes).
         }
-        // Workgroup barrier over the LDS only: the wavefronts share Z, the node data and the partials, all in LDS,
-        // and nothing in global memory, so the barrier waits for this wavefront's LDS operations and leaves its
-        // loads into raw in flight (a __syncthreads would wait for them too, as LDS writes).
-        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        if constexpr (!STAGGER) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
         // records of round r - 1: the first half adds the second half's partials
         if (r > 0 && kh == 0) {
             const int64_t p0 = v0 + (int64_t)(r - 1) * V;
@@ -543,6 +577,18 @@ __device__ __attribute__((always_inline)) inline void quad_rounds(
             if (kh == 1) red[(cur * NC + wc) * 64 + lane] = d;
             else dprev = d;
         }
+        if constexpr (STAGGER) {
+            if (kh == 0 && r + 1 < rounds) {
+                // this thread's rows of round r + 1 and (node-data wavefront) the node data of round r + 3, both
+                // fetched in round r - 1, have landed
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                form(r + 1);
+                if (r + 3 < rounds) prepare(r + 3);
+                if (r + 2 < rounds) quad_fetch_rows_rb<V, KZ, NT>(raw, basis, mt.ray[(r + 2) & 7], tid);
+            }
+            // the partials, tile r + 1 and the node data are complete; the loads into raw stay in flight
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        }
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // no load into LDS outlives the workgroup
 }
@@ -555,7 +601,7 @@ __device__ __attribute__((always_inline)) inline void quad_rounds(
 // The second half's partial products of round r go through LDS and are added by the first half after the barrier
 // of round r + 1, so the sum is (half 0) + (half 1) as in sh_products. Tiles and partials are double-buffered; the
 // node ids of round r are read after the barrier of round r + 1, so node data takes eight slots.
-template <int G, int L>
+template <int G, int L, bool STAGGER>
 __global__ __launch_bounds__(G * 32) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void sh_records_quad_kernel(ShArgs a) {
     constexpr int K = (L + 1) * (L + 2) / 2;
@@ -595,7 +641,8 @@ void sh_records_quad_kernel(ShArgs a) {
                 b[4 * i + u] = (i < ng && k < K) ? pl[(int64_t)k * G] : 0.0f;
             }
     }
-    quad_rounds<G, L>(a, &Z[0][0][0], raw, &ndata[0][0], &mt, km, &red[0][0][0], b, v0, v1, corner, col, kh, wc, g0, ng);
+    quad_rounds<G, L, STAGGER>(a, &Z[0][0][0], raw, &ndata[0][0], &mt, km, &red[0][0][0], b, v0, v1, corner, col, kh, wc,
+                               g0, ng);
 }
 
 template <int G, int L>
@@ -1272,7 +1319,9 @@ static int sh_launch(const ShArgs &a, bool half, hipStream_t s) {
         else if (a.tile == 1)
             hipLaunchKernelGGL((sh_records_step_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 16), 0, s, a);
         else if (a.tile == 2)
-            hipLaunchKernelGGL((sh_records_quad_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 32), 0, s, a);
+            hipLaunchKernelGGL((sh_records_quad_kernel<G, L, false>), dim3((unsigned)a.items), dim3(G * 32), 0, s, a);
+        else if (a.tile == 3)
+            hipLaunchKernelGGL((sh_records_quad_kernel<G, L, true>), dim3((unsigned)a.items), dim3(G * 32), 0, s, a);
         else hipLaunchKernelGGL((sh_records_kernel<G, L>), dim3((unsigned)a.items), dim3(G * 16), 0, s, a);
     }
     hipError_t e = hipGetLastError();

@@ -398,7 +398,7 @@ class Kernel:
         self.metrics = dict(samples=0, launches=0)
         self.timed = None  # list of (name, start event, end event) while timing is on
         # launch shapes chosen by the autotuner (sosfuse2_amd.tune); results do not depend on them
-        self.tuning = dict(blocks_per_cu=0, node_chunk=1024, sh_tile=2)
+        self.tuning = dict(blocks_per_cu=0, node_chunk=1024, sh_tile=3)
         self.counters = None  # device uint64 [2] while counting is on
         # bundles of more quadrature nodes than this run in parts of whole groups, one visit plan at a time (each
         # part's plan is built for its launch and freed after it): about 2^27 nodes x (232 + 16 G) bytes of plan,

@@ -1,8 +1,9 @@
 """Autotuner of the launch shapes: the SH-records work-item size (nodes of one base cell per workgroup), the FP32
 SH-records workgroup form (flow: double-buffered basis tiles with one barrier per round, two wavefronts per SIMD;
 quad: the same rounds on four wavefronts per SIMD, each holding half of the K range, the next round's basis rows
-fetched straight into LDS; the stepwise form, seeds, basis and products separated by barriers, is never faster and
-is not timed by default) and the transport's persistent workgroups per CU. None changes a result (every
+fetched straight into LDS; staggered quad: the quad rounds with the two K halves forming the next tile before and
+after their products; the stepwise form, seeds, basis and products separated by barriers, is never faster and is not
+timed by default) and the transport's persistent workgroups per CU. None changes a result (every
 record and every ray is computed the same way whatever the shape), so the tuner times one source evaluation per
 candidate on the solver's own grid and lane count and keeps the fastest. Results are cached per (grid, L, lanes,
 precision) in memory and, when a path is given, in a JSON file.
@@ -18,7 +19,7 @@ from .kernel import lane_count
 
 NODE_CHUNKS = (256, 512, 1024, 2048)
 BLOCKS_PER_CU = (4, 8, 16)
-SH_TILES = (0, 2)
+SH_TILES = (0, 2, 3)
 _CACHE = {}
 
 
