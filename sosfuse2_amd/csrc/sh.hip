@@ -500,22 +500,44 @@ __device__ __attribute__((always_inline)) inline void quad_rounds(
     if (rounds > AHEAD) quad_fetch_nodes<V>(ndata + (AHEAD & 3) * 3 * V, node + ndel, v0 + AHEAD * V, v1, nlane, lane);
     __syncthreads();
     const float *basis = a.basis;
+    // per thread and pass: its chunk of four positions (row v, chunk k4 of the row-block layout), the cosine offsets
+    // of the chunk's first order mA and other order mB (a chunk holds at most two orders; mB = mA for one) and the
+    // positions that take mB (bit 4 j + e; positions past K read mA, their basis values are zero)
+    constexpr int KG = NT / 256, NGR = raw_groups<KZ>(), PASSES = (NGR + KG - 1) / KG;
+    const int rb = wave & 3, vrow = 4 * rb + (lane >> 4);
+    int offa[PASSES], offb[PASSES];
+    unsigned sel = 0;
+#pragma unroll
+    for (int j = 0; j < PASSES; ++j) {
+        const int k4 = 16 * (KG * j + (tid >> 8)) + (lane & 15);
+        int ma = 0, mb = 0;
+        if (k4 < KZ / 4) {
+            ma = km[4 * k4];
+            mb = ma;
+#pragma unroll
+            for (int e = 1; e < 4; ++e)
+                if (4 * k4 + e < K && km[4 * k4 + e] != ma) {
+                    mb = km[4 * k4 + e];
+                    sel |= 1u << (4 * j + e);
+                }
+        }
+        offa[j] = vrow * (L + 1) + ma;
+        offb[j] = vrow * (L + 1) + mb;
+    }
     // this thread's chunks of tile q (into Z[q & 1]) from its chunks of the rows of round q in raw
     auto form = [&](int q) {
-        constexpr int KG = NT / 256, NGR = raw_groups<KZ>(), PASSES = (NGR + KG - 1) / KG;
-        int t = tid;
-        asm volatile("" : "+v"(t));  // positions recomputed per round (registers)
-        const int rb = (t >> 6) & 3, v = 4 * rb + ((t & 63) >> 4);
-        const float *cm = mt.cm[q & 7][v];
-        float *zr = Z + (q & 1) * V * ZS + v * ZS;
+        const float *cmr = &mt.cm[q & 7][0][0];
+        float *zr = Z + (q & 1) * V * ZS + vrow * ZS;
 #pragma unroll
         for (int j = 0; j < PASSES; ++j) {
-            const int grp = KG * j + (t >> 8);
-            const int k4 = 16 * grp + (t & 15);
+            const int grp = KG * j + (tid >> 8);
+            const int k4 = 16 * grp + (lane & 15);
             if (grp < NGR && k4 < KZ / 4) {
-                const float4 p = raw[(grp * 4 + rb) * 64 + (t & 63)];
-                const uchar4 m = reinterpret_cast<const uchar4 *>(km)[k4];
-                reinterpret_cast<float4 *>(zr)[k4] = float4{p.x * cm[m.x], p.y * cm[m.y], p.z * cm[m.z], p.w * cm[m.w]};
+                const float4 p = raw[(grp * 4 + rb) * 64 + lane];
+                const float ca = cmr[offa[j]], cb = cmr[offb[j]];
+                const unsigned sb = sel >> (4 * j);
+                reinterpret_cast<float4 *>(zr)[k4] = float4{p.x * ca, p.y * ((sb & 2u) ? cb : ca),
+                                                            p.z * ((sb & 4u) ? cb : ca), p.w * ((sb & 8u) ? cb : ca)};
             }
         }
     };
