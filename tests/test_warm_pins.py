@@ -135,8 +135,8 @@ def _pool_context(state, qscale, L=8):
     return engine.Context(grid, optics, SimpleNamespace(L=L), sun, pools)
 
 
-@pytest.fixture(scope='module')
-def warm_case():
+def make_warm_case():
+    """The warm pins' state, context, scope and a concatenated pack of rays sampled from four cells."""
     st = atmosphere.State(aerosol_height_km=4.0, sulfate_aod550=0.02, no2_column=5e15, o4=True, aod550=0.15)
     ctx, scope = _context(st)
     nc = ctx.geo.nc
@@ -148,6 +148,11 @@ def warm_case():
     g = geometry.concat_packs(picks)
     g['end_columns'] = np.concatenate([p['end_columns'] for p in picks])
     return ctx, scope, g
+
+
+@pytest.fixture(scope='module')
+def warm_case():
+    return make_warm_case()
 
 
 def _pu(g):
