@@ -19,19 +19,27 @@ int set_error(const char *fmt, ...);
 // (16-lane N-tile nt, 16-visit M-tile mt) holds the cell's pool radiances of its lanes as the MFMA B operand in
 // registers (NCH chunks of PC pools, loaded once per item), so the pools are staged once however many there are;
 // per round and chunk the 32 x PC phase tile of each phase group is generated in LDS by all threads and multiplied.
+// Step 4 i + u of a chunk's products covers pool 16 i + 4 (lane / 16) + u, so one ds_read_b128 of the tile feeds four
+// MFMAs. The tile is generated only up to the chunk's pools rounded up to 16 (the products stop there); a thread
+// takes one visit (two at 16 lanes) and the pools p = 8 j + (thread mod 8), so its direction stays in registers and
+// no index is divided. Slots past the cell's pools and absent visits evaluate the phase of a zero direction (finite),
+// against a zero B or into a record never stored.
 template <int G, int NCH>
 __global__ __launch_bounds__(64 * (G / 16) * 2) void ground_records_kernel(GroundArgs a) {
     constexpr int NW = G / 16;               // N-tiles
     constexpr int NT = 64 * NW * 2;          // two M-tiles per N-tile
-    constexpr int V = 32, PC = 160, PS = PC + 4;
-    __shared__ float PT[V][PS], PU[V][PS];
-    __shared__ float gd[NCH * PC][3];
+    constexpr int V = 32, PC = 160, PS = PC + 4, NQ = PC / 16;
+    constexpr int VS = NT / 8, VT = V / VS;  // visits per pass of the tile generation, passes
+    __shared__ __attribute__((aligned(16))) float PT[V][PS];
+    __shared__ __attribute__((aligned(16))) float PU[V][PS];
+    __shared__ float4 gd[NCH * PC];
     __shared__ float4 sdir[V];
     __shared__ int sid[V];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nt = wave % NW, mt = wave / NW;
     const int col = nt * 16 + (lane & 15);
     const int kk = lane >> 4, row = mt * 16 + (lane & 15);
+    const int pl = tid & 7, vt = tid >> 3;
     const int64_t item = blockIdx.x;
     const int cell = a.item_cell[item];
     const int64_t v0 = a.item_v0[item], v1 = a.item_v1[item];
@@ -46,19 +54,20 @@ __global__ __launch_bounds__(64 * (G / 16) * 2) void ground_records_kernel(Groun
         for (int i = 0; i < a.groups; ++i)
             if (a.gg[i] == lg && (!a.strato || a.sgg[i] == lsg)) { mygroup = i; break; }
     }
-    // B operand: bq[ch][s] = gval[pool ch PC + 4 s + kk][col] (0 past the cell's pools)
+    // B operand: bq[ch][4 i + u] = gval[pool ch PC + 16 i + 4 kk + u][col] (0 past the cell's pools)
     float bq[NCH][PC / 4];
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch)
 #pragma unroll
-        for (int s = 0; s < PC / 4; ++s) {
-            const int pp = ch * PC + 4 * s + kk;
-            bq[ch][s] = pp < np ? a.gval[(p0 + pp) * G + col] : 0.0f;
-        }
-    for (int e = tid; e < NCH * PC * 3; e += NT) {
-        const int p = e / 3;
-        gd[p][e % 3] = p < np ? a.gdir[(p0 + p) * 3 + e % 3] : 0.0f;
-    }
+        for (int i = 0; i < NQ; ++i)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int pp = ch * PC + 16 * i + 4 * kk + u;
+                bq[ch][4 * i + u] = pp < np ? a.gval[(p0 + pp) * G + col] : 0.0f;
+            }
+    for (int p = tid; p < NCH * PC; p += NT)
+        gd[p] = p < np ? make_float4(a.gdir[(p0 + p) * 3], a.gdir[(p0 + p) * 3 + 1], a.gdir[(p0 + p) * 3 + 2], 0.0f)
+                       : make_float4(0.f, 0.f, 0.f, 0.f);
     for (int64_t r0 = v0; r0 < v1; r0 += V) {
         const int nv = (int)min((int64_t)V, v1 - r0);
         if (tid < V) {
@@ -72,37 +81,50 @@ __global__ __launch_bounds__(64 * (G / 16) * 2) void ground_records_kernel(Groun
             sdir[tid] = dv;
         }
         __syncthreads();
+        float4 dv[VT];
+#pragma unroll
+        for (int i = 0; i < VT; ++i) dv[i] = sdir[vt + VS * i];
         floatx4 acct = floatx4{0.f, 0.f, 0.f, 0.f}, accs = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) {
             if (ch >= chunks) break;
             const int n = min(PC, np - ch * PC);
+            const int n16 = (n + 15) & ~15;
             for (int grp = 0; grp < a.groups; ++grp) {
                 const HgConst ht = hg_const(a.gg[grp]), hs = hg_const(a.sgg[grp]);
-                for (int e = tid; e < V * PC; e += NT) {
-                    const int v = e / PC, p = e % PC;
-                    float pt = 0.0f, ps = 0.0f;
-                    if (v < nv && p < n) {
-                        const float4 dv = sdir[v];
-                        const float *g3 = gd[ch * PC + p];
-                        const float m0 = dv.x * g3[2] + dv.y * g3[0];
-                        const float m1 = dv.z * g3[1];
+#pragma unroll
+                for (int j = 0; j < PC / 8; ++j) {
+                    if (8 * j >= n16) break;  // uniform: the chunk's pools rounded up to 16
+                    const int p = 8 * j + pl;
+                    const float4 g3 = gd[ch * PC + p];
+#pragma unroll
+                    for (int i = 0; i < VT; ++i) {
+                        const float m0 = dv[i].x * g3.z + dv[i].y * g3.x;
+                        const float m1 = dv[i].z * g3.y;
                         const float up = clampf(m0 + m1, -1.0f, 1.0f), dn = clampf(m0 - m1, -1.0f, 1.0f);
-                        pt = hg_pair(up, dn, ht);
-                        if (a.strato) ps = hg_pair(up, dn, hs);
+                        PT[vt + VS * i][p] = hg_pair(up, dn, ht);
+                        if (a.strato) PU[vt + VS * i][p] = hg_pair(up, dn, hs);
                     }
-                    PT[v][p] = pt;
-                    PU[v][p] = ps;
                 }
                 __syncthreads();
                 // every wavefront runs the products (the MFMA needs the whole wavefront); lanes of other groups
                 // discard them
                 floatx4 t = floatx4{0.f, 0.f, 0.f, 0.f}, u = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int s = 0; s < PC / 4; ++s) {
-                    if (4 * s >= n) break;  // past the chunk's pools both operands are zero
-                    t = __builtin_amdgcn_mfma_f32_16x16x4f32(PT[row][4 * s + kk], bq[ch][s], t, 0, 0, 0);
-                    if (a.strato) u = __builtin_amdgcn_mfma_f32_16x16x4f32(PU[row][4 * s + kk], bq[ch][s], u, 0, 0, 0);
+                for (int i = 0; i < NQ; ++i) {
+                    if (16 * i >= n) break;  // past the chunk's pools both operands are zero
+                    const float4 at = *reinterpret_cast<const float4 *>(&PT[row][16 * i + 4 * kk]);
+                    t = __builtin_amdgcn_mfma_f32_16x16x4f32(at.x, bq[ch][4 * i], t, 0, 0, 0);
+                    t = __builtin_amdgcn_mfma_f32_16x16x4f32(at.y, bq[ch][4 * i + 1], t, 0, 0, 0);
+                    t = __builtin_amdgcn_mfma_f32_16x16x4f32(at.z, bq[ch][4 * i + 2], t, 0, 0, 0);
+                    t = __builtin_amdgcn_mfma_f32_16x16x4f32(at.w, bq[ch][4 * i + 3], t, 0, 0, 0);
+                    if (a.strato) {
+                        const float4 as = *reinterpret_cast<const float4 *>(&PU[row][16 * i + 4 * kk]);
+                        u = __builtin_amdgcn_mfma_f32_16x16x4f32(as.x, bq[ch][4 * i], u, 0, 0, 0);
+                        u = __builtin_amdgcn_mfma_f32_16x16x4f32(as.y, bq[ch][4 * i + 1], u, 0, 0, 0);
+                        u = __builtin_amdgcn_mfma_f32_16x16x4f32(as.z, bq[ch][4 * i + 2], u, 0, 0, 0);
+                        u = __builtin_amdgcn_mfma_f32_16x16x4f32(as.w, bq[ch][4 * i + 3], u, 0, 0, 0);
+                    }
                 }
                 if (grp == mygroup) {
                     acct = acct + t;
