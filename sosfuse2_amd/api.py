@@ -151,23 +151,21 @@ def create_configured(output, cfg, wavelengths, backend, state, block_size=256, 
 
 
 def stokes_from_tensor(comp, view):
-    """I, Q, U [rays, W, 3] from the six radiance tensor components in the observer frame. With
-    e_h = (-sin az, cos az, 0) and e_v = (-sin el cos az, -sin el sin az, cos el) spanning the plane normal to the view:
-    I = trace, Q = e_v.P.e_v - e_h.P.e_h, U = 2 e_v.P.e_h."""
-    el, az = view[:, 0], view[:, 1]
-    eh = np.stack([-np.sin(az), np.cos(az), np.zeros_like(az)], -1)
-    ev = np.stack([-np.sin(el) * np.cos(az), -np.sin(el) * np.sin(az), np.cos(el)], -1)
-    P = np.zeros(comp.shape[:2] + (3, 3))
-    c = comp.astype('f8')
-    P[..., 0, 0], P[..., 1, 1], P[..., 2, 2] = c[..., 0], c[..., 1], c[..., 2]
-    P[..., 0, 2] = P[..., 2, 0] = c[..., 3]
-    P[..., 0, 1] = P[..., 1, 0] = c[..., 4]
-    P[..., 1, 2] = P[..., 2, 1] = c[..., 5]
-    I = (c[..., 0] + c[..., 1]) + c[..., 2]
-    vPv = np.einsum('ni,nwij,nj->nw', ev, P, ev)
-    hPh = np.einsum('ni,nwij,nj->nw', eh, P, eh)
-    vPh = np.einsum('ni,nwij,nj->nw', ev, P, eh)
-    return np.stack([I, vPv - hPh, 2 * vPh], -1)
+    """I, Q, U [rays, W, 3] from the six radiance tensor components [rays, W, 6] in the observer frame, on the
+    components' device in their dtype (torch tensors; view [rays, 2] = elevation, azimuth in radians). With
+    e_h = (-sin az, cos az, 0) and e_v = (-sin el cos az, -sin el sin az, cos el) spanning the plane normal to the view
+    and P = [[c0, c4, c3], [c4, c1, c5], [c3, c5, c2]]: I = trace, Q = e_v.P.e_v - e_h.P.e_h, U = 2 e_v.P.e_h, the
+    quadratic forms written out over the six components (e_h has no z component)."""
+    el, az = view[:, 0, None], view[:, 1, None]
+    se, ce, sa, ca = torch.sin(el), torch.cos(el), torch.sin(az), torch.cos(az)
+    h0, h1 = -sa, ca
+    v0, v1, v2 = -se * ca, -se * sa, ce
+    c0, c1, c2, c3, c4, c5 = comp.unbind(-1)
+    I = (c0 + c1) + c2
+    vpv = (v0 * v0) * c0 + (v1 * v1) * c1 + (v2 * v2) * c2 + 2 * ((v0 * v2) * c3 + (v0 * v1) * c4 + (v1 * v2) * c5)
+    hph = (h0 * h0) * c0 + (h1 * h1) * c1 + 2 * (h0 * h1) * c4
+    vph = (v0 * h0) * c0 + (v1 * h1) * c1 + (v2 * h0) * c3 + (v0 * h1 + v1 * h0) * c4 + (v2 * h1) * c5
+    return torch.stack([I, vpv - hph, 2 * vph], -1)
 
 
 def make_table(s, source, hs, el, az, group=64, batch=25, log=print, stokes=True):
@@ -201,12 +199,17 @@ def make_table(s, source, hs, el, az, group=64, batch=25, log=print, stokes=True
         host, db = observer_bundle(s, [hs[j] for j in part], ee, aa)
         one = k.radiance(db, s, None, True, None, pairs)
         many = k.radiance(db, s, source, False, None, pairs, plane)
-        tot = (one.double() + many.double()).cpu().numpy()[:, : s.W]
+        # I (Q, U) and the solar scaling in FP64 on the device; the table leaves it in FP32
+        tot = (one.double() + many.double())[:, : s.W]
         if stokes:
-            v = stokes_from_tensor(tot, host.view)
+            view = getattr(host, 'view_dev', None)
+            if view is None or view.device != tot.device:
+                view = host.view_dev = torch.from_numpy(np.ascontiguousarray(host.view, 'f8')).to(tot.device)
+            v = stokes_from_tensor(tot, view)
         else:
             v = (tot[..., 0] + tot[..., 1]) + tot[..., 2]
-        v = v * (solar[None, :, None] if stokes else solar[None, :])
+        sol = torch.from_numpy(np.asarray(solar, 'f8')).to(tot.device)
+        v = (v * (sol[None, :, None] if stokes else sol[None, :])).float().cpu().numpy()
         off = 0
         for j, n in zip(part, host.sizes):
             spec = v[off : off + n].reshape(len(el), len(az), *v.shape[1:])

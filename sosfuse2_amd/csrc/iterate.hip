@@ -237,30 +237,35 @@ __global__ __launch_bounds__(256) void residual_kernel(int64_t cells, int rows, 
         }
 }
 
-// Block partial dot products: part[b][w][j] = sum over the block's range of f[slot][w][i] f[j][w][i].
+// Block partial dot products: part[b][w][j] = sum over chunk b of f[slot][w][i] f[j][w][i]. Block (b, w) streams
+// chunk b of lane w's residuals (contiguous in the lane-major ring): each thread reads f[slot] once per element and
+// every filled slot beside it, keeps the depth sums in registers (FP64), and the block reduces them once.
 __global__ __launch_bounds__(256) void gram_partial(int64_t n, int G, int depth, int slot, unsigned filled,
                                                     const float *__restrict__ ring, double *__restrict__ part) {
-    __shared__ double red[256];
+    constexpr int DMAX = 8;
+    __shared__ double red[4][DMAX];
+    const int w = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t per = (n + gridDim.x - 1) / gridDim.x;
     const int64_t i0 = blockIdx.x * per, i1 = min(n, i0 + per);
-    for (int w = 0; w < G; ++w) {
-        const float *fs = ring + ((int64_t)slot * G + w) * n;
-        for (int j = 0; j < depth; ++j) {
-            double s = 0.0;
-            if ((filled >> j) & 1u) {
-                const float *fj = ring + ((int64_t)j * G + w) * n;
-                for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) s += (double)fs[i] * (double)fj[i];
-            }
-            red[threadIdx.x] = s;
-            __syncthreads();
-            for (int st = 128; st > 0; st >>= 1) {
-                if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-                __syncthreads();
-            }
-            if (threadIdx.x == 0) part[((int64_t)blockIdx.x * G + w) * depth + j] = red[0];
-            __syncthreads();
-        }
+    const float *fs = ring + ((int64_t)slot * G + w) * n;
+    double s[DMAX];
+#pragma unroll
+    for (int j = 0; j < DMAX; ++j) s[j] = 0.0;
+    for (int64_t i = i0 + tid; i < i1; i += 256) {
+        const double a = fs[i];
+#pragma unroll
+        for (int j = 0; j < DMAX; ++j)
+            if (j < depth && ((filled >> j) & 1u)) s[j] += a * (double)ring[((int64_t)j * G + w) * n + i];
     }
+#pragma unroll
+    for (int j = 0; j < DMAX; ++j) {
+        double v = s[j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) red[wave][j] = v;
+    }
+    __syncthreads();
+    if (tid < depth) part[((int64_t)blockIdx.x * G + w) * depth + tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
 }
 
 __global__ void gram_finish(int blocks, int G, int depth, int slot, unsigned filled, const double *__restrict__ part,
@@ -323,12 +328,40 @@ __global__ void solve_kernel(int G, int depth, int m, const int *__restrict__ or
     for (int i = 0; i < m; ++i) wo[i] = (float)x[i];
 }
 
-__global__ void mix_kernel(int64_t elems, int G, int depth, int m, const float *__restrict__ ring,
-                           const int *__restrict__ order, const float *__restrict__ wts, float *__restrict__ x) {
+// x = sum_j ring[order[j]] wts[lane][j] in float4 groups of four lanes (G is a multiple of 4, so a group never
+// straddles a row); a lane whose weights are NaN (frozen) keeps x.
+__global__ __launch_bounds__(256) void mix_kernel(int64_t elems, int G, int depth, int m, const float *__restrict__ ring,
+                                                  const int *__restrict__ order, const float *__restrict__ wts,
+                                                  float *__restrict__ x) {
+    const int64_t n4 = elems * G / 4;
+    const float4 *r4 = reinterpret_cast<const float4 *>(ring);
+    float4 *x4 = reinterpret_cast<float4 *>(x);
+    const int g4 = G / 4;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n4; e += (int64_t)gridDim.x * blockDim.x) {
+        const int w0 = 4 * (int)(e % g4);
+        const float *wa = wts + w0 * depth, *wb = wa + depth, *wc = wb + depth, *wd = wc + depth;
+        const bool fa = isnan(wa[0]), fb = isnan(wb[0]), fc = isnan(wc[0]), fd = isnan(wd[0]);
+        if (fa && fb && fc && fd) continue;
+        float4 r = r4[(int64_t)order[0] * n4 + e];
+        float4 v = float4{r.x * wa[0], r.y * wb[0], r.z * wc[0], r.w * wd[0]};
+        for (int j = 1; j < m; ++j) {
+            r = r4[(int64_t)order[j] * n4 + e];
+            v = float4{v.x + r.x * wa[j], v.y + r.y * wb[j], v.z + r.z * wc[j], v.w + r.w * wd[j]};
+        }
+        if (fa || fb || fc || fd) {
+            const float4 o = x4[e];
+            v = float4{fa ? o.x : v.x, fb ? o.y : v.y, fc ? o.z : v.z, fd ? o.w : v.w};
+        }
+        x4[e] = v;
+    }
+}
+
+// The same for a lane count that is not a multiple of 4 (or unaligned buffers), one element per thread.
+__global__ void mix_scalar_kernel(int64_t elems, int G, int depth, int m, const float *__restrict__ ring,
+                                  const int *__restrict__ order, const float *__restrict__ wts, float *__restrict__ x) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < elems * G;
          e += (int64_t)gridDim.x * blockDim.x) {
-        const int w = (int)(e % G);
-        const float *wl = wts + w * depth;
+        const float *wl = wts + (int)(e % G) * depth;
         if (isnan(wl[0])) continue;
         float v = ring[(int64_t)order[0] * elems * G + e] * wl[0];
         for (int j = 1; j < m; ++j) v = v + ring[(int64_t)order[j] * elems * G + e] * wl[j];
@@ -382,15 +415,16 @@ int residual_launch(int64_t cells, int rows, int G, const float *g, const float 
 
 constexpr int GRAM_MAX_BLOCKS = 1024;
 
+// chunks per lane of gram_partial: about 64 Ki residuals each, at most 128 (the grid is chunks x lanes)
 static int gram_blocks(int64_t n) {
-    int blocks = (int)min((int64_t)GRAM_MAX_BLOCKS, (n + 4095) / 4096);
+    int blocks = (int)min((int64_t)128, (n + 65535) / 65536);
     return blocks < 1 ? 1 : blocks;
 }
 
 int gram_launch(int64_t n, int G, int depth, int slot, unsigned filled, const float *ring, double *work, double *gram,
                 hipStream_t s) {
     const int blocks = gram_blocks(n);
-    hipLaunchKernelGGL(gram_partial, dim3(blocks), dim3(256), 0, s, n, G, depth, slot, filled, ring, work);
+    hipLaunchKernelGGL(gram_partial, dim3(blocks, G), dim3(256), 0, s, n, G, depth, slot, filled, ring, work);
     hipLaunchKernelGGL(gram_finish, dim3(1), dim3(256), 0, s, blocks, G, depth, slot, filled, work, gram);
     return check("anderson_gram");
 }
@@ -423,7 +457,7 @@ int gram_static_launch(int64_t n, int G, int depth, int slot, unsigned filled, c
                          G, depth, (long long)GRAM_SCRATCH);
     double *work = scratch_address();
     if (!work) return set_error("anderson_gram_static: hipGetSymbolAddress of the scratch array failed");
-    hipLaunchKernelGGL(gram_partial, dim3(blocks), dim3(256), 0, s, n, G, depth, slot, filled, ring, work);
+    hipLaunchKernelGGL(gram_partial, dim3(blocks, G), dim3(256), 0, s, n, G, depth, slot, filled, ring, work);
     hipLaunchKernelGGL(gram_finish, dim3(1), dim3(256), 0, s, blocks, G, depth, slot, filled, work, gram);
     return check("anderson_gram_static");
 }
@@ -437,7 +471,12 @@ int solve_launch(int G, int depth, int m, const int *order, const double *gram, 
 int mix_launch(int64_t cells, int rows, int G, int depth, int m, const float *ring, const int *order, const float *w,
                float *x, hipStream_t s) {
     const int64_t elems = cells * rows;
-    hipLaunchKernelGGL(mix_kernel, dim3(blocks_for(elems * G)), dim3(256), 0, s, elems, G, depth, m, ring, order, w, x);
+    if (G % 4 == 0 && ((uintptr_t)ring % 16) == 0 && ((uintptr_t)x % 16) == 0)
+        hipLaunchKernelGGL(mix_kernel, dim3(blocks_for(elems * G / 4)), dim3(256), 0, s, elems, G, depth, m, ring,
+                           order, w, x);
+    else
+        hipLaunchKernelGGL(mix_scalar_kernel, dim3(blocks_for(elems * G)), dim3(256), 0, s, elems, G, depth, m, ring,
+                           order, w, x);
     return check("anderson_mix");
 }
 

@@ -152,7 +152,7 @@ class Solver:
         t = time.perf_counter()
         s, k = self.s, self.kernel
         dc = k.constants(s, self.geometry.bundle.nq)
-        plane = k.plane(s, field, dc)
+        plane = self._plane_of(field, take=True)
         _, flux = self.geometry.evaluate(s, field, phase=1, ground_only=True, plane=plane)
         boundary = flux + self.first_flux[: s.geo.nc]
         moment, _ = self.geometry.evaluate(s, field, phase=0, boundary=boundary, plane=plane)
@@ -162,10 +162,21 @@ class Solver:
         self.timings['apply_seconds'].append(time.perf_counter() - t)
         return out
 
+    def _plane_of(self, field, take=False):
+        """The SH plane of a field. The observer boxes of x and the evaluation of x read the same plane, so the
+        boxes keep it for the evaluation that follows (matched by tensor identity, used once)."""
+        last = getattr(self, '_last_plane', None)
+        if last is not None and last[0] is field:
+            plane = last[1]
+        else:
+            plane = self.kernel.plane(self.s, field, self.kernel.constants(self.s, self.geometry.bundle.nq))
+        self._last_plane = None if take else (field, plane)
+        return plane
+
     def boxes(self, field):
         """Observer monitor [stop + record solar elevations, 7, G] (`engine_core.py:228-238`)."""
         host, db = self._box
-        rad = self.kernel.radiance(db, self.s, field, False, None, 4)
+        rad = self.kernel.radiance(db, self.s, field, False, None, 4, self._plane_of(field))
         I = ((rad[..., 0] + rad[..., 1]) + rad[..., 2]).double().cpu().numpy()
         solar = np.r_[self.s.solar, np.full(self.G - self.s.W, self.s.solar[-1])] if self.G > self.s.W else self.s.solar
         I = I * solar[None, :]
@@ -254,6 +265,7 @@ class Solver:
                 hip.call('sos_anderson_mix', C, rows, G, D, m, hip.ptr(gring), hip.ptr(order_dev), hip.ptr(wts),
                          hip.ptr(x), k.stream())
         k.sync()
+        self._last_plane = None
         if not done[:W].all():
             raise RuntimeError(f'E_NOT_CONVERGED: {done[:W].sum()}/{W} within {tolerance}')
         return x, dict(evaluations=evaluations, tolerance=tolerance, converged=done[:W].tolist(), history=history,

@@ -168,3 +168,35 @@ def test_state_queue_packs_prepares_and_fails_per_chunk(tmp_path):
     assert q.stats['states'] + q.stats['failed'] == len(jobs)
     with pytest.raises(RuntimeError, match='E_QUEUE_CLOSED'):
         q.submit(base, [550.0], [-3.0])
+
+
+def test_stokes_of_pure_and_unpolarised_tensors():
+    """api.stokes_from_tensor (device-side I, Q, U of the table) against closed forms: an isotropic tensor is
+    unpolarised (I = 3, Q = U = 0); P = e e^T with e = e_v, e_h or (e_v + e_h)/sqrt 2 gives (1, 1, 0), (1, -1, 0) and
+    (1, 0, 1); a view-parallel term d d^T adds to I only. Random tensors agree with the oracle's arrangement of the
+    quadratic forms (oracle.solve.stokes)."""
+    import torch
+
+    from oracle.solve import stokes
+
+    rng = np.random.default_rng(3)
+    n = 64
+    view = np.column_stack([rng.uniform(-1.5, 1.5, n), rng.uniform(-np.pi, np.pi, n)])
+    el, az = view[:, 0], view[:, 1]
+    eh = np.stack([-np.sin(az), np.cos(az), np.zeros(n)], -1)
+    ev = np.stack([-np.sin(el) * np.cos(az), -np.sin(el) * np.sin(az), np.cos(el)], -1)
+    d = np.cross(eh, ev)
+
+    def comps(P):  # (xx, yy, zz, xz, xy, yz)
+        return np.stack([P[:, 0, 0], P[:, 1, 1], P[:, 2, 2], P[:, 0, 2], P[:, 0, 1], P[:, 1, 2]], -1)[:, None, :]
+
+    outer = lambda e: e[:, :, None] * e[:, None, :]
+    cases = [(np.broadcast_to(np.eye(3), (n, 3, 3)), (3, 0, 0)), (outer(ev), (1, 1, 0)), (outer(eh), (1, -1, 0)),
+             (outer((ev + eh) / np.sqrt(2)), (1, 0, 1)), (outer(ev) + 2 * outer(d), (3, 1, 0))]
+    tv = torch.from_numpy(view)
+    for P, want in cases:
+        got = api.stokes_from_tensor(torch.from_numpy(comps(P)), tv).numpy()[:, 0]
+        np.testing.assert_allclose(got, np.broadcast_to(want, got.shape), atol=1e-12)
+    comp = rng.normal(size=(n, 5, 6))
+    got = api.stokes_from_tensor(torch.from_numpy(comp), tv).numpy()
+    np.testing.assert_allclose(got, stokes(comp, view), rtol=1e-12, atol=1e-12)
