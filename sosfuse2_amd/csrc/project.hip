@@ -15,39 +15,46 @@ namespace sos {
 
 
 
+// Ground pass (flux only): FP64 in ray order, one thread per lane.
+template <int G>
+__device__ inline void project_flux_only(const ProjectArgs &a, int cell, int64_t r0, int64_t r1) {
+    const int w = threadIdx.x;
+    if (w >= G) return;
+    double s = 0.0;
+    for (int64_t r = r0; r < r1; ++r) {
+        const float *x = a.rad + (r * G + w) * 4;
+        const float tr = (x[0] + x[1]) + x[2];
+        s += (double)(tr * (a.weights[r] * fmaxf(0.0f, a.dloc[3 * r + 2])));
+    }
+    a.flux[(int64_t)cell * G + w] = (float)(s * (double)a.lane[SOS_LANE_ALBEDO * G + w] / SOS_PI);
+}
+
+// One workgroup per cell, its rays in rounds of 16, staged in LDS: Z (basis row x cos(m phi), into the l-major
+// moment order), V (trace x weight), and the radiance tensors, from which threads 0 .. 5 G - 1 add the tensor moments
+// and the flux term of each ray in ray order (FP64, the same order as a sequential pass over the cell's rays).
 template <int G, int L>
 __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
     constexpr int K = (L + 1) * (L + 2) / 2;
     constexpr int KT = (K + 15) / 16;
     constexpr int KP = KT * 16;
     constexpr int NT = G >= 16 ? G / 16 : 1;
-    constexpr int TPW = (KT + 3) / 4;  // harmonic tiles per wavefront
+    constexpr int TPW = (KT + 3) / 4;        // harmonic tiles per wavefront
+    constexpr int ZQ = 16 * (KP / 4);        // float4 chunks of a round's basis rows
+    constexpr int RQ = 16 * G;               // float4 radiance tensors of a round
     __shared__ float Z[16][KP];
     __shared__ float V[16][G];
+    __shared__ float4 Rd[16][G];
     __shared__ float cmv[16][L + 1];
-    __shared__ short kl[KP];            // l-major position kidx(l, m) of m-major position k' (K for the padding)
-    __shared__ unsigned char km[KP];    // m of k'
+    __shared__ float wr[16], wf[16];        // weight; weight x max(0, dloc_z)
+    __shared__ short kl[KP];                // l-major position kidx(l, m) of m-major position k' (K for the padding)
+    __shared__ unsigned char km[KP];        // m of k'
     const int cell = blockIdx.x;
     const int tid = threadIdx.x;
     const int64_t r0 = a.group_off[cell], r1 = a.group_off[cell + 1];
-    // tensor moments and flux, FP64 in ray order
-    if (tid < 5 * G) {
-        const int comp = tid / G, w = tid % G;
-        if (comp < 4 && !a.ground_only) {
-            double s = 0.0;
-            for (int64_t r = r0; r < r1; ++r) s += (double)(a.rad[(r * G + w) * 4 + comp] * a.weights[r]);
-            a.moments[((int64_t)cell * (4 + K) + comp) * G + w] = (float)s;
-        } else if (comp == 4) {
-            double s = 0.0;
-            for (int64_t r = r0; r < r1; ++r) {
-                const float *x = a.rad + (r * G + w) * 4;
-                const float tr = (x[0] + x[1]) + x[2];
-                s += (double)(tr * (a.weights[r] * fmaxf(0.0f, a.dloc[3 * r + 2])));
-            }
-            a.flux[(int64_t)cell * G + w] = (float)(s * (double)a.lane[SOS_LANE_ALBEDO * G + w] / SOS_PI);
-        }
+    if (a.ground_only) {
+        project_flux_only<G>(a, cell, r0, r1);
+        return;
     }
-    if (a.ground_only) return;
     for (int k = tid; k < KP; k += 256) {
         int m = 0;
         if (k < K)
@@ -55,6 +62,8 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
         km[k] = (unsigned char)m;
         kl[k] = (short)(k < K ? kidx(k - moff<L>(m) + m, m) : K);
     }
+    if constexpr (KP > K)
+        for (int e = tid; e < 16 * (KP - K); e += 256) Z[e / (KP - K)][K + e % (KP - K)] = 0.0f;
     const int wave = tid >> 6, lane = tid & 63;
     floatx4 acc[TPW][NT];
 #pragma unroll
@@ -64,15 +73,18 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
     float accv[(K + 255) / 256];
 #pragma unroll
     for (int i = 0; i < (K + 255) / 256; ++i) accv[i] = 0.f;
+    double tsum = 0.0;  // thread tid < 5 G: tensor moment tid / G (0 .. 3) or the flux term (4) of lane tid % G
     for (int64_t c0 = r0; c0 < r1; c0 += 16) {
         const int n = (int)min((int64_t)16, r1 - c0);
         if (tid < 16) {
             // cos(m phi) of the ray's azimuth about the sun axis (FP32 Chebyshev, arithmetic.py:112-119)
             const int i = tid;
-            float cp = 1.0f;
+            float cp = 1.0f, wt = 0.0f, dl = 0.0f;
             if (i < n) {
                 const int64_t r = c0 + i;
                 const double dx = a.dirs[3 * r], dz = a.dirs[3 * r + 2];
+                wt = a.weights[r];
+                dl = a.dloc[3 * r + 2];
                 const double st = sqrt(fmax(0.0, 1.0 - dz * dz));
                 cp = (float)(st > 1e-12 ? dx / fmax(st, 1e-30) : 1.0);
                 cp = clampf(cp, -1.0f, 1.0f);
@@ -85,21 +97,17 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
                 else { cm = (2.0f * cp) * c1m - c0m; c0m = c1m; c1m = cm; }
                 cmv[i][m] = cm;
             }
+            wr[i] = i < n ? wt : 0.0f;
+            wf[i] = i < n ? wt * fmaxf(0.0f, dl) : 0.0f;
         }
-        for (int e = tid; e < 16 * G; e += 256) {
+        for (int e = tid; e < RQ; e += 256) {
             const int i = e / G, w = e % G;
-            float v = 0.0f;
-            if (i < n) {
-                const int64_t r = c0 + i;
-                const float *x = a.rad + (r * G + w) * 4;
-                v = ((x[0] + x[1]) + x[2]) * a.weights[r];
-            }
-            V[i][w] = v;
+            Rd[i][w] = i < n ? reinterpret_cast<const float4 *>(a.rad)[(c0 + i) * G + w] : float4{0.f, 0.f, 0.f, 0.f};
         }
         __syncthreads();
         // basis rows: the ray's row of the bundle's basis table (fp32(x_m P_lm(mu))) times cos(m phi), into the
         // l-major order of the moments
-        for (int e = tid; e < 16 * (KP / 4); e += 256) {
+        for (int e = tid; e < ZQ; e += 256) {
             const int i = e / (KP / 4), q = e % (KP / 4);
             float4 z = float4{0.0f, 0.0f, 0.0f, 0.0f};
             if (i < n) z = reinterpret_cast<const float4 *>(a.basis + (c0 + i) * KP)[q];
@@ -107,11 +115,22 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int k = 4 * q + u;
-                Z[i][kl[k]] = zz[u] * cmv[i][km[k]];
+                if (k < K) Z[i][kl[k]] = zz[u] * cmv[i][km[k]];
             }
         }
-        if constexpr (KP > K)
-            for (int e = tid; e < 16 * (KP - K); e += 256) Z[e / (KP - K)][K + e % (KP - K)] = 0.0f;
+        for (int e = tid; e < 16 * G; e += 256) {
+            const int i = e / G, w = e % G;
+            const float4 x = Rd[i][w];
+            V[i][w] = ((x.x + x.y) + x.z) * wr[i];
+        }
+        if (tid < 5 * G) {
+            const int comp = tid / G, w = tid % G;
+            for (int i = 0; i < n; ++i) {
+                const float4 x = Rd[i][w];
+                if (comp == 4) tsum += (double)(((x.x + x.y) + x.z) * wf[i]);
+                else tsum += (double)((comp == 0 ? x.x : comp == 1 ? x.y : comp == 2 ? x.z : x.w) * wr[i]);
+            }
+        }
         __syncthreads();
         if constexpr (G >= 16) {
             const int col = lane & 15, kk = lane >> 4;
@@ -137,11 +156,15 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
                     for (int i = 0; i < n; ++i) accv[q] = accv[q] + Z[i][k] * V[i][0];
             }
         }
-        __syncthreads();
+    }
+    if (tid < 5 * G) {
+        const int comp = tid / G, w = tid % G;
+        if (comp < 4) a.moments[((int64_t)cell * (4 + K) + comp) * G + w] = (float)tsum;
+        else a.flux[(int64_t)cell * G + w] = (float)(tsum * (double)a.lane[SOS_LANE_ALBEDO * G + w] / SOS_PI);
     }
     float *out = a.moments + (int64_t)cell * (4 + K) * G;
     if constexpr (G >= 16) {
-        const int col = lane & 15, rb = 4 * (lane >> 4);
+        const int col = lane & 15, rb4 = 4 * (lane >> 4);
 #pragma unroll
         for (int t = 0; t < TPW; ++t) {
             const int kt = wave + 4 * t;
@@ -150,7 +173,7 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
                 for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        const int k = kt * 16 + rb + v;
+                        const int k = kt * 16 + rb4 + v;
                         if (k < K) out[(4 + k) * G + nt * 16 + col] = acc[t][nt][v];
                     }
             }
