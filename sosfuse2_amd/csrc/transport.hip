@@ -266,7 +266,8 @@ __global__ void plan_geometry_kernel(TransportArgs a, const int64_t *node_off, f
                 }
             }
             float4 *o = geom + n * 8;
-            const unsigned cellbits = (unsigned)(ng.ir * a.nc + ng.j) | (last ? 0x80000000u : 0u);
+            // base cell as (ring << 16) | column, so the consumers need no division by the column count
+            const unsigned cellbits = ((unsigned)ng.ir << 16) | (unsigned)ng.j | (last ? 0x80000000u : 0u);
             o[0] = make_float4(piece[0], piece[1], piece[2], piece[3]);
             o[1] = make_float4(piece[4], piece[5], post[4], post[5]);
             o[2] = make_float4(post[0], post[1], post[2], post[3]);
@@ -329,8 +330,8 @@ __global__ __launch_bounds__(256) void transport_geom_kernel(TransportArgs a) {
             const float att = g3.x * exp_hw(-mid);
             const float ah = g3.y, ac = g3.z;
             const unsigned cb = __float_as_uint(g3.w);
-            const int cell0 = (int)(cb & 0x7fffffffu);
-            const int ir = cell0 / nc;
+            const int ir = (int)((cb >> 16) & 0x7fffu);
+            const int cell0 = ir * nc + (int)(cb & 0xffffu);
             const int cells[4] = {cell0, cell0 + 1, cell0 + nc, cell0 + nc + 1};
             const float sw[4] = {(1.0f - ah) * (1.0f - ac), (1.0f - ah) * ac, ah * (1.0f - ac), ah * ac};
             float T[P][4];
@@ -531,7 +532,7 @@ __global__ __launch_bounds__(256) void transport_first_geom_kernel(TransportArgs
             const float att = g3.x * exp_hw(-mid);
             const float ah = g3.y, ac = g3.z;
             const unsigned cb = __float_as_uint(g3.w);
-            const int cell0 = (int)(cb & 0x7fffffffu);
+            const int cell0 = (int)((cb >> 16) & 0x7fffu) * nc + (int)(cb & 0xffffu);
             const int cells[4] = {cell0, cell0 + 1, cell0 + nc, cell0 + nc + 1};
             const float sw[4] = {(1.0f - ah) * (1.0f - ac), (1.0f - ah) * ac, ah * (1.0f - ac), ah * ac};
             const float4 *sg = a.node_sun + n * 3;
@@ -1096,6 +1097,8 @@ int plan_sun_launch(const TransportArgs &a, const int64_t *node_off, float *sun,
 }
 
 int plan_geometry_launch(const TransportArgs &a, const int64_t *node_off, float *geom, hipStream_t s) {
+    // the node geometry packs a base cell as 15 bits of ring and 16 bits of column
+    if (a.nh > 32768 || a.nc > 65536) return set_error("plan_geometry: %d x %d cells exceed 32768 x 65536", a.nh, a.nc);
     const unsigned b = (unsigned)((a.rays + 255) / 256);
     if (b)
         hipLaunchKernelGGL(plan_geometry_kernel, dim3(b), dim3(256), 0, s, a, node_off, reinterpret_cast<float4 *>(geom));

@@ -162,6 +162,7 @@ class Plan:
 
     def __init__(self, kernel, dc, db, chunk=2048, node_chunk=1024):
         self.node_chunk = node_chunk
+        self.arena = getattr(db, 'arena', None)
         dev = kernel.device
         st = kernel.stream()
         counts = torch.empty(max(db.rays, 1), dtype=torch.int64, device=dev)
@@ -191,7 +192,7 @@ class Plan:
         self._chunk = chunk
         self.vis = None  # (node, corner) visits by visited cell, built on first use (visit_items())
         self.visits = 4 * N
-        self.records = torch.empty((max(4 * N, 1), dc.G), dtype=torch.float32, device=dev)
+        self.records = self._buf('records', 4 * N, 'visits', dc.G)
         self.geom = None  # static node geometry of the warm consumer, built on first use (geometry())
         self.sun = None  # sun geometry of first light, built on first use (sun_geometry())
 
@@ -211,14 +212,14 @@ class Plan:
         """[rays, KZ] Legendre part of every ray's SH basis row (sos_plan_basis), built once."""
         if self.basis is None:
             kz = (kernel.K + 15) // 16 * 16
-            self.basis = torch.empty((max(db.rays, 1), kz), dtype=torch.float32, device=self.node_off.device)
+            self.basis = self._buf('basis', db.rays, 'rays', kz)
             hip.call('sos_plan_basis', dc.struct, db.struct, hip.ptr(self.basis), kernel.stream())
         return self.basis
 
     def geometry(self, kernel, dc, db):
         """[nodes, 32] static node geometry (sos_plan_geometry), built once."""
         if self.geom is None:
-            self.geom = torch.empty((max(self.nodes, 1), 32), dtype=torch.float32, device=self.node_off.device)
+            self.geom = self._buf('geom', self.nodes, 'nodes', 32)
             hip.call('sos_plan_geometry', dc.struct, db.struct, hip.ptr(self.node_off), hip.ptr(self.geom),
                      kernel.stream())
             db.struct.node_geom_dev = hip.ptr(self.geom)
@@ -227,7 +228,7 @@ class Plan:
     def sun_geometry(self, kernel, dc, db):
         """[nodes, 12] sun geometry of first light (sos_plan_sun), built once."""
         if self.sun is None:
-            self.sun = torch.empty((max(self.nodes, 1), 12), dtype=torch.float32, device=self.node_off.device)
+            self.sun = self._buf('sun', self.nodes, 'nodes', 12)
             hip.call('sos_plan_sun', dc.struct, db.struct, hip.ptr(self.node_off), hip.ptr(self.sun), kernel.stream())
             db.struct.node_sun_dev = hip.ptr(self.sun)
         return self.sun
@@ -244,8 +245,24 @@ class Plan:
         """The records buffer [4 nodes, G] (reallocated when a chunk of another lane count uses the plan)."""
         if self.records.shape[1] != G:
             self.records = None
-            self.records = torch.empty((max(4 * self.nodes, 1), G), dtype=torch.float32, device=self.node_off.device)
+            self.records = self._buf('records', 4 * self.nodes, 'visits', G)
         return self.records
+
+    def _buf(self, key, rows, kind, width):
+        """An FP32 [rows, width] buffer. The plan of a bundle part takes it from the arena its parts share: one
+        buffer per key, sized for the largest part (its nodes, visits or rays) and reused by every part, so the parts'
+        plans, built one after another, do not allocate and free tens of GB each."""
+        dev = self.node_off.device
+        a = self.arena
+        if a is None:
+            return torch.empty((max(rows, 1), width), dtype=torch.float32, device=dev)
+        need = max(rows, 1) * width
+        buf = a['bufs'].get(key)
+        if buf is None or buf.numel() < need:
+            a['bufs'][key] = buf = None
+            cap = {'nodes': a['nodes'], 'visits': 4 * a['nodes'], 'rays': a['rays']}[kind]
+            buf = a['bufs'][key] = torch.empty(max(need, max(cap, 1) * width), dtype=torch.float32, device=dev)
+        return buf[:need].view(max(rows, 1), width)
 
 
 def _items(key, cells, chunk, dev):
@@ -473,9 +490,17 @@ class Kernel:
             cum[1:] = torch.cumsum(counts[: db.rays], 0)
             per_group = cum[db.group_off[1:]] - cum[db.group_off[:-1]]
             db.nodes_total = int(cum[-1].item())
-            out = [db.part(g0, g1) for g0, g1 in split_groups(per_group.cpu().numpy(), self.part_nodes)]
+            per_group = per_group.cpu().numpy()
+            spans = split_groups(per_group, self.part_nodes)
+            out = [db.part(g0, g1) for g0, g1 in spans]
             if len(out) == 1:
                 out = [db]
+            else:
+                # the parts' plans share their large buffers, sized for the largest part
+                arena = dict(nodes=int(max(per_group[g0:g1].sum() for g0, g1 in spans)),
+                             rays=max(p.rays for p in out), bufs={})
+                for p in out:
+                    p.arena = arena
         db._parts = (key, out)
         return out
 
