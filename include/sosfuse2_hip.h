@@ -139,6 +139,9 @@ int sos_transport(const sos_grid *grid, const sos_bundle *bundle, int32_t flags,
 /* transport on the bundle's static node geometry (node_geom_dev; pairs 4): the warm records consumer (with
  * SOS_RECORDS), or first light (with SOS_FIRST and SOS_GROUND_RECORDS; also node_sun_dev) */
 #define SOS_NODE_GEOMETRY 64
+/* with SOS_NODE_GEOMETRY and SOS_RECORDS (warm): field_dev holds only the Rayleigh tensor rows, lane-major as
+ * [cells][lanes][4] (rows 0..3 of each lane together, one 16-byte read per corner); rows is not read */
+#define SOS_TENSOR4 128
 
 /* Visit plan of a bundle (fixed by its geometry): counts_dev [rays] = quadrature nodes of every ray; then with
  * node_off_dev = exclusive prefix sum of the counts ([rays + 1]), every node's cell (ir * nc + j, its corner 0),

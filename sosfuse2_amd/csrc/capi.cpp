@@ -360,7 +360,12 @@ int sos_transport(const sos_grid *g, const sos_bundle *b, int32_t flags, int32_t
         return arg_error("sos_transport: warm transport needs field and plane");
     if (!first && records && (!records_dev || !b->node_off_dev))
         return arg_error("sos_transport: records mode needs the records and the bundle's node offsets");
-    if (!first && rows < 10) return arg_error("sos_transport: rows %d < 10", rows);
+    const bool tensor4 = flags & SOS_TENSOR4;
+    if (tensor4 && (first || !records || !(flags & SOS_NODE_GEOMETRY)))
+        return arg_error("sos_transport: SOS_TENSOR4 is a form of the warm records consumer on node geometry");
+    if (tensor4 && (int64_t)g->nh * g->nc * g->lanes >= ((int64_t)1 << 28))
+        return arg_error("sos_transport: SOS_TENSOR4 tensor copy of %d x %d cells too large", g->nh, g->nc);
+    if (!first && !tensor4 && rows < 10) return arg_error("sos_transport: rows %d < 10", rows);
     if ((flags & SOS_BOUNDARY) && !boundary_dev) return arg_error("sos_transport: boundary flag without boundary");
     if (first && (flags & SOS_BOUNDARY)) return arg_error("sos_transport: first light takes no boundary");
     if (first && (!g->sun_col_dev || g->sun_nh < 2 || g->sun_nx < 2 || !g->gm_dev || !g->goff_dev))

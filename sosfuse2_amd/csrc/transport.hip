@@ -283,7 +283,8 @@ __global__ void plan_geometry_kernel(TransportArgs a, const int64_t *node_off, f
 // Warm consumer on the static node geometry (records mode, pairs xx yy zz xz): per node and lane only the lane's
 // optical depth, attenuation, the four corners (Rayleigh tensor from the field, SH value from the records) with the
 // log-space trace blend, and the Kahan radiance sum remain. Same operations as transport_kernel's warm path.
-template <int G, bool BOUNDARY>
+// TENSOR4: the field is the lane-major tensor copy [cells][G][4] (one 16-byte read per corner, 32-bit offsets).
+template <int G, bool BOUNDARY, bool TENSOR4>
 __global__ __launch_bounds__(256) void transport_geom_kernel(TransportArgs a) {
     constexpr int RPW = 64 / G;
     constexpr int P = 4;
@@ -354,8 +355,14 @@ __global__ __launch_bounds__(256) void transport_geom_kernel(TransportArgs a) {
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const float aero = rec[c * G];
-                const float *f = a.field + (int64_t)cells[c] * a.rows * G + w;
-                const float rm[4] = {f[0], f[G], f[2 * G], f[3 * G]};
+                float rm[4];
+                if constexpr (TENSOR4) {
+                    const float4 t = reinterpret_cast<const float4 *>(a.field)[(uint32_t)(cells[c] * G + w)];
+                    rm[0] = t.x; rm[1] = t.y; rm[2] = t.z; rm[3] = t.w;
+                } else {
+                    const float *f = a.field + (int64_t)cells[c] * a.rows * G + w;
+                    rm[0] = f[0]; rm[1] = f[G]; rm[2] = f[2 * G]; rm[3] = f[3 * G];
+                }
                 const float k = kmix * ((rm[0] + rm[1]) + rm[2]);
                 const float density = a.hr[ir + (c >> 1)];
                 const float rot3 = ((t3[0] * rm[0] + t3[1] * rm[1]) + t3[2] * rm[2]) + t3[3] * rm[3];
@@ -1009,7 +1016,7 @@ static int launch_one(const TransportArgs &a, hipStream_t s) {
     return SOS_OK;
 }
 
-template <int G, bool BOUNDARY, bool FIRST>
+template <int G, bool BOUNDARY, bool FIRST, bool TENSOR4 = false>
 static int launch_geom(const TransportArgs &a, hipStream_t s) {
     int dev = 0, cus = 256;
     hipGetDevice(&dev);
@@ -1020,7 +1027,7 @@ static int launch_geom(const TransportArgs &a, hipStream_t s) {
     grid = ((grid + 7) / 8) * 8;
     if (grid < 8) grid = 8;
     if constexpr (FIRST) hipLaunchKernelGGL((transport_first_geom_kernel<G>), dim3((unsigned)grid), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((transport_geom_kernel<G, BOUNDARY>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((transport_geom_kernel<G, BOUNDARY, TENSOR4>), dim3((unsigned)grid), dim3(256), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error("transport launch: %s", hipGetErrorString(e));
     return SOS_OK;
@@ -1039,8 +1046,13 @@ static int launch_flags(const TransportArgs &a, int L, hipStream_t s) {
         return (a.flags & SOS_UNIFORM_PHASE) ? launch_one<G, 8, P, true, false, RECORDS, true>(a, s)
                                              : launch_one<G, 8, P, true, false, RECORDS, false>(a, s);
     if (records && (a.flags & SOS_NODE_GEOMETRY)) {
-        if constexpr (P == 4) return bnd ? launch_geom<G, true, false>(a, s) : launch_geom<G, false, false>(a, s);
-        else return set_error("node geometry: pairs must be 4");
+        if constexpr (P == 4) {
+            if (a.flags & SOS_TENSOR4)
+                return bnd ? launch_geom<G, true, false, true>(a, s) : launch_geom<G, false, false, true>(a, s);
+            return bnd ? launch_geom<G, true, false>(a, s) : launch_geom<G, false, false>(a, s);
+        } else {
+            return set_error("node geometry: pairs must be 4");
+        }
     }
     if (records)
         return bnd ? launch_one<G, 8, P, false, true, RECORDS>(a, s) : launch_one<G, 8, P, false, false, RECORDS>(a, s);

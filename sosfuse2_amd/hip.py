@@ -10,7 +10,7 @@ P = ctypes.c_void_p
 
 LANE_CONSTANTS = 12
 PROFILE_SHAPES, PROFILE_EXP, PROFILE_GAUSS, PROFILE_TRIANGLE, PROFILE_TABLE, PROFILE_AIR_SQUARED, PROFILE_ZERO = range(7)
-FIRST, BOUNDARY, RECORDS, UNIFORM_PHASE, GROUND_RECORDS, NODE_GEOMETRY = 1, 2, 8, 16, 32, 64
+FIRST, BOUNDARY, RECORDS, UNIFORM_PHASE, GROUND_RECORDS, NODE_GEOMETRY, TENSOR4 = 1, 2, 8, 16, 32, 64, 128
 
 
 class Bundle(ctypes.Structure):

@@ -575,7 +575,12 @@ class Kernel:
                     pl.geometry(self, dc, db)
                     flags |= hip.NODE_GEOMETRY
         rows = ctx.rows
-        if not first and (flags & hip.RECORDS) and self.tuning.get('tensor_rows', True) and field.shape[1] > 10:
+        if not first and (flags & hip.RECORDS) and (flags & hip.NODE_GEOMETRY) and self.tuning.get('tensor_rows', True):
+            # the node-geometry consumer reads only the four Rayleigh tensor rows of the field: a lane-major copy
+            # [cells][lanes][4] gives each corner one 16-byte read with a 32-bit offset
+            field = field[:, :4].transpose(1, 2).contiguous()
+            flags |= hip.TENSOR4
+        elif not first and (flags & hip.RECORDS) and self.tuning.get('tensor_rows', True) and field.shape[1] > 10:
             # with records the warm pass reads only the tensor rows of the field: a compact copy [cells][10][lanes]
             # keeps every corner's rows within 1.3 KB of each other instead of a full field row (4.5 KB at L = 32)
             # apart, so the transport's corner reads stay in few pages and cache lines
