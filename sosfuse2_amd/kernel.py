@@ -165,36 +165,84 @@ class Plan:
         self.arena = getattr(db, 'arena', None)
         dev = kernel.device
         st = kernel.stream()
-        counts = torch.empty(max(db.rays, 1), dtype=torch.int64, device=dev)
-        hip.call('sos_plan_count', dc.struct, db.struct, hip.ptr(counts), st)
-        self.node_off = torch.zeros(db.rays + 1, dtype=torch.int64, device=dev)
-        self.node_off[1:] = torch.cumsum(counts[: db.rays], 0)
-        db.struct.node_off_dev = hip.ptr(self.node_off)
-        self.nodes = N = check_plan_size(self.node_off[-1].item())
-        self.node_cell = torch.empty(max(N, 1), dtype=torch.int32, device=dev)
-        self.node_mu = torch.empty(max(N, 1), dtype=torch.float32, device=dev)
-        self.node_cp = torch.empty(max(N, 1), dtype=torch.float32, device=dev)
-        self.node_dir = torch.empty((max(N, 1), 4), dtype=torch.float32, device=dev)
-        hip.call('sos_plan_nodes', dc.struct, db.struct, hip.ptr(self.node_off), hip.ptr(self.node_cell),
-                 hip.ptr(self.node_mu), hip.ptr(self.node_cp), hip.ptr(self.node_dir), st)
+        self._nodes_args = (dc.struct, db.struct, st)
         nc = dc.struct.nc
         cells = dc.struct.nh * nc
-        # nodes by base cell (SH records), with each node's ray and cos(phi) in the same order
-        self.node = torch.argsort(self.node_cell[:N], stable=True).to(torch.int32)
-        ray_of = torch.repeat_interleave(torch.arange(db.rays, dtype=torch.int32, device=dev), counts[: db.rays])
-        self.sorted_ray = ray_of[self.node.long()] if N else torch.zeros(1, dtype=torch.int32, device=dev)
-        del ray_of
-        self.sorted_cp = self.node_cp[self.node.long()]
-        self.basis = None  # basis table [rays][KZ] (sos_plan_basis), built on first use (basis_table())
         self._cells = cells
         self._nc = nc
-        self.set_node_chunk(node_chunk)
         self._chunk = chunk
+        kept = getattr(db, 'kept', None) if self.arena is not None else None
+        if kept is not None and kept['node_chunk'] == node_chunk:
+            # a bundle part's node offsets, sorted nodes and work items, kept from its first plan: the plans of the
+            # later launches skip the node data, the sort by base cell and the work items
+            self.node_off = kept['node_off']
+            db.struct.node_off_dev = hip.ptr(self.node_off)
+            self.nodes = N = kept['nodes']
+            self.node, self.sorted_ray, self.sorted_cp = kept['node'], kept['sorted_ray'], kept['sorted_cp']
+            self.sh_cell, self.sh_v0, self.sh_v1 = kept['items']
+            self.sh_items = len(self.sh_cell)
+            self._node_cell = self._node_mu = self._node_cp = self._node_dir = None  # on demand (first light)
+        else:
+            counts = torch.empty(max(db.rays, 1), dtype=torch.int64, device=dev)
+            hip.call('sos_plan_count', dc.struct, db.struct, hip.ptr(counts), st)
+            self.node_off = torch.zeros(db.rays + 1, dtype=torch.int64, device=dev)
+            self.node_off[1:] = torch.cumsum(counts[: db.rays], 0)
+            db.struct.node_off_dev = hip.ptr(self.node_off)
+            self.nodes = N = check_plan_size(self.node_off[-1].item())
+            self._node_cell = None
+            self._fill_nodes()
+            # nodes by base cell (SH records), with each node's ray and cos(phi) in the same order
+            self.node = torch.argsort(self.node_cell[:N], stable=True).to(torch.int32)
+            ray_of = torch.repeat_interleave(torch.arange(db.rays, dtype=torch.int32, device=dev), counts[: db.rays])
+            self.sorted_ray = ray_of[self.node.long()] if N else torch.zeros(1, dtype=torch.int32, device=dev)
+            del ray_of
+            self.sorted_cp = self.node_cp[self.node.long()]
+            self.set_node_chunk(node_chunk)
+            if self.arena is not None:
+                db.kept = dict(node_chunk=node_chunk, node_off=self.node_off, nodes=N, node=self.node,
+                               sorted_ray=self.sorted_ray, sorted_cp=self.sorted_cp,
+                               items=(self.sh_cell, self.sh_v0, self.sh_v1))
+        self.basis = None  # basis table [rays][KZ] (sos_plan_basis), built on first use (basis_table())
         self.vis = None  # (node, corner) visits by visited cell, built on first use (visit_items())
         self.visits = 4 * N
         self.records = self._buf('records', 4 * N, 'visits', dc.G)
         self.geom = None  # static node geometry of the warm consumer, built on first use (geometry())
         self.sun = None  # sun geometry of first light, built on first use (sun_geometry())
+
+    def _fill_nodes(self):
+        """Every node's cell (ir nc + j), mu, cos(phi) and direction (sos_plan_nodes)."""
+        N, dev = self.nodes, self.node_off.device
+        grid, bundle, st = self._nodes_args
+        self._node_cell = torch.empty(max(N, 1), dtype=torch.int32, device=dev)
+        self._node_mu = torch.empty(max(N, 1), dtype=torch.float32, device=dev)
+        self._node_cp = torch.empty(max(N, 1), dtype=torch.float32, device=dev)
+        self._node_dir = torch.empty((max(N, 1), 4), dtype=torch.float32, device=dev)
+        hip.call('sos_plan_nodes', grid, bundle, hip.ptr(self.node_off), hip.ptr(self._node_cell),
+                 hip.ptr(self._node_mu), hip.ptr(self._node_cp), hip.ptr(self._node_dir), st)
+
+    @property
+    def node_cell(self):
+        if self._node_cell is None:
+            self._fill_nodes()
+        return self._node_cell
+
+    @property
+    def node_mu(self):
+        if self._node_cell is None:
+            self._fill_nodes()
+        return self._node_mu
+
+    @property
+    def node_cp(self):
+        if self._node_cell is None:
+            self._fill_nodes()
+        return self._node_cp
+
+    @property
+    def node_dir(self):
+        if self._node_cell is None:
+            self._fill_nodes()
+        return self._node_dir
 
     def visit_items(self):
         """The (node, corner) visits sorted by visited cell and their work items of at most `chunk` visits (the
