@@ -493,6 +493,7 @@ def main():
         bytes=f'{cbytes / 1e9:.2f} GB per launch: records 4 x {plan.nodes} nodes x {G} lanes, node geometry 32 '
               f'floats per node, tensor rows, ray constants and radiance out')
     log(f'warm consumer: {t_c * 1e3:.2f} ms per launch, {cbytes / 1e9:.2f} GB -> {cbytes / t_c / 1e9:.0f} GB/s')
+    del plan  # the counts are taken; the bundle alone holds its plan (released before the production table)
     out = dict(
         metric='radiance tables/hour/GPU at <=1e-4 rel err vs FP64; learned+batched vs plain',
         value=round(3600.0 / per_step, 2),
@@ -553,6 +554,8 @@ def main():
         def release():
             solver.geometry.bundle.plan = None
             solver.geometry.ground_bundle.plan = None
+            # the earlier tables' query bundles (one per elevation set of the batched chunks) and their plans
+            k.__dict__.get('_observer_bundles', {}).clear()
 
         out['production'] = production_variant(pscope, k, state, tol, table, release)
         out['production']['vs_headline'] = round(out['production']['tables_per_hour'] / out['value'], 4)
