@@ -27,23 +27,36 @@ static unsigned blocks_for(int64_t n, int threads = 256, int64_t cap = 65535 * 8
     return (unsigned)b;
 }
 
+// log2 of a power-of-two lane count, -1 otherwise
+static int lane_shift(int G) {
+    int gs = 0;
+    while ((1 << gs) < G) ++gs;
+    return (1 << gs) == G ? gs : -1;
+}
+
 // ----------------------------------------------------------------------------------------------------- compose
-__global__ void compose_kernel(int64_t cells, int rows, int K, int G, const float *__restrict__ mom,
-                               const float *__restrict__ first, const float *__restrict__ factor,
-                               float *__restrict__ out) {
-    const int64_t total = cells * rows * G;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int w = (int)(e % G);
-        const int64_t cr = e / G;
-        const int row = (int)(cr % rows);
-        const int64_t cell = cr / rows;
-        const int mr = row < 10 ? row : 4 + (row - 10) % K;
-        const int64_t src = (cell * (4 + K) + mr) * G + w;
-        float m = mom[src];
-        if (first) m = m + first[src];
+// One workgroup per cell; G is a power of two (1, 16 or 32), so a row and lane come from a shift and a mask.
+__global__ __launch_bounds__(256) void compose_kernel(int rows, int K, int G, int gs, const float *__restrict__ mom,
+                                                      const float *__restrict__ first, const float *__restrict__ factor,
+                                                      float *__restrict__ out) {
+    const int64_t cell = blockIdx.x;
+    const float *mc = mom + cell * (4 + K) * G;
+    const float *fc = first ? first + cell * (4 + K) * G : nullptr;
+    float *oc = out + cell * (int64_t)rows * G;
+    const int n = rows * G;
+    for (int idx = threadIdx.x; idx < n; idx += 256) {
+        const int row = idx >> gs, w = idx & (G - 1);
+        int mr = row;
+        if (row >= 10) {
+            mr = row - 10;
+            while (mr >= K) mr -= K;
+            mr += 4;
+        }
+        float m = mc[mr * G + w];
+        if (fc) m = m + fc[mr * G + w];
         float v = factor[row * G + w] * m;
         if (fabsf(v) < SOS_FLT_MIN) v = 0.0f;
-        out[e] = v;
+        oc[idx] = v;
     }
 }
 
@@ -56,20 +69,26 @@ __device__ inline int plane_k(int L, int j) {
     return l * (l + 1) / 2 + m;
 }
 
-__global__ void plane_kernel(int nh, int nc, int L, int rows, int G, int strato, const float *__restrict__ src,
-                             const float *__restrict__ ha, const float *__restrict__ sd, float *__restrict__ plane) {
+// One workgroup per cell: the l-major index of every m-major position in LDS, the cell's profile weights once.
+__global__ __launch_bounds__(256) void plane_kernel(int nc, int L, int rows, int G, int gs, int strato,
+                                                    const float *__restrict__ src, const float *__restrict__ ha,
+                                                    const float *__restrict__ sd, float *__restrict__ plane) {
     const int K = (L + 1) * (L + 2) / 2;
-    const int64_t total = (int64_t)nh * nc * K * G;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int w = (int)(e % G);
-        const int64_t ck = e / G;
-        const int k = plane_k(L, (int)(ck % K));
-        const int64_t cell = ck / K;
-        const int ih = (int)(cell / nc);
-        const float *s = src + cell * rows * G + w;
-        float v = s[(10 + k) * G] * ha[ih];
-        if (strato) v = v + s[(10 + K + k) * G] * sd[ih];
-        plane[e] = v;
+    __shared__ short kl[(32 + 1) * (32 + 2) / 2];
+    for (int j = threadIdx.x; j < K; j += 256) kl[j] = (short)plane_k(L, j);
+    __syncthreads();
+    const int64_t cell = blockIdx.x;
+    const int ih = (int)(cell / nc);
+    const float hav = ha[ih], sdv = strato ? sd[ih] : 0.0f;
+    const float *sc = src + cell * rows * G;
+    float *pc = plane + cell * (int64_t)K * G;
+    const int n = K * G;
+    for (int idx = threadIdx.x; idx < n; idx += 256) {
+        const int j = idx >> gs, w = idx & (G - 1);
+        const int k = kl[j];
+        float v = sc[(10 + k) * G + w] * hav;
+        if (strato) v = v + sc[(10 + K + k) * G + w] * sdv;
+        pc[idx] = v;
     }
 }
 
@@ -372,21 +391,26 @@ __global__ void mix_scalar_kernel(int64_t elems, int G, int depth, int m, const 
 // ------------------------------------------------------------------------------------------------ host entries
 int compose_launch(int64_t cells, int rows, int K, int G, const float *mom, const float *first, const float *factor,
                    float *out, hipStream_t s) {
-    hipLaunchKernelGGL(compose_kernel, dim3(blocks_for(cells * rows * G)), dim3(256), 0, s, cells, rows, K, G, mom,
-                       first, factor, out);
+    const int gs = lane_shift(G);
+    if (gs < 0) return set_error("compose: lanes %d not a power of two", G);
+    if (cells > 0)
+        hipLaunchKernelGGL(compose_kernel, dim3((unsigned)cells), dim3(256), 0, s, rows, K, G, gs, mom, first, factor,
+                           out);
     return check("compose");
 }
 
 int plane_launch(int nh, int nc, int L, int rows, int G, int strato, const float *src, const float *ha,
                  const float *sd, int half, void *plane, float *scale, hipStream_t s) {
-    const int K = (L + 1) * (L + 2) / 2;
     if (half) {
         if (nh * nc > 0)
             hipLaunchKernelGGL(plane_half_kernel, dim3((unsigned)(nh * nc)), dim3(256), 0, s, nc, L, rows, G, strato,
                                src, ha, sd, reinterpret_cast<_Float16 *>(plane), scale);
     } else {
-        hipLaunchKernelGGL(plane_kernel, dim3(blocks_for((int64_t)nh * nc * K * G)), dim3(256), 0, s, nh, nc, L, rows,
-                           G, strato, src, ha, sd, reinterpret_cast<float *>(plane));
+        const int gs = lane_shift(G);
+        if (gs < 0 || L > 32) return set_error("make_plane: lanes %d or L %d", G, L);
+        if (nh * nc > 0)
+            hipLaunchKernelGGL(plane_kernel, dim3((unsigned)(nh * nc)), dim3(256), 0, s, nc, L, rows, G, gs, strato,
+                               src, ha, sd, reinterpret_cast<float *>(plane));
     }
     return check("make_plane");
 }
