@@ -195,21 +195,27 @@ __global__ void metrics_finish(int G, const unsigned *acc, float *out) {
 }
 
 // --------------------------------------------------------------------------------------------------- anderson
-__global__ void norms_kernel(int64_t cells, int rows, int G, const float *__restrict__ g, float *__restrict__ norms,
-                             unsigned *__restrict__ maxbits) {
-    const int64_t total = cells * G;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int w = (int)(e % G);
-        const int64_t cell = e / G;
-        const float *p = g + cell * rows * G + w;
-        double s = 0.0;
-        for (int r = 0; r < rows; ++r) {
-            const double v = p[(int64_t)r * G];
-            s += v * v;
-        }
-        const float nrm = (float)sqrt(s);
-        norms[e] = nrm;
-        atomicMax(&maxbits[w], __float_as_uint(nrm));
+// One workgroup per cell: thread t sums the squares of rows t / G, t / G + 256 / G, ... of lane t mod G in FP64, and
+// the row groups are added in a fixed order.
+__global__ __launch_bounds__(256) void norms_kernel(int rows, int G, int gs, const float *__restrict__ g,
+                                                    float *__restrict__ norms, unsigned *__restrict__ maxbits) {
+    __shared__ double part[256];
+    const int64_t cell = blockIdx.x;
+    const int t = threadIdx.x, w = t & (G - 1), rg = t >> gs, groups = 256 >> gs;
+    const float *p = g + cell * rows * G + w;
+    double s = 0.0;
+    for (int r = rg; r < rows; r += groups) {
+        const double v = p[r * G];
+        s += v * v;
+    }
+    part[t] = s;
+    __syncthreads();
+    if (t < G) {
+        double tot = 0.0;
+        for (int q = 0; q < groups; ++q) tot += part[(q << gs) + t];
+        const float nrm = (float)sqrt(tot);
+        norms[cell * G + t] = nrm;
+        atomicMax(&maxbits[t], __float_as_uint(nrm));
     }
 }
 
@@ -229,14 +235,17 @@ __global__ __launch_bounds__(256) void residual_kernel(int64_t cells, int rows, 
     if (threadIdx.x < G) floor_[threadIdx.x] = __uint_as_float(maxbits[threadIdx.x]) * 1e-8f;
     __syncthreads();
     const int64_t tiles = (nsrc + R - 1) / R;
+    __shared__ int64_t tcell[R];
     for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const int64_t r0 = t * R;
+        if (threadIdx.x < R) tcell[threadIdx.x] = (r0 + threadIdx.x) / rows;  // the cell of each row of the tile
+        __syncthreads();
         for (int e = threadIdx.x; e < R * G; e += blockDim.x) {
             const int r = e / G, w = e % G;
             const int64_t i = r0 + r;
             float v = 0.0f;
             if (i < nsrc) {
-                const float scale = fmaxf(norms[(i / rows) * G + w], floor_[w]);
+                const float scale = fmaxf(norms[tcell[r] * G + w], floor_[w]);
                 v = (g[i * G + w] - x[i * G + w]) / scale;
             }
             tile[w][r] = v;
@@ -270,7 +279,22 @@ __global__ __launch_bounds__(256) void gram_partial(int64_t n, int G, int depth,
     double s[DMAX];
 #pragma unroll
     for (int j = 0; j < DMAX; ++j) s[j] = 0.0;
-    for (int64_t i = i0 + tid; i < i1; i += 256) {
+    // four strided elements per step, so every thread keeps up to 4 (depth + 1) loads in flight
+    int64_t i = i0 + tid;
+    for (; i + 768 < i1; i += 1024) {
+        const double a0 = fs[i], a1 = fs[i + 256], a2 = fs[i + 512], a3 = fs[i + 768];
+#pragma unroll
+        for (int j = 0; j < DMAX; ++j)
+            if (j < depth && ((filled >> j) & 1u)) {
+                const float *fj = ring + ((int64_t)j * G + w) * n + i;
+                const float b0 = fj[0], b1 = fj[256], b2 = fj[512], b3 = fj[768];
+                s[j] += a0 * (double)b0;
+                s[j] += a1 * (double)b1;
+                s[j] += a2 * (double)b2;
+                s[j] += a3 * (double)b3;
+            }
+    }
+    for (; i < i1; i += 256) {
         const double a = fs[i];
 #pragma unroll
         for (int j = 0; j < DMAX; ++j)
@@ -430,7 +454,9 @@ int residual_launch(int64_t cells, int rows, int G, const float *g, const float 
     float *norms = work;
     unsigned *maxbits = reinterpret_cast<unsigned *>(work + cells * G);
     hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, maxbits, G);
-    hipLaunchKernelGGL(norms_kernel, dim3(blocks_for(cells * G)), dim3(256), 0, s, cells, rows, G, g, norms, maxbits);
+    const int gs = lane_shift(G);
+    if (gs < 0) return set_error("anderson_residual: lanes %d not a power of two", G);
+    if (cells > 0) hipLaunchKernelGGL(norms_kernel, dim3((unsigned)cells), dim3(256), 0, s, rows, G, gs, g, norms, maxbits);
     const int64_t tiles = (cells * rows + 63) / 64;
     hipLaunchKernelGGL(residual_kernel, dim3((unsigned)max((int64_t)1, min(tiles, (int64_t)8192))), dim3(256), 0, s,
                        cells, rows, G, g, x, norms, maxbits, obs, nobs, f);
