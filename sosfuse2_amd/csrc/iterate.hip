@@ -372,25 +372,40 @@ __global__ void solve_kernel(int G, int depth, int m, const int *__restrict__ or
 }
 
 // x = sum_j ring[order[j]] wts[lane][j] in float4 groups of four lanes (G is a multiple of 4, so a group never
-// straddles a row); a lane whose weights are NaN (frozen) keeps x.
+// straddles a row); a lane whose weights are NaN (frozen) keeps x. The grid stride is a multiple of G / 4, so every
+// thread keeps the same four lanes and reads their weights and the slot order once.
 __global__ __launch_bounds__(256) void mix_kernel(int64_t elems, int G, int depth, int m, const float *__restrict__ ring,
                                                   const int *__restrict__ order, const float *__restrict__ wts,
                                                   float *__restrict__ x) {
+    constexpr int DMAX = 8;
     const int64_t n4 = elems * G / 4;
     const float4 *r4 = reinterpret_cast<const float4 *>(ring);
     float4 *x4 = reinterpret_cast<float4 *>(x);
     const int g4 = G / 4;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n4; e += (int64_t)gridDim.x * blockDim.x) {
-        const int w0 = 4 * (int)(e % g4);
-        const float *wa = wts + w0 * depth, *wb = wa + depth, *wc = wb + depth, *wd = wc + depth;
-        const bool fa = isnan(wa[0]), fb = isnan(wb[0]), fc = isnan(wc[0]), fd = isnan(wd[0]);
-        if (fa && fb && fc && fd) continue;
-        float4 r = r4[(int64_t)order[0] * n4 + e];
+    const int64_t e0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int w0 = 4 * (int)(e0 % g4);
+    float wa[DMAX], wb[DMAX], wc[DMAX], wd[DMAX];
+    int64_t off[DMAX];
+#pragma unroll
+    for (int j = 0; j < DMAX; ++j) {
+        const bool in = j < m;
+        wa[j] = in ? wts[w0 * depth + j] : 0.0f;
+        wb[j] = in ? wts[(w0 + 1) * depth + j] : 0.0f;
+        wc[j] = in ? wts[(w0 + 2) * depth + j] : 0.0f;
+        wd[j] = in ? wts[(w0 + 3) * depth + j] : 0.0f;
+        off[j] = in ? (int64_t)order[j] * n4 : 0;
+    }
+    const bool fa = isnan(wa[0]), fb = isnan(wb[0]), fc = isnan(wc[0]), fd = isnan(wd[0]);
+    if (fa && fb && fc && fd) return;
+    for (int64_t e = e0; e < n4; e += (int64_t)gridDim.x * blockDim.x) {
+        float4 r = r4[off[0] + e];
         float4 v = float4{r.x * wa[0], r.y * wb[0], r.z * wc[0], r.w * wd[0]};
-        for (int j = 1; j < m; ++j) {
-            r = r4[(int64_t)order[j] * n4 + e];
-            v = float4{v.x + r.x * wa[j], v.y + r.y * wb[j], v.z + r.z * wc[j], v.w + r.w * wd[j]};
-        }
+#pragma unroll
+        for (int j = 1; j < DMAX; ++j)
+            if (j < m) {
+                r = r4[off[j] + e];
+                v = float4{v.x + r.x * wa[j], v.y + r.y * wb[j], v.z + r.z * wc[j], v.w + r.w * wd[j]};
+            }
         if (fa || fb || fc || fd) {
             const float4 o = x4[e];
             v = float4{fa ? o.x : v.x, fb ? o.y : v.y, fc ? o.z : v.z, fd ? o.w : v.w};
@@ -522,8 +537,9 @@ int mix_launch(int64_t cells, int rows, int G, int depth, int m, const float *ri
                float *x, hipStream_t s) {
     const int64_t elems = cells * rows;
     if (G % 4 == 0 && ((uintptr_t)ring % 16) == 0 && ((uintptr_t)x % 16) == 0)
-        hipLaunchKernelGGL(mix_kernel, dim3(blocks_for(elems * G / 4)), dim3(256), 0, s, elems, G, depth, m, ring,
-                           order, w, x);
+        // a grid of a few workgroups per CU: each thread streams many float4 groups after reading its weights once
+        hipLaunchKernelGGL(mix_kernel, dim3(blocks_for(elems * G / 4, 256, 4096)), dim3(256), 0, s, elems, G, depth, m,
+                           ring, order, w, x);
     else
         hipLaunchKernelGGL(mix_scalar_kernel, dim3(blocks_for(elems * G)), dim3(256), 0, s, elems, G, depth, m, ring,
                            order, w, x);
