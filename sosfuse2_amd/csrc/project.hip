@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
     constexpr int TPW = (KT + 3) / 4;        // harmonic tiles per wavefront
     constexpr int ZQ = 16 * (KP / 4);        // float4 chunks of a round's basis rows
     constexpr int RQ = 16 * G;               // float4 radiance tensors of a round
-    __shared__ float Z[16][KP];
+    __shared__ __attribute__((aligned(16))) float Z[16][KP];
     __shared__ float V[16][G];
     __shared__ float4 Rd[16][G];
     __shared__ float cmv[16][L + 1];
@@ -62,8 +62,6 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
         km[k] = (unsigned char)m;
         kl[k] = (short)(k < K ? kidx(k - moff<L>(m) + m, m) : K);
     }
-    if constexpr (KP > K)
-        for (int e = tid; e < 16 * (KP - K); e += 256) Z[e / (KP - K)][K + e % (KP - K)] = 0.0f;
     const int wave = tid >> 6, lane = tid & 63;
     floatx4 acc[TPW][NT];
 #pragma unroll
@@ -105,18 +103,16 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
             Rd[i][w] = i < n ? reinterpret_cast<const float4 *>(a.rad)[(c0 + i) * G + w] : float4{0.f, 0.f, 0.f, 0.f};
         }
         __syncthreads();
-        // basis rows: the ray's row of the bundle's basis table (fp32(x_m P_lm(mu))) times cos(m phi), into the
-        // l-major order of the moments
+        // basis rows: the ray's row of the bundle's basis table (fp32(x_m P_lm(mu))) times cos(m phi), in the table's
+        // m-major order (one 16-byte store per chunk; the moments are put in l-major order as they are written)
         for (int e = tid; e < ZQ; e += 256) {
             const int i = e / (KP / 4), q = e % (KP / 4);
             float4 z = float4{0.0f, 0.0f, 0.0f, 0.0f};
             if (i < n) z = reinterpret_cast<const float4 *>(a.basis + (c0 + i) * KP)[q];
-            const float zz[4] = {z.x, z.y, z.z, z.w};
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int k = 4 * q + u;
-                if (k < K) Z[i][kl[k]] = zz[u] * cmv[i][km[k]];
-            }
+            const float *cm = cmv[i];
+            const int k = 4 * q;
+            reinterpret_cast<float4 *>(&Z[i][0])[q] =
+                float4{z.x * cm[km[k]], z.y * cm[km[k + 1]], z.z * cm[km[k + 2]], z.w * cm[km[k + 3]]};
         }
         for (int e = tid; e < 16 * G; e += 256) {
             const int i = e / G, w = e % G;
@@ -173,8 +169,8 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
                 for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        const int k = kt * 16 + rb4 + v;
-                        if (k < K) out[(4 + k) * G + nt * 16 + col] = acc[t][nt][v];
+                        const int k = kt * 16 + rb4 + v;  // m-major position: moment kl[k]
+                        if (k < K) out[(4 + kl[k]) * G + nt * 16 + col] = acc[t][nt][v];
                     }
             }
         }
@@ -182,7 +178,7 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
 #pragma unroll
         for (int q = 0; q < (K + 255) / 256; ++q) {
             const int k = tid + 256 * q;
-            if (k < K) out[(4 + k) * G] = accv[q];
+            if (k < K) out[(4 + kl[k]) * G] = accv[q];
         }
     }
 }
