@@ -570,19 +570,22 @@ __global__ __launch_bounds__(256) void transport_first_geom_kernel(TransportArgs
                 const float4 t4 = g[4 + q];
                 T[q][0] = t4.x; T[q][1] = t4.y; T[q][2] = t4.z; T[q][3] = t4.w;
             }
+            // the corners' ground tensors are blended before the node's rotation (linear in them): R = sum_c sw rm,
+            // S = sum_c sw trace(rm), one 4 x 4 rotation per node instead of four
+            float R[4] = {0.0f, 0.0f, 0.0f, 0.0f}, S = 0.0f;
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const float *gm = a.gm + (int64_t)cells[c] * 4 * G + w;
                 const float rm[4] = {gm[0], gm[G], gm[2 * G], gm[3 * G]};
-                const float trace = (rm[0] + rm[1]) + rm[2];
 #pragma unroll
-                for (int q = 0; q < P; ++q) {
-                    float rot = 0.0f;
+                for (int uu = 0; uu < 4; ++uu) R[uu] = R[uu] + sw[c] * rm[uu];
+                S = S + sw[c] * ((rm[0] + rm[1]) + rm[2]);
+            }
+            const float kS = (1.0f - mix) / 3.0f * S;
 #pragma unroll
-                    for (int uu = 0; uu < 4; ++uu) rot = rot + T[q][uu] * rm[uu];
-                    const float tensor = prof0 * (mix * rot + ((1.0f - mix) / 3.0f * trace) * pu[q]);
-                    val[q] = val[q] + sw[c] * tensor;
-                }
+            for (int q = 0; q < P; ++q) {
+                const float rot = ((T[q][0] * R[0] + T[q][1] * R[1]) + T[q][2] * R[2]) + T[q][3] * R[3];
+                val[q] = val[q] + prof0 * (mix * rot + kS * pu[q]);
             }
 #pragma unroll
             for (int q = 0; q < P; ++q) {
