@@ -191,8 +191,26 @@ class Solver:
         m = out.cpu().numpy()
         return int(m[0].sum()), m[2] / np.maximum(m[1], np.float32(1e-30))
 
-    def solve(self, max_eval=40, tolerance=1e-5, x0=None):
-        """Iterate to max(observer, source) relative change <= tolerance on every lane (from evaluation 2)."""
+    def solve(self, max_eval=40, tolerance=1e-5, x0=None, fallback=True):
+        """Iterate to max(observer, source) relative change <= tolerance on every lane (from evaluation 2).
+        x0: a starting source (a warm start); it only seeds the iteration. With fallback, a seeded iteration that does
+        not converge within max_eval evaluations is run again from the cold start, so a start can cost evaluations
+        but never the answer; info['evaluations'] then counts both runs and info['fallback'] is set."""
+        x, info, ok = self._iterate(max_eval, tolerance, x0)
+        if not ok and x0 is not None and fallback:
+            if self.log:
+                self.log(f'SOSFUSE seeded iteration not converged after {info["evaluations"]} evaluations; '
+                         f'solving from the cold start')
+            spent = info['evaluations']
+            x, info, ok = self._iterate(max_eval, tolerance, None)
+            info = dict(info, evaluations=info['evaluations'] + spent, evaluations_seeded=spent, fallback=True)
+        if not ok:
+            W = self.s.W
+            raise RuntimeError(f'E_NOT_CONVERGED: {sum(info["converged"][:W])}/{W} within {tolerance}')
+        return x, info
+
+    def _iterate(self, max_eval, tolerance, x0):
+        """One run of the iteration: (x, info, converged on every lane)."""
         s, k = self.s, self.kernel
         G, D = self.G, self.depth
         W = s.W
@@ -266,7 +284,5 @@ class Solver:
                          hip.ptr(x), k.stream())
         k.sync()
         self._last_plane = None
-        if not done[:W].all():
-            raise RuntimeError(f'E_NOT_CONVERGED: {done[:W].sum()}/{W} within {tolerance}')
         return x, dict(evaluations=evaluations, tolerance=tolerance, converged=done[:W].tolist(), history=history,
-                       seconds=time.perf_counter() - started)
+                       seconds=time.perf_counter() - started), bool(done[:W].all())

@@ -200,3 +200,32 @@ def test_stokes_of_pure_and_unpolarised_tensors():
     comp = rng.normal(size=(n, 5, 6))
     got = api.stokes_from_tensor(torch.from_numpy(comp), tv).numpy()
     np.testing.assert_allclose(got, stokes(comp, view), rtol=1e-12, atol=1e-12)
+
+
+def test_seeded_solve_falls_back_to_the_cold_start():
+    """Solver.solve with a start x0 that does not converge runs the iteration again from the cold start and counts
+    both runs; without a start, or with fallback=False, the non-convergence is raised (E_NOT_CONVERGED)."""
+    from types import SimpleNamespace
+
+    from sosfuse2_amd.solver import Solver
+
+    calls = []
+
+    def iterate(max_eval, tolerance, x0):
+        calls.append(x0)
+        ok = x0 is None  # the seeded run stalls, the cold one converges
+        return 'x', dict(evaluations=max_eval if not ok else 7, converged=[ok, True, ok]), ok
+
+    sv = Solver.__new__(Solver)
+    sv.s, sv.log = SimpleNamespace(W=3), None
+    sv._iterate = iterate
+    x, info = sv.solve(20, 1e-5, x0='start')
+    assert calls == ['start', None] and x == 'x'
+    assert info['fallback'] and info['evaluations_seeded'] == 20 and info['evaluations'] == 27
+    calls.clear()
+    with pytest.raises(RuntimeError, match='E_NOT_CONVERGED: 1/3'):
+        sv.solve(20, 1e-5, x0='start', fallback=False)
+    assert calls == ['start']
+    sv._iterate = lambda m, t, x0: ('x', dict(evaluations=m, converged=[False, True, False]), False)
+    with pytest.raises(RuntimeError, match='E_NOT_CONVERGED'):
+        sv.solve(20, 1e-5)

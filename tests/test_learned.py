@@ -68,8 +68,9 @@ def test_training_lowers_the_loss(gpu):
 
 def test_warm_start_is_closer_and_reaches_the_same_tables(gpu):
     """Train on solves of seeded states of the batched distribution (small grid, three wavelengths), then solve
-    held-out states cold and warm-started: every warm-started solve converges (to 1e-5, within the same 60
-    evaluations) and reaches the cold solve's table within 1e-4, so a start never changes the answer or stalls the
+    held-out states cold and warm-started: every warm-started solve converges to 1e-5 (Solver.solve runs a seeded
+    iteration that does not converge within its 60 evaluations again from the cold start; the count of such fallbacks
+    is printed) and reaches the cold solve's table within 1e-4, so a start never changes the answer or stalls the
     solve. What the network gains is its start's distance to the converged source: the relative L2 distance over the
     sunlit cells' tensor and isotropic rows, asserted at least 2x smaller than the first-light start's. The evaluation
     counts of both solves are printed, not bounded: on this coarse grid they are erratic for either start (cold counts
@@ -102,6 +103,7 @@ def test_warm_start_is_closer_and_reaches_the_same_tables(gpu):
     el = 90 - (np.arange(3) + 0.5) * 30
     az = (np.arange(4) + 0.5) * 45
     cold, warm, d_first, d_warm = [], [], [], []
+    fallbacks = 0
     for st in test_states:
         s = scope.context(atmosphere.lane_optics(st, wl))
         try:
@@ -120,10 +122,11 @@ def test_warm_start_is_closer_and_reaches_the_same_tables(gpu):
         xw, iw = sv.solve(60, 1e-5, x0=x0)
         cold.append(ic['evaluations'])
         warm.append(iw['evaluations'])
+        fallbacks += int(iw.get('fallback', False))
         tc, _ = api.make_table(s, xc, hs, el, az, log=None)
         tw, _ = api.make_table(s, xw, hs, el, az, log=None)
         assert np.max(np.abs(tw[..., 0] - tc[..., 0]) / tc[..., 0]) < 1e-4
-    print(f'evaluations cold {cold} warm {warm}')
+    print(f'evaluations cold {cold} warm {warm} (seeded runs restarted cold: {fallbacks})')
     print(f'distance to the converged source: first light {np.round(d_first, 4)}, network {np.round(d_warm, 4)}')
     assert len(cold) >= 4
     assert np.mean(d_warm) < 0.5 * np.mean(d_first)
