@@ -233,13 +233,13 @@ int sos_anderson_residual(int64_t cells, int32_t rows, int32_t lanes, const floa
 
 /* Row and column `slot` of the per-lane Gram matrices gram_dev [lanes][depth][depth] (FP64):
  * gram[lane][slot][j] = gram[lane][j][slot] = sum_i f_ring[slot][lane][i] f_ring[j][lane][i] for every j with bit j
- * of filled set. f_ring: [depth][lanes][n]. work_dev: double [1024 * lanes * depth] scratch. Fixed-order sums. */
+ * of filled set. f_ring: [depth][lanes][n]. work_dev: double [128 * lanes * depth] scratch (or more). Fixed-order sums. */
 int sos_anderson_gram(int64_t n, int32_t lanes, int32_t depth, int32_t slot, uint32_t filled, const float *ring_dev,
                       double *work_dev, double *gram_dev, void *stream);
 
 /* sos_anderson_gram without the work buffer; no counterpart in the reference. For callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops): it calls no allocator. The partial sums go to
- * a file-scope device array of 1024 x 32 x 8 doubles (2 MiB per device, held until the process ends), enough for
+ * a file-scope device array of 128 x 32 x 8 doubles (256 KiB per device, held until the process ends), enough for
  * every argument the limits allow (lanes <= 32, depth <= 8); a larger need is rejected with an error. Every call on a
  * device shares that device's array, so calls that overlap on one device race. Results equal sos_anderson_gram's
  * bit for bit (the same kernels and summation order). */

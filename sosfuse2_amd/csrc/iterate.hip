@@ -478,11 +478,11 @@ int residual_launch(int64_t cells, int rows, int G, const float *g, const float 
     return check("anderson_residual");
 }
 
-constexpr int GRAM_MAX_BLOCKS = 1024;
+constexpr int GRAM_MAX_BLOCKS = 128;
 
-// chunks per lane of gram_partial: about 64 Ki residuals each, at most 128 (the grid is chunks x lanes)
+// chunks per lane of gram_partial: about 64 Ki residuals each, at most GRAM_MAX_BLOCKS (the grid is chunks x lanes)
 static int gram_blocks(int64_t n) {
-    int blocks = (int)min((int64_t)128, (n + 65535) / 65536);
+    int blocks = (int)min((int64_t)GRAM_MAX_BLOCKS, (n + 65535) / 65536);
     return blocks < 1 ? 1 : blocks;
 }
 
@@ -497,7 +497,7 @@ int gram_launch(int64_t n, int G, int depth, int slot, unsigned filled, const fl
 // Allocation-free Gram products: the per-block partial sums live in a file-scope device array instead of a caller
 // buffer. Its host-side name is only a stand-in for the symbol, so the launcher passes the device address that
 // hipGetSymbolAddress reports for the current device, looked up once per device and kept.
-constexpr int64_t GRAM_SCRATCH = (int64_t)GRAM_MAX_BLOCKS * 32 * 8;  // blocks x lanes x depth at their limits (2 MiB)
+constexpr int64_t GRAM_SCRATCH = (int64_t)GRAM_MAX_BLOCKS * 32 * 8;  // blocks x lanes x depth at their limits (256 KiB)
 static __device__ __align__(256) double scratch[GRAM_SCRATCH];
 
 static double *scratch_address() {

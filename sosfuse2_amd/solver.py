@@ -224,7 +224,7 @@ class Solver:
         fring = torch.empty((D, G, n), dtype=torch.float32, device=self.device)
         gram = torch.zeros((G, D, D), dtype=torch.float64, device=self.device)
         work = torch.empty((C * G + G,), dtype=torch.float32, device=self.device)
-        gwork = torch.empty((1024 * G * D,), dtype=torch.float64, device=self.device)
+        gwork = torch.empty((128 * G * D,), dtype=torch.float64, device=self.device)
         wts = torch.empty((G, D), dtype=torch.float32, device=self.device)
         order_dev = torch.zeros((D,), dtype=torch.int32, device=self.device)
         count = 0

@@ -24,7 +24,7 @@ def test_gram_without_allocation_matches_gram_with_work_buffer(gpu):
     for n, lanes, depth, slot, filled, seed in ((3_000_000, 32, 5, 3, 0b11111, 1), (70_001, 16, 4, 1, 0b0111, 2)):
         print(f'no-alloc gram: n={n} lanes={lanes} depth={depth} slot={slot}', flush=True)
         ring = _case(n, lanes, depth, slot, filled, seed)
-        blocks = min(1024, (n + 4095) // 4096)
+        blocks = min(128, (n + 65535) // 65536)  # the Gram kernel's chunks per lane
         work = torch.empty(blocks * lanes * depth, dtype=torch.float64, device='cuda')
         gram_a = torch.zeros((lanes, depth, depth), dtype=torch.float64, device='cuda')
         gram_b = torch.zeros_like(gram_a)
