@@ -555,7 +555,9 @@ def main():
             solver.geometry.bundle.plan = None
             solver.geometry.ground_bundle.plan = None
             # the earlier tables' query bundles (one per elevation set of the batched chunks) and their plans
-            k.__dict__.get('_observer_bundles', {}).clear()
+            from sosfuse2_amd.solver import drop_observer_bundles
+
+            drop_observer_bundles(k)
 
         out['production'] = production_variant(pscope, k, state, tol, table, release)
         out['production']['vs_headline'] = round(out['production']['tables_per_hour'] / out['value'], 4)

@@ -64,6 +64,11 @@ def observer_bundle(s, hs, el, az, kernel=None):
     return cache[key]
 
 
+def drop_observer_bundles(kernel):
+    """Release the observer bundles (and their plans) kept on a kernel; they are rebuilt on next use."""
+    kernel.__dict__.pop('_observer_bundles', None)
+
+
 def _observer_bundle(s, hs, el, az, k):
     packs = []
     sizes = []
