@@ -601,6 +601,13 @@ def test_node_geometry_consumer_matches_per_lane_path(gpu, W, L):
     k.node_geometry = True
     a = k.radiance(db, s, f, False, bnd, 4).cpu().numpy()
     fa = k.radiance(db, s, None, True, None, 4).cpu().numpy()
+    # the lane-major copy of the tensor rows (SOS_TENSOR4) against the same consumer reading the full field
+    k.tuning['tensor_rows'] = False
+    try:
+        a_full = k.radiance(db, s, f, False, bnd, 4).cpu().numpy()
+    finally:
+        k.tuning['tensor_rows'] = True
+    assert np.array_equal(a, a_full)
     k.node_geometry = False
     b = k.radiance(db, s, f, False, bnd, 4).cpu().numpy()
     fb = k.radiance(db, s, None, True, None, 4).cpu().numpy()
