@@ -322,9 +322,20 @@ __global__ __launch_bounds__(256) void transport_geom_kernel(TransportArgs a) {
         for (int q = 0; q < P; ++q) total[q] = corr[q] = 0.0f;
         float tau = 0.0f, tc = 0.0f, acc = 0.0f;
         const int64_t n0 = a.node_off[ray], n1 = a.node_off[ray + 1];
+        // the panels and cell of node n + 1 are loaded while node n is evaluated (its records and tensor reads
+        // depend on the cell)
+        float4 p0 = float4{0.f, 0.f, 0.f, 0.f}, p1 = p0, p3n = p0;
+        if (n0 < n1) {
+            const float4 *gp = a.node_geom + n0 * 8;
+            p0 = gp[0]; p1 = gp[1]; p3n = gp[3];
+        }
         for (int64_t n = n0; n < n1; ++n) {
             const float4 *g = a.node_geom + n * 8;
-            const float4 g0 = g[0], g1 = g[1], g3 = g[3];
+            const float4 g0 = p0, g1 = p1, g3 = p3n;
+            if (n + 1 < n1) {
+                const float4 *gn = g + 8;
+                p0 = gn[0]; p1 = gn[1]; p3n = gn[3];
+            }
             acc = acc + (((((g0.x * ext6[0] + g0.y * ext6[1]) + g0.z * ext6[2]) + g0.w * ext6[3]) + g1.x * ext6[4]) +
                          g1.y * ext6[5]);
             const float mid = (tau - tc) + acc;
