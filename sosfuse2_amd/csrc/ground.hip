@@ -68,17 +68,24 @@ __global__ __launch_bounds__(64 * (G / 16) * 2) void ground_records_kernel(Groun
     for (int p = tid; p < NCH * PC; p += NT)
         gd[p] = p < np ? make_float4(a.gdir[(p0 + p) * 3], a.gdir[(p0 + p) * 3 + 1], a.gdir[(p0 + p) * 3 + 2], 0.0f)
                        : make_float4(0.f, 0.f, 0.f, 0.f);
+    // the visit ids and directions of round r + 1 are loaded (threads 0 .. V - 1) while round r is evaluated
+    int nid = -1;
+    float4 ndir = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (tid < V && v0 + tid < v1) {
+        nid = a.vis[v0 + tid];
+        ndir = a.node_dir[nid >> 2];
+    }
     for (int64_t r0 = v0; r0 < v1; r0 += V) {
         const int nv = (int)min((int64_t)V, v1 - r0);
         if (tid < V) {
-            int id = -1;
-            float4 dv = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (tid < nv) {
-                id = a.vis[r0 + tid];
-                dv = a.node_dir[id >> 2];
+            sid[tid] = nid;
+            sdir[tid] = ndir;
+            nid = -1;
+            ndir = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (r0 + V + tid < v1) {
+                nid = a.vis[r0 + V + tid];
+                ndir = a.node_dir[nid >> 2];
             }
-            sid[tid] = id;
-            sdir[tid] = dv;
         }
         __syncthreads();
         float4 dv[VT];
